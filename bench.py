@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark: the 30-qubit complex128 layered h/rx/cx circuit (depth 20, seed 1002) on one MI355X.
+
+A step is one execution of the whole circuit on the HBM-resident state (the
+state is allocated and initialised before timing; successive steps keep
+evolving it, so every step does identical work). Prints one JSON line.
+
+  value      algorithmic HBM GB/s over the circuit: sum over executed sweeps of
+             f * 2 * 16 B * 2^30 (f = 1 for full sweeps, 1/2 for CX/SWAP) / step time
+  roofline   the dominant kernel's algorithmic bytes per launch / its mean HIP-event
+             duration on the state's stream, against 8 TB/s
+  cpu_baseline  the oracle (C++ restatement of CPUOpenMPEngine, Release flags) on a
+             bounded sample of the same 30q circuit, on this host's cores
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "qlassical-quantum-hpc-simulator_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "30q circuit time & HBM GB/s; served circuits/s + p99 at infidelity≤1e-6"
+PEAK_HBM = 8.0e12
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def pass_fraction(t):
+    return {0x20: 0.5, 0x21: 0.5, 0xE1: 0.5, 0x03: 0.5, 0x04: 0.5, 0x30: 0.25, 0x31: 0.25, 0xF0: 0.0}.get(int(t), 1.0)
+
+
+def algorithmic_bytes(instrs, nq, amp_bytes=16):
+    bp = 2.0 * amp_bytes * (1 << nq)
+    return sum(pass_fraction(t) for t in instrs["type"]) * bp
+
+
+def cpu_baseline(prog, nq, budget_s):
+    """Times the oracle (Release build) on the first layer (RX on every qubit + its CX
+    brick) of the same circuit: a bounded sample with the circuit's gate mix."""
+    import numpy as np
+
+    import oracle
+    layer0 = prog.instrs[nq: nq + nq + (nq // 2)]
+    # shrink the sample if one RX sweep suggests it would exceed the budget
+    t1 = oracle.time_execute(nq, layer0[:1], reps=1, lib="fast")
+    per_pass = max(t1, 1e-6)
+    n_pass_budget = max(2, int(budget_s / per_pass))
+    sample = layer0
+    if sum(pass_fraction(t) for t in sample["type"]) > n_pass_budget:
+        keep = []
+        acc = 0.0
+        for g in sample:
+            acc += pass_fraction(g["type"])
+            if acc > n_pass_budget:
+                break
+            keep.append(g)
+        sample = np.array(keep, dtype=prog.instrs.dtype)
+    t = oracle.time_execute(nq, sample, reps=1, lib="fast")
+    b = algorithmic_bytes(sample, nq)
+    rate = b / t
+    full = algorithmic_bytes(prog.instrs, nq)
+    return {
+        "value": round(rate / 1e9, 3), "unit": "GB/s", "cores": int(oracle.num_threads("fast")), "kind": "port",
+        "sample": f"{len(sample)} gates ({int((sample['type'] == 0x10).sum())} RX, "
+                  f"{int((sample['type'] == 0x20).sum())} CX) of the {nq}q circuit's first layer, "
+                  f"execute() only, {t:.2f} s; oracle/_build/liboracle_fast.so (-O3 -ffast-math -fopenmp)",
+        "circuit_s_est": round(full / rate, 1),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=20)
+    ap.add_argument("--seed", type=int, default=1002)
+    ap.add_argument("--mode", choices=["auto", "plain", "windowed"], default="auto")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this backend has no multi-GPU path "
+              "(a state shards by its top log2(G) qubits; see the design notes). Use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import qlx
+
+    nq, depth = args.qubits, args.depth
+    prog = qlx.layered(nq, depth, args.seed)
+    modes = ["windowed", "plain"] if args.mode == "auto" else [args.mode]
+    st = qlx.State(nq, "c128")
+    mode, opts = None, None
+    for m in modes:
+        o = qlx.exec_opts(m)
+        try:
+            st.execute(prog.slice(0, 1), o)
+            mode, opts = m, o
+            break
+        except qlx.QlxUnsupported:
+            continue
+    st.reset()
+    log(f"bench: {nq}q depth {depth} ({len(prog)} gates), mode {mode}, warmup {args.warmup}, steps {args.steps}")
+    for _ in range(args.warmup):
+        st.execute(prog, opts)
+    st.sync()
+    st.timing(True)
+    st.timing_reset()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st.execute(prog, opts)
+    st.sync()
+    t1 = time.perf_counter()
+    timing = st.timing_read()
+    st.timing(False)
+    stats = st.last_stats()
+    ms = (t1 - t0) / args.steps * 1e3
+    norm = st.norm_sq()
+    log(f"bench: {ms:.2f} ms/step, norm^2-1 = {norm - 1:.2e}, kernels {timing}")
+    assert abs(norm - 1.0) < 1e-9, norm
+
+    bytes_step = stats["algorithmic_bytes"]
+    gbps = bytes_step / (ms / 1e3) / 1e9
+    dom = max(timing.items(), key=lambda kv: kv[1][1])
+    name, (launches, tot_ms, tot_bytes) = dom
+    avg_ms = tot_ms / launches
+    achieved = (tot_bytes / launches) / (avg_ms / 1e3)
+    unfused_pass_eq = algorithmic_bytes(prog.instrs, nq) / (2.0 * 16 * (1 << nq))
+    out = {
+        "metric": METRIC,
+        "value": round(gbps, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"layered h/rx/cx circuit, {nq} qubits, depth {depth}, seed {args.seed}, complex128 "
+                        f"({(16 << nq) / 2**30:.0f} GiB state), {len(prog)} gates; {mode} execution",
+            "qubits": nq, "depth": depth, "gates": len(prog), "mode": mode,
+            "sweeps_per_step": {"full": stats["sweeps_full"], "half": stats["sweeps_half"],
+                                "quarter": stats["sweeps_quarter"]},
+            "circuit_ms": round(ms, 3),
+            "effective_gbps_unfused_equiv": round(unfused_pass_eq * 2 * 16 * (1 << nq) / (ms / 1e3) / 1e9, 1),
+            "hbm_frac_circuit": round(gbps * 1e9 / PEAK_HBM, 4),
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": name, "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4), "traffic": None,
+            "launches": launches, "avg_ms": round(avg_ms, 4),
+            "bytes_per_launch": tot_bytes / launches,
+        },
+    }
+    st.close()
+    if not args.no_cpu:
+        log("bench: cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(prog, nq, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

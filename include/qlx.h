@@ -1,0 +1,202 @@
+/*
+ * qlx.h — C ABI of the MI355X (gfx950) state-vector execution backend.
+ *
+ * This is the drop-in boundary under Qlassical's C++ `ExecutionEngine` /
+ * `StateVector` interfaces. The C++20 classes `HIPEngine` / `GPUStateVector`
+ * (header-only, qlassical-quantum-hpc-simulator_amd/include/backend/) call these
+ * functions; nothing here uses C++ or torch types.
+ *
+ * Conventions
+ *  - Every function returns an int status: QLX_OK (0) or a negative QLX_E* code.
+ *    qlx_last_error() returns a thread-local message for the last failure.
+ *  - Amplitudes cross the boundary as interleaved complex128 (re, im doubles),
+ *    little-endian qubit order: bit q of the basis index is qubit q
+ *    (reference: include/backend/state_vector_utils.hpp:23-34).
+ *  - Gate streams cross as the reference's 16-byte GateInstr records
+ *    (reference: include/frontend/frontend.hpp:84-105) and the UnitaryPool as
+ *    (data, metadata) arrays (reference: include/frontend/frontend.hpp:117-164).
+ *  - Work is ordered on one HIP stream per state. Functions that return data
+ *    to the host synchronise that stream; qlx_execute* only enqueue
+ *    (qlx_sync waits and reports kernel-side errors).
+ */
+#ifndef QLX_ABI_H_
+#define QLX_ABI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mapped back to C++ exceptions by HIPEngine) ---------- */
+#define QLX_OK            0
+#define QLX_EINVAL       -1  /* std::invalid_argument */
+#define QLX_ENOMEM       -2  /* std::bad_alloc */
+#define QLX_ERANGE       -3  /* std::out_of_range */
+#define QLX_EUNSUPPORTED -4  /* std::runtime_error (unknown opcode etc.) */
+#define QLX_EHIP         -5  /* std::runtime_error (HIP runtime failure) */
+
+/* ---- amplitude precision of a device state ----------------------------- */
+#define QLX_C128 0   /* complex128: the reference's std::complex<double> */
+#define QLX_C64  1   /* complex64 working state (precision/batch/serve paths) */
+
+/* ---- IR records, byte-identical to the reference ------------------------ */
+/* GateInstr: reference include/frontend/frontend.hpp:84-105 (16 bytes).
+ * Extension: when arity()==5 (FUSED_BLOCK only) the fifth qubit is stored in
+ * the low byte of `pad` (the reference's `_pad`, bytes 6-7). */
+typedef struct qlx_gate_instr {
+    uint8_t  type;
+    uint8_t  flags_arity;   /* low nibble = arity, high nibble = flags */
+    uint8_t  qubits[4];
+    uint16_t pad;
+    union { float param; uint32_t matrix_idx; } payload;
+    uint32_t uid;
+} qlx_gate_instr;
+
+/* UnitaryPool::MatrixEntry: reference include/frontend/frontend.hpp:118-123 (12 bytes). */
+typedef struct qlx_matrix_entry {
+    uint32_t offset;        /* start index (in complex elements) into pool data */
+    uint32_t num_elements;  /* 4^k */
+    uint8_t  num_qubits;    /* k */
+    uint8_t  pad[3];
+} qlx_matrix_entry;
+
+/* GateType values: reference include/frontend/frontend.hpp:26-51. */
+enum {
+    QLX_H = 0x00, QLX_X = 0x01, QLX_Y = 0x02, QLX_Z = 0x03, QLX_S = 0x04,
+    QLX_RX = 0x10, QLX_RY = 0x11, QLX_RZ = 0x12,
+    QLX_CX = 0x20, QLX_SWAP = 0x21,
+    QLX_CCX = 0x30, QLX_CSWAP = 0x31,
+    QLX_UNITARY = 0x80, QLX_FUSED_BLOCK = 0xE0, QLX_GLOBAL_SWAP = 0xE1,
+    QLX_BARRIER = 0xF0, QLX_MEASURE = 0xF1
+};
+
+/* ---- execution options --------------------------------------------------- */
+#define QLX_EXEC_PLAIN    0  /* one HBM sweep per instruction (pair/quad/dense kernels) */
+#define QLX_EXEC_WINDOWED 1  /* gates grouped into LDS/register window sweeps */
+
+/* GLOBAL_SWAP semantics. The reference engine treats GLOBAL_SWAP as a no-op
+ * (include/backend/cpu_openmp_engine.hpp:217-219); the GPU backend applies it
+ * as a real qubit permutation by default. */
+#define QLX_GSWAP_PERMUTE 0
+#define QLX_GSWAP_NOOP    1
+
+typedef struct qlx_exec_opts {
+    uint32_t mode;            /* QLX_EXEC_PLAIN | QLX_EXEC_WINDOWED */
+    uint32_t window_qubits;   /* 0 = default for the precision (c128: 12, c64: 13) */
+    uint32_t coalesce_qubits; /* low qubits always kept in the window; 0 = default (3) */
+    uint32_t gswap_mode;      /* QLX_GSWAP_PERMUTE | QLX_GSWAP_NOOP */
+    uint32_t reserved[4];
+} qlx_exec_opts;
+
+/* Statistics of the last qlx_execute on a state (sweep accounting). */
+typedef struct qlx_exec_stats {
+    uint64_t sweeps_full;     /* kernels touching the whole state (f = 1) */
+    uint64_t sweeps_half;     /* kernels touching half (CX/SWAP/Z/S, f = 1/2) */
+    uint64_t sweeps_quarter;  /* kernels touching a quarter (CCX/CSWAP, f = 1/4) */
+    uint64_t gates_applied;   /* source instructions executed (excl. BARRIER) */
+    double   algorithmic_bytes; /* sum of f * 2 * sizeof(amp) * 2^N */
+} qlx_exec_stats;
+
+/* Device description backing the GPU HardwareTopology
+ * (replaces the hwloc NUMA query of reference include/middleware/hw_topology.hpp:60-102). */
+typedef struct qlx_device_info {
+    uint64_t hbm_bytes;        /* total device memory */
+    uint64_t hbm_free_bytes;   /* free device memory now */
+    uint32_t lds_bytes_per_cu; /* LDS per compute unit (workgroup max) */
+    uint32_t num_cus;
+    uint32_t wavefront_size;
+    uint32_t clock_khz;
+    char     arch[64];         /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+} qlx_device_info;
+
+typedef struct qlx_state qlx_state;
+
+/* ---- library / device ---------------------------------------------------- */
+const char* qlx_last_error(void);
+/* Version string of this library. */
+const char* qlx_version(void);
+/* Replaces CPUHwlocTopology (reference include/middleware/hw_topology.hpp:60-112). */
+int qlx_device_query(int device, qlx_device_info* out);
+
+/* ---- StateVector (reference include/backend/state_vector.hpp:17-47) ------ */
+/* initialize(num_qubits): HBM allocation + |0...0>.
+ * Reference: include/backend/cpu_state_vector.hpp:54-78. */
+int qlx_state_create(uint32_t num_qubits, int dtype, qlx_state** out);
+/* initialize(span): power-of-two dimension, sum |c|^2 within 1e-6 of 1, copy.
+ * Reference: include/backend/cpu_state_vector.hpp:82-115. */
+int qlx_state_create_from_host(const double* amps_c128, size_t dim, int dtype, qlx_state** out);
+int qlx_state_destroy(qlx_state* st);
+/* ExecutionEngine::reset() -> |0...0> (reference include/backend/cpu_openmp_engine.hpp:119-122). */
+int qlx_state_reset(qlx_state* st);
+/* ExecutionEngine::reset(span) (reference include/backend/cpu_openmp_engine.hpp:125-128);
+ * the dimension may differ from the current one (the state is re-allocated). */
+int qlx_state_load(qlx_state* st, const double* amps_c128, size_t dim);
+/* Backs ExecutionEngine::state() (reference include/backend/execution_engine.hpp:66-67):
+ * synchronises and copies the whole state to host as complex128 (c64 is widened). */
+int qlx_state_to_host(qlx_state* st, double* out_c128, size_t dim);
+int qlx_state_num_qubits(const qlx_state* st, uint32_t* out);
+int qlx_state_dtype(const qlx_state* st, int* out);
+/* Device pointer of the amplitude array (for zero-copy hand-off between engines). */
+int qlx_state_device_ptr(const qlx_state* st, void** out);
+/* FP64 sum |c|^2 on the GPU (reference reduction: cpu_state_vector.hpp:97-100). */
+int qlx_state_norm_sq(qlx_state* st, double* out);
+/* FP64 <a|b> on the GPU; a and b may differ in precision (c64 vs c128). */
+int qlx_state_inner(qlx_state* a, qlx_state* b, double* re, double* im);
+/* FP64 max |a_i - b_i| on the GPU. */
+int qlx_state_max_abs_diff(qlx_state* a, qlx_state* b, double* out);
+/* Copy b into a (same qubit count; precisions may differ). */
+int qlx_state_copy(qlx_state* dst, qlx_state* src);
+
+/* ---- ExecutionEngine::execute (reference include/backend/execution_engine.hpp:55-56,
+ *      implemented by include/backend/cpu_openmp_engine.hpp:143-225) ---------
+ * Enqueues instructions [0, n_instr) up to (not including) the first MEASURE;
+ * *stop_at receives its index (or n_instr). The host then draws the outcome with
+ * its own mt19937_64 (reference :632-633) via qlx_measure_prob1/qlx_collapse and
+ * resumes after it. Unknown opcodes -> QLX_EUNSUPPORTED before anything runs
+ * (reference :221-222). opts may be NULL (plain mode, real GLOBAL_SWAP). */
+int qlx_execute(qlx_state* st,
+                const qlx_gate_instr* instrs, size_t n_instr,
+                const double* pool_c128, size_t pool_elems,
+                const qlx_matrix_entry* meta, size_t n_meta,
+                const qlx_exec_opts* opts, size_t* stop_at);
+/* Sweep accounting of the last qlx_execute on this state. */
+int qlx_last_exec_stats(const qlx_state* st, qlx_exec_stats* out);
+
+/* ---- MEASURE (reference include/backend/cpu_openmp_engine.hpp:617-671) ---- */
+/* P(qubit q = 1), FP64 reduction on the GPU (reference :621-630). Synchronous. */
+int qlx_measure_prob1(qlx_state* st, uint32_t q, double* p1);
+/* Zero the other half and scale the kept half by inv_norm (reference :649-670). */
+int qlx_collapse(qlx_state* st, uint32_t q, int outcome, double inv_norm);
+
+/* Wait for all work on the state's stream; reports asynchronous kernel errors. */
+int qlx_sync(qlx_state* st);
+
+/* ---- kernel timing (no reference counterpart: the reference times execute() with
+ *      Google Benchmark wall clock, benchmarks/bench_hw_awareness.cpp:102-107) ---
+ * When enabled, every kernel the state launches is bracketed by a HIP event pair on
+ * the state's stream; qlx_timing_read synchronises and returns per-kernel totals
+ * accumulated since the last qlx_timing_reset. */
+typedef struct qlx_kernel_timing {
+    char     kernel[48];
+    uint64_t launches;
+    double   total_ms;
+    double   algorithmic_bytes;  /* sum over launches of f * 2 * sizeof(amp) * 2^N */
+} qlx_kernel_timing;
+int qlx_timing_enable(qlx_state* st, int on);
+int qlx_timing_reset(qlx_state* st);
+int qlx_timing_read(qlx_state* st, qlx_kernel_timing* out, size_t cap, size_t* n_out);
+
+/* ---- host-side helpers (CPU; no GPU needed) ------------------------------ */
+/* Seeded layered h/rx/cx circuit: H on every qubit, then `depth` layers of
+ * RX(theta) on every qubit (theta = (mt19937_64() >> 11) * 2^-53 * 2*pi, stored
+ * as float, layer-major) followed by CX(q, q+1) for q = l%2, l%2+2, ... < n-1.
+ * Writes at most cap instructions; *n_out = required count. */
+int qlx_gen_layered(uint32_t num_qubits, uint32_t depth, uint64_t seed,
+                    qlx_gate_instr* out, size_t cap, size_t* n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QLX_ABI_H_ */

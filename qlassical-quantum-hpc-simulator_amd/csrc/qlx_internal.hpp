@@ -1,0 +1,151 @@
+// qlx_internal.hpp — shared definitions of the gfx950 backend (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qlx.h"
+
+namespace qlx {
+
+// ---- error reporting ---------------------------------------------------------
+void set_error(const std::string& msg);
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+// Throw inside the library; converted to a status code at the ABI edge.
+[[noreturn]] void fail(int code, const std::string& msg);
+
+#define QLX_HIP_CHECK(expr)                                                        \
+    do {                                                                           \
+        hipError_t qlx_e_ = (expr);                                                \
+        if (qlx_e_ != hipSuccess)                                                  \
+            ::qlx::fail(QLX_EHIP, std::string(#expr ": ") + hipGetErrorString(qlx_e_)); \
+    } while (0)
+
+// Wraps an ABI entry point body: maps exceptions to status codes.
+template <class F>
+int guard(F&& f) noexcept {
+    try {
+        f();
+        return QLX_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return QLX_ENOMEM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return QLX_EHIP;
+    } catch (...) {
+        set_error("unknown error");
+        return QLX_EHIP;
+    }
+}
+
+// ---- device buffer that grows on demand ---------------------------------------
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    void reserve(size_t bytes);
+    void release();
+};
+
+}  // namespace qlx
+
+// ---- kernel timing records --------------------------------------------------------
+namespace qlx {
+struct TimingRec {
+    const char* kernel;
+    hipEvent_t e0, e1;
+    double bytes;
+};
+struct TimingAgg {
+    uint64_t launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+};
+}  // namespace qlx
+
+// ---- the opaque state handle -----------------------------------------------------
+struct qlx_state {
+    bool timing = false;
+    std::vector<qlx::TimingRec> pending;      // event pairs not yet read back
+    std::vector<hipEvent_t> free_events;      // recycled events
+    std::vector<std::pair<std::string, qlx::TimingAgg>> timing_agg;
+    uint32_t nq = 0;
+    int dtype = QLX_C128;
+    int device = 0;
+    void* amp = nullptr;          // 2^nq amplitudes (double2 or float2), HBM resident
+    size_t bytes = 0;
+    hipStream_t stream = nullptr;
+    qlx::DevBuf scratch;          // reduction partials
+    qlx::DevBuf prog;             // uploaded gate descriptors / matrices
+    double* host_pinned = nullptr;  // pinned staging for reductions (small)
+    qlx_exec_stats stats{};
+};
+
+namespace qlx {
+
+inline size_t amp_size(int dtype) { return dtype == QLX_C128 ? 16 : 8; }
+
+// Brackets one kernel launch with a HIP event pair when timing is enabled.
+class TimedLaunch {
+public:
+    TimedLaunch(qlx_state* st, const char* kernel, double bytes);
+    ~TimedLaunch() noexcept(false);
+    TimedLaunch(const TimedLaunch&) = delete;
+    TimedLaunch& operator=(const TimedLaunch&) = delete;
+
+private:
+    qlx_state* st_;
+    TimingRec rec_{};
+};
+
+// Algorithmic bytes of one full-state HBM pass: read + write of every amplitude.
+inline double pass_bytes(const qlx_state* st) {
+    return 2.0 * (double)amp_size(st->dtype) * (double)(1ull << st->nq);
+}
+
+// Kernel launch helpers (defined in qlx_state.hip).
+void launch_fill_zero_state(qlx_state* st);
+double reduce_norm_sq(qlx_state* st);
+
+// Plain (one sweep per instruction) executor, qlx_gates.hip.
+void execute_plain(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
+                   const double* pool, size_t pool_elems,
+                   const qlx_matrix_entry* meta, size_t n_meta,
+                   const qlx_exec_opts& opts, qlx_exec_stats& stats);
+
+// Windowed executor, qlx_sweep.hip.
+void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
+                      const double* pool, size_t pool_elems,
+                      const qlx_matrix_entry* meta, size_t n_meta,
+                      const qlx_exec_opts& opts, qlx_exec_stats& stats);
+
+// Dense k-qubit unitary on the device state (k <= 5), matrix already on device
+// as row-major complex128. Used by both executors.
+void launch_dense(qlx_state* st, const double* d_matrix, const uint8_t* qubits, int k);
+
+// Instruction helpers shared by the executors.
+inline int instr_arity(const qlx_gate_instr& g) {
+    int a = g.flags_arity & 0x0F;
+    return a;
+}
+inline int instr_qubit(const qlx_gate_instr& g, int i) {
+    return i < 4 ? g.qubits[i] : (g.pad & 0xFF);
+}
+
+// Validates qubit operands of a standard/matrix instruction; throws QLX_ERANGE/EINVAL.
+void validate_instr(const qlx_gate_instr& g, uint32_t nq, const qlx_matrix_entry* meta,
+                    size_t n_meta, size_t pool_elems);
+
+}  // namespace qlx

@@ -1,0 +1,18 @@
+"""qlx — Python binding of the gfx950 Qlassical execution backend (libqlx.so).
+
+The product is the HIP library and its C ABI (include/qlx.h) plus the C++20
+HIPEngine/GPUStateVector headers; this package is the ctypes plumbing used by
+the tests, the benchmark and the serving loop.
+"""
+from . import _abi
+from ._abi import (LIB_PATH, QlxError, QlxInvalidArgument, QlxOutOfMemory, QlxOutOfRange, QlxUnsupported,
+                   lib)
+from .engine import Engine, State, device_info, exec_opts
+from .ir import (GATE_DTYPE, META_DTYPE, MT19937_64, Circuit, Program, bits_f32, f32_bits, inverse_layered,
+                 layered)
+
+__all__ = [
+    "_abi", "LIB_PATH", "QlxError", "QlxInvalidArgument", "QlxOutOfMemory", "QlxOutOfRange", "QlxUnsupported",
+    "lib", "Engine", "State", "device_info", "exec_opts", "GATE_DTYPE", "META_DTYPE", "MT19937_64", "Circuit",
+    "Program", "bits_f32", "f32_bits", "inverse_layered", "layered",
+]

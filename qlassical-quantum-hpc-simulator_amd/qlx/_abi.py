@@ -1,0 +1,157 @@
+"""ctypes binding of libqlx.so (the C ABI declared in include/qlx.h).
+
+The library is loaded from ``qlx/_lib/libqlx.so`` next to this file. There is no
+fallback: if the HIP library is missing, importing the binding raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libqlx.so")
+
+QLX_OK = 0
+QLX_EINVAL = -1
+QLX_ENOMEM = -2
+QLX_ERANGE = -3
+QLX_EUNSUPPORTED = -4
+QLX_EHIP = -5
+
+QLX_C128 = 0
+QLX_C64 = 1
+
+QLX_EXEC_PLAIN = 0
+QLX_EXEC_WINDOWED = 1
+QLX_GSWAP_PERMUTE = 0
+QLX_GSWAP_NOOP = 1
+
+
+class QlxError(RuntimeError):
+    """Base class of errors reported by the C ABI."""
+
+    code = QLX_EHIP
+
+
+class QlxInvalidArgument(QlxError, ValueError):
+    code = QLX_EINVAL
+
+
+class QlxOutOfMemory(QlxError, MemoryError):
+    code = QLX_ENOMEM
+
+
+class QlxOutOfRange(QlxError, IndexError):
+    code = QLX_ERANGE
+
+
+class QlxUnsupported(QlxError):
+    code = QLX_EUNSUPPORTED
+
+
+_ERR = {
+    QLX_EINVAL: QlxInvalidArgument,
+    QLX_ENOMEM: QlxOutOfMemory,
+    QLX_ERANGE: QlxOutOfRange,
+    QLX_EUNSUPPORTED: QlxUnsupported,
+    QLX_EHIP: QlxError,
+}
+
+
+class ExecOpts(C.Structure):
+    _fields_ = [
+        ("mode", C.c_uint32),
+        ("window_qubits", C.c_uint32),
+        ("coalesce_qubits", C.c_uint32),
+        ("gswap_mode", C.c_uint32),
+        ("reserved", C.c_uint32 * 4),
+    ]
+
+
+class ExecStats(C.Structure):
+    _fields_ = [
+        ("sweeps_full", C.c_uint64),
+        ("sweeps_half", C.c_uint64),
+        ("sweeps_quarter", C.c_uint64),
+        ("gates_applied", C.c_uint64),
+        ("algorithmic_bytes", C.c_double),
+    ]
+
+
+class KernelTiming(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_char * 48),
+        ("launches", C.c_uint64),
+        ("total_ms", C.c_double),
+        ("algorithmic_bytes", C.c_double),
+    ]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [
+        ("hbm_bytes", C.c_uint64),
+        ("hbm_free_bytes", C.c_uint64),
+        ("lds_bytes_per_cu", C.c_uint32),
+        ("num_cus", C.c_uint32),
+        ("wavefront_size", C.c_uint32),
+        ("clock_khz", C.c_uint32),
+        ("arch", C.c_char * 64),
+    ]
+
+
+# (name, argtypes) of every function include/qlx.h declares, in header order.
+SIGNATURES = [
+    ("qlx_last_error", []),
+    ("qlx_version", []),
+    ("qlx_device_query", [C.c_int, C.POINTER(DeviceInfo)]),
+    ("qlx_state_create", [C.c_uint32, C.c_int, C.POINTER(C.c_void_p)]),
+    ("qlx_state_create_from_host", [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]),
+    ("qlx_state_destroy", [C.c_void_p]),
+    ("qlx_state_reset", [C.c_void_p]),
+    ("qlx_state_load", [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("qlx_state_to_host", [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("qlx_state_num_qubits", [C.c_void_p, C.POINTER(C.c_uint32)]),
+    ("qlx_state_dtype", [C.c_void_p, C.POINTER(C.c_int)]),
+    ("qlx_state_device_ptr", [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("qlx_state_norm_sq", [C.c_void_p, C.POINTER(C.c_double)]),
+    ("qlx_state_inner", [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("qlx_state_max_abs_diff", [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
+    ("qlx_state_copy", [C.c_void_p, C.c_void_p]),
+    ("qlx_execute", [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
+                     C.POINTER(ExecOpts), C.POINTER(C.c_size_t)]),
+    ("qlx_last_exec_stats", [C.c_void_p, C.POINTER(ExecStats)]),
+    ("qlx_measure_prob1", [C.c_void_p, C.c_uint32, C.POINTER(C.c_double)]),
+    ("qlx_collapse", [C.c_void_p, C.c_uint32, C.c_int, C.c_double]),
+    ("qlx_sync", [C.c_void_p]),
+    ("qlx_timing_enable", [C.c_void_p, C.c_int]),
+    ("qlx_timing_reset", [C.c_void_p]),
+    ("qlx_timing_read", [C.c_void_p, C.POINTER(KernelTiming), C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("qlx_gen_layered", [C.c_uint32, C.c_uint32, C.c_uint64, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+]
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libqlx.so once (RTLD_GLOBAL so the C++ test binaries share it)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"libqlx.so is not built at {LIB_PATH}; build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc, gfx950)")
+    h = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+    for name, args in SIGNATURES:
+        fn = getattr(h, name)
+        fn.argtypes = args
+        fn.restype = C.c_char_p if name in ("qlx_last_error", "qlx_version") else C.c_int
+    _lib = h
+    return h
+
+
+def check(rc: int) -> None:
+    if rc == QLX_OK:
+        return
+    msg = lib().qlx_last_error().decode(errors="replace")
+    raise _ERR.get(rc, QlxError)(f"[{rc}] {msg}")

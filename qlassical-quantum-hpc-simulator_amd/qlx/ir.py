@@ -1,0 +1,262 @@
+"""The Qlassical IR as numpy records, and a builder mirroring ``QuantumCircuit``.
+
+Layouts are byte-identical to the reference:
+  GateInstr    16 bytes   include/frontend/frontend.hpp:84-105
+  MatrixEntry  12 bytes   include/frontend/frontend.hpp:118-123
+  UnitaryPool  complex128 row-major matrices, 4^k entries each (frontend.hpp:117-164)
+
+``Circuit`` follows ``QuantumCircuit`` (frontend.hpp:218-497): same gate methods,
+argument order (qubit first, then angle), float narrowing of angles, validation
+errors (IndexError for std::out_of_range, ValueError for std::invalid_argument)
+and a running uid.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+
+GATE_DTYPE = np.dtype([
+    ("type", "u1"), ("flags_arity", "u1"), ("qubits", "u1", (4,)), ("pad", "<u2"),
+    ("payload", "<u4"), ("uid", "<u4"),
+])
+META_DTYPE = np.dtype([
+    ("offset", "<u4"), ("num_elements", "<u4"), ("num_qubits", "u1"), ("pad", "u1", (3,)),
+])
+assert GATE_DTYPE.itemsize == 16 and META_DTYPE.itemsize == 12
+
+# GateType (frontend.hpp:26-51)
+H, X, Y, Z, S = 0x00, 0x01, 0x02, 0x03, 0x04
+RX, RY, RZ = 0x10, 0x11, 0x12
+CX, SWAP = 0x20, 0x21
+CCX, CSWAP = 0x30, 0x31
+UNITARY, FUSED_BLOCK, GLOBAL_SWAP = 0x80, 0xE0, 0xE1
+BARRIER, MEASURE = 0xF0, 0xF1
+NO_MATRIX = 0xFFFFFFFF
+
+GATE_NAMES = {H: "H", X: "X", Y: "Y", Z: "Z", S: "S", RX: "RX", RY: "RY", RZ: "RZ", CX: "CX",
+              SWAP: "SWAP", CCX: "CCX", CSWAP: "CSWAP", UNITARY: "UNITARY", FUSED_BLOCK: "FUSED_BLOCK",
+              GLOBAL_SWAP: "GLOBAL_SWAP", BARRIER: "BARRIER", MEASURE: "MEASURE"}
+
+
+def f32_bits(x: float) -> int:
+    return int(np.array([x], dtype=np.float32).view(np.uint32)[0])
+
+
+def bits_f32(u: int) -> float:
+    return float(np.array([u], dtype=np.uint32).view(np.float32)[0])
+
+
+def arity(g) -> int:
+    return int(g["flags_arity"]) & 0x0F
+
+
+@dataclass
+class Program:
+    """An IRModule: gate stream + unitary pool."""
+    num_qubits: int
+    instrs: np.ndarray = field(default_factory=lambda: np.zeros(0, GATE_DTYPE))
+    pool: np.ndarray = field(default_factory=lambda: np.zeros(0, np.complex128))
+    meta: np.ndarray = field(default_factory=lambda: np.zeros(0, META_DTYPE))
+
+    def __len__(self) -> int:
+        return len(self.instrs)
+
+    def store(self, matrix: np.ndarray, k: int) -> int:
+        """UnitaryPool::store (frontend.hpp:132-150): returns the matrix index."""
+        m = np.ascontiguousarray(matrix, dtype=np.complex128).reshape(-1)
+        expected = 1 << (2 * k)
+        if m.size != expected:
+            raise ValueError(f"UnitaryPool::store: matrix size mismatch. Expected {expected}, got {m.size}")
+        idx = len(self.meta)
+        e = np.zeros(1, META_DTYPE)
+        e["offset"] = len(self.pool)
+        e["num_elements"] = expected
+        e["num_qubits"] = k
+        self.pool = np.concatenate([self.pool, m])
+        self.meta = np.concatenate([self.meta, e])
+        return idx
+
+    def slice(self, lo: int, hi: int) -> "Program":
+        return Program(self.num_qubits, self.instrs[lo:hi].copy(), self.pool, self.meta)
+
+    def describe(self) -> list:
+        out = []
+        for g in self.instrs:
+            a = arity(g)
+            qs = [int(q) for q in g["qubits"][:min(a, 4)]]
+            if a == 5:
+                qs.append(int(g["pad"]) & 0xFF)
+            out.append((GATE_NAMES.get(int(g["type"]), hex(int(g["type"]))), qs))
+        return out
+
+
+class Circuit:
+    """Builder mirroring qlassical::QuantumCircuit (frontend.hpp:218-497)."""
+
+    def __init__(self, num_qubits: int):
+        self.num_qubits = int(num_qubits)
+        self._rows: list = []
+        self._prog = Program(self.num_qubits)
+        self._uid = 0
+
+    # -- validation (frontend.hpp:410-416, 446-477)
+    def _validate(self, q: int) -> None:
+        if q < 0 or q >= 256 or q >= self.num_qubits:
+            raise IndexError(f"Qubit index {q} out of range [0, min(256, {self.num_qubits}))")
+
+    def _push(self, type_: int, qubits, payload: int = 0) -> "Circuit":
+        g = np.zeros(1, GATE_DTYPE)
+        g["type"] = type_
+        g["flags_arity"] = len(qubits) & 0x0F
+        q4 = [255, 255, 255, 255]
+        for i, q in enumerate(qubits[:4]):
+            q4[i] = q
+        g["qubits"] = q4
+        if len(qubits) == 5:
+            g["pad"] = qubits[4]
+        g["payload"] = payload
+        g["uid"] = self._uid
+        self._uid += 1
+        self._rows.append(g)
+        return self
+
+    def _1q(self, t, q):
+        self._validate(q)
+        return self._push(t, [q])
+
+    def _1qp(self, t, q, theta):
+        self._validate(q)
+        return self._push(t, [q], f32_bits(float(np.float32(theta))))
+
+    def _2q(self, t, q0, q1):
+        self._validate(q0)
+        self._validate(q1)
+        if q0 == q1:
+            raise ValueError(f"Two-qubit gate requires distinct qubits, got {q0} == {q1}")
+        return self._push(t, [q0, q1])
+
+    def _3q(self, t, q0, q1, q2):
+        for q in (q0, q1, q2):
+            self._validate(q)
+        if q0 == q1 or q0 == q2 or q1 == q2:
+            raise ValueError("Three-qubit gate requires distinct qubits")
+        return self._push(t, [q0, q1, q2])
+
+    def h(self, q): return self._1q(H, q)
+    def x(self, q): return self._1q(X, q)
+    def y(self, q): return self._1q(Y, q)
+    def z(self, q): return self._1q(Z, q)
+    def s(self, q): return self._1q(S, q)
+    def rx(self, q, theta): return self._1qp(RX, q, theta)
+    def ry(self, q, theta): return self._1qp(RY, q, theta)
+    def rz(self, q, theta): return self._1qp(RZ, q, theta)
+    def cx(self, c, t): return self._2q(CX, c, t)
+    cnot = cx
+    def swap(self, a, b): return self._2q(SWAP, a, b)
+    def ccx(self, c0, c1, t): return self._3q(CCX, c0, c1, t)
+    toffoli = ccx
+    def cswap(self, c, a, b): return self._3q(CSWAP, c, a, b)
+    fredkin = cswap
+
+    def unitary(self, qubits, matrix) -> "Circuit":
+        """QuantumCircuit::unitary (frontend.hpp:340-363): arity 1..4 in the reference;
+        arity 5 is accepted here (the fifth qubit is carried in the pad byte)."""
+        k = len(qubits)
+        if k == 0 or k > 5:
+            raise ValueError(f"unitary: arity must be 1-5, got {k}")
+        midx = self._prog.store(matrix, k)
+        for q in qubits:
+            self._validate(q)
+        if len(set(qubits)) != k:
+            raise ValueError("unitary requires distinct qubits")
+        return self._push(UNITARY, list(qubits), midx)
+
+    def barrier(self, qubits=()) -> "Circuit":
+        qs = list(qubits)[:4]
+        for q in qs:
+            self._validate(q)
+        return self._push(BARRIER, qs)
+
+    def measure(self, q) -> "Circuit":
+        if isinstance(q, (list, tuple)):
+            for x in q:
+                self.measure(x)
+            return self
+        return self._1q(MEASURE, q)
+
+    def release(self) -> Program:
+        p = self._prog
+        p.instrs = np.concatenate(self._rows) if self._rows else np.zeros(0, GATE_DTYPE)
+        self._rows = []
+        self._prog = Program(self.num_qubits)
+        return p
+
+
+def layered(num_qubits: int, depth: int, seed: int) -> Program:
+    """Seeded layered h/rx/cx circuit, generated by the library (qlx_gen_layered)."""
+    L = _abi.lib()
+    n = C.c_size_t(0)
+    _abi.check(L.qlx_gen_layered(num_qubits, depth, seed, None, 0, C.byref(n)))
+    arr = np.zeros(n.value, GATE_DTYPE)
+    _abi.check(L.qlx_gen_layered(num_qubits, depth, seed, arr.ctypes.data, n.value, C.byref(n)))
+    return Program(num_qubits, arr)
+
+
+def inverse_layered(prog: Program) -> Program:
+    """The inverse of an H/RX/CX program: reversed order, RX(theta) -> RX(-theta)."""
+    inv = prog.instrs[::-1].copy()
+    for g in inv:
+        if g["type"] == RX:
+            g["payload"] = f32_bits(-bits_f32(int(g["payload"])))
+        elif g["type"] not in (H, CX, X, Z, SWAP):
+            raise ValueError("inverse_layered only handles H/RX/CX/X/Z/SWAP")
+    return Program(prog.num_qubits, inv, prog.pool, prog.meta)
+
+
+class MT19937_64:
+    """std::mt19937_64 (64-bit Mersenne Twister, the C++ standard's parameters)."""
+
+    _M = 0xFFFFFFFFFFFFFFFF
+
+    def __init__(self, seed: int = 5489):
+        self.seed(seed)
+
+    def seed(self, seed: int) -> None:
+        mt = [0] * 312
+        mt[0] = seed & self._M
+        for i in range(1, 312):
+            mt[i] = (6364136223846793005 * (mt[i - 1] ^ (mt[i - 1] >> 62)) + i) & self._M
+        self._mt, self._i = mt, 312
+
+    def _twist(self) -> None:
+        mt = self._mt
+        for i in range(312):
+            x = (mt[i] & 0xFFFFFFFF80000000) | (mt[(i + 1) % 312] & 0x7FFFFFFF)
+            xa = x >> 1
+            if x & 1:
+                xa ^= 0xB5026F5AA96619E9
+            mt[i] = mt[(i + 156) % 312] ^ xa
+        self._i = 0
+
+    def __call__(self) -> int:
+        if self._i >= 312:
+            self._twist()
+        y = self._mt[self._i]
+        self._i += 1
+        y ^= (y >> 29) & 0x5555555555555555
+        y ^= (y << 17) & 0x71D67FFFEDA60000
+        y ^= (y << 37) & 0xFFF7EEE000000000
+        y ^= y >> 43
+        return y & self._M
+
+    def uniform01(self) -> float:
+        """std::uniform_real_distribution<double>(0, 1) over this engine, as libstdc++
+        evaluates it: one 64-bit draw, double(x) / 2^64, clamped below 1."""
+        r = float(self()) / 18446744073709551616.0
+        if r >= 1.0:
+            r = float(np.nextafter(1.0, 0.0))
+        return r
