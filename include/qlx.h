@@ -161,6 +161,25 @@ int qlx_execute(qlx_state* st,
                 const double* pool_c128, size_t pool_elems,
                 const qlx_matrix_entry* meta, size_t n_meta,
                 const qlx_exec_opts* opts, size_t* stop_at);
+/* Host-only planning of QLX_EXEC_WINDOWED execution (no GPU work): the order in which
+ * the instructions of a MEASURE-free stream would run. order[k] is the source index of
+ * the k-th executed instruction and sweep_of[k] the window sweep that runs it (-1: its
+ * own kernel). Gates are only reordered past gates on disjoint qubits. */
+typedef struct qlx_plan_info {
+    uint64_t sweeps;           /* window-sweep launches (one full HBM pass each) */
+    uint64_t phases;           /* register phases over all sweeps (LDS re-layouts = phases - sweeps) */
+    uint64_t plain_steps;      /* instructions run by their own kernel */
+    uint64_t gates_in_sweeps;
+    uint32_t window_qubits;    /* W: qubits per chunk held by one work-group */
+    uint32_t register_qubits;  /* qubits held in each work-item's registers */
+    uint32_t threads_per_group;
+    uint32_t coalesce_qubits;  /* low qubits always in the window */
+} qlx_plan_info;
+int qlx_plan_windowed(uint32_t num_qubits, int dtype, const qlx_gate_instr* instrs, size_t n,
+                      const double* pool_c128, size_t pool_elems,
+                      const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts* opts,
+                      int64_t* order, int32_t* sweep_of, size_t cap, size_t* n_out, qlx_plan_info* info);
+
 /* Sweep accounting of the last qlx_execute on this state. */
 int qlx_last_exec_stats(const qlx_state* st, qlx_exec_stats* out);
 

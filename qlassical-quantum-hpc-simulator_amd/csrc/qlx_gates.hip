@@ -403,92 +403,98 @@ static int kind_for(uint8_t type) {
     }
 }
 
-void execute_plain(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const double* pool,
-                   size_t pool_elems, const qlx_matrix_entry* meta, size_t n_meta,
-                   const qlx_exec_opts& opts, qlx_exec_stats& stats) {
-    bool need_pool = false;
-    for (size_t i = 0; i < n; ++i)
-        if (instrs[i].type == QLX_UNITARY || instrs[i].type == QLX_FUSED_BLOCK) need_pool = true;
-    const double* dpool = nullptr;
-    if (need_pool) {
-        st->prog.reserve(pool_elems * 16);
-        QLX_HIP_CHECK(hipMemcpyAsync(st->prog.ptr, pool, pool_elems * 16, hipMemcpyHostToDevice, st->stream));
-        dpool = (const double*)st->prog.ptr;
-    }
+const double* upload_pool(qlx_state* st, const double* pool, size_t pool_elems) {
+    if (!pool_elems) return nullptr;
+    st->prog.reserve(pool_elems * 16, st->stream);
+    QLX_HIP_CHECK(hipMemcpyAsync(st->prog.ptr, pool, pool_elems * 16, hipMemcpyHostToDevice, st->stream));
+    // the source is pageable caller memory: complete the copy before returning
+    QLX_HIP_CHECK(hipStreamSynchronize(st->stream));
+    return (const double*)st->prog.ptr;
+}
+
+void launch_plain_instr(qlx_state* st, const qlx_gate_instr& g, const double* dpool, const qlx_matrix_entry* meta,
+                        const qlx_exec_opts& opts, qlx_exec_stats& stats) {
     const double pass = pass_bytes(st);
     auto account = [&](double f) {
         if (f >= 1.0) stats.sweeps_full++;
         else if (f >= 0.5) stats.sweeps_half++;
         else stats.sweeps_quarter++;
         stats.algorithmic_bytes += f * pass;
-    };
-    for (size_t i = 0; i < n; ++i) {
-        const qlx_gate_instr& g = instrs[i];
-        const uint8_t* q = g.qubits;
-        switch (g.type) {
-            case QLX_H: case QLX_X: case QLX_Y: case QLX_RX: case QLX_RY: case QLX_RZ: {
-                TimedLaunch tl(st, "k_1q", pass);
-                launch_1q(st, kind_for(g.type), q[0], coef_for(g));
-                account(1.0);
-                break;
-            }
-            case QLX_Z: case QLX_S: {
-                TimedLaunch tl(st, "k_1q_phase", 0.5 * pass);
-                launch_1q(st, kind_for(g.type), q[0], coef_for(g));
-                account(0.5);
-                break;
-            }
-            case QLX_CX: {
-                TimedLaunch tl(st, "k_2hole", 0.5 * pass);
-                launch_2hole(st, q[0], q[1], 1ull << q[0], (1ull << q[0]) | (1ull << q[1]));
-                account(0.5);
-                break;
-            }
-            case QLX_SWAP: {
-                TimedLaunch tl(st, "k_2hole", 0.5 * pass);
-                launch_2hole(st, q[0], q[1], 1ull << q[1], 1ull << q[0]);
-                account(0.5);
-                break;
-            }
-            case QLX_GLOBAL_SWAP: {
-                if (opts.gswap_mode == QLX_GSWAP_NOOP) break;
-                TimedLaunch tl(st, "k_2hole", 0.5 * pass);
-                launch_2hole(st, q[0], q[1], 1ull << q[1], 1ull << q[0]);
-                account(0.5);
-                break;
-            }
-            case QLX_CCX: {
-                TimedLaunch tl(st, "k_3hole", 0.25 * pass);
-                const uint64_t cm = (1ull << q[0]) | (1ull << q[1]);
-                launch_3hole(st, q[0], q[1], q[2], cm, cm | (1ull << q[2]));
-                account(0.25);
-                break;
-            }
-            case QLX_CSWAP: {
-                TimedLaunch tl(st, "k_3hole", 0.25 * pass);
-                const uint64_t c = 1ull << q[0];
-                launch_3hole(st, q[0], q[1], q[2], c | (1ull << q[2]), c | (1ull << q[1]));
-                account(0.25);
-                break;
-            }
-            case QLX_UNITARY:
-            case QLX_FUSED_BLOCK: {
-                const int k = instr_arity(g);
-                uint8_t qs[5];
-                for (int j = 0; j < k; ++j) qs[j] = (uint8_t)instr_qubit(g, j);
-                const qlx_matrix_entry& e = meta[g.payload.matrix_idx];
-                TimedLaunch tl(st, "k_dense", pass);
-                launch_dense(st, dpool + 2 * (size_t)e.offset, qs, k);
-                account(1.0);
-                break;
-            }
-            case QLX_BARRIER:
-                continue;
-            default:
-                fail(QLX_EUNSUPPORTED, "CPUOpenMPEngine: Unrecognized instruction type");
-        }
         stats.gates_applied++;
+    };
+    const uint8_t* q = g.qubits;
+    switch (g.type) {
+        case QLX_H: case QLX_X: case QLX_Y: case QLX_RX: case QLX_RY: case QLX_RZ: {
+            TimedLaunch tl(st, "k_1q", pass);
+            launch_1q(st, kind_for(g.type), q[0], coef_for(g));
+            account(1.0);
+            break;
+        }
+        case QLX_Z: case QLX_S: {
+            TimedLaunch tl(st, "k_1q_phase", 0.5 * pass);
+            launch_1q(st, kind_for(g.type), q[0], coef_for(g));
+            account(0.5);
+            break;
+        }
+        case QLX_CX: {
+            TimedLaunch tl(st, "k_2hole", 0.5 * pass);
+            launch_2hole(st, q[0], q[1], 1ull << q[0], (1ull << q[0]) | (1ull << q[1]));
+            account(0.5);
+            break;
+        }
+        case QLX_SWAP: {
+            TimedLaunch tl(st, "k_2hole", 0.5 * pass);
+            launch_2hole(st, q[0], q[1], 1ull << q[1], 1ull << q[0]);
+            account(0.5);
+            break;
+        }
+        case QLX_GLOBAL_SWAP: {
+            if (opts.gswap_mode == QLX_GSWAP_NOOP) break;
+            TimedLaunch tl(st, "k_2hole", 0.5 * pass);
+            launch_2hole(st, q[0], q[1], 1ull << q[1], 1ull << q[0]);
+            account(0.5);
+            break;
+        }
+        case QLX_CCX: {
+            TimedLaunch tl(st, "k_3hole", 0.25 * pass);
+            const uint64_t cm = (1ull << q[0]) | (1ull << q[1]);
+            launch_3hole(st, q[0], q[1], q[2], cm, cm | (1ull << q[2]));
+            account(0.25);
+            break;
+        }
+        case QLX_CSWAP: {
+            TimedLaunch tl(st, "k_3hole", 0.25 * pass);
+            const uint64_t c = 1ull << q[0];
+            launch_3hole(st, q[0], q[1], q[2], c | (1ull << q[2]), c | (1ull << q[1]));
+            account(0.25);
+            break;
+        }
+        case QLX_UNITARY:
+        case QLX_FUSED_BLOCK: {
+            const int k = instr_arity(g);
+            uint8_t qs[5];
+            for (int j = 0; j < k; ++j) qs[j] = (uint8_t)instr_qubit(g, j);
+            const qlx_matrix_entry& e = meta[g.payload.matrix_idx];
+            TimedLaunch tl(st, "k_dense", pass);
+            launch_dense(st, dpool + 2 * (size_t)e.offset, qs, k);
+            account(1.0);
+            break;
+        }
+        case QLX_BARRIER:
+            break;
+        default:
+            fail(QLX_EUNSUPPORTED, "CPUOpenMPEngine: Unrecognized instruction type");
     }
+}
+
+void execute_plain(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const double* pool,
+                   size_t pool_elems, const qlx_matrix_entry* meta, size_t,
+                   const qlx_exec_opts& opts, qlx_exec_stats& stats) {
+    bool need_pool = false;
+    for (size_t i = 0; i < n; ++i)
+        if (instrs[i].type == QLX_UNITARY || instrs[i].type == QLX_FUSED_BLOCK) need_pool = true;
+    const double* dpool = need_pool ? upload_pool(st, pool, pool_elems) : nullptr;
+    for (size_t i = 0; i < n; ++i) launch_plain_instr(st, instrs[i], dpool, meta, opts, stats);
 }
 
 }  // namespace qlx
@@ -542,7 +548,7 @@ int qlx_measure_prob1(qlx_state* st, uint32_t q, double* p1) {
         const uint64_t half = 1ull << (st->nq - 1);
         uint64_t nb = (half + kThreads - 1) / kThreads;
         nb = std::min<uint64_t>(std::max<uint64_t>(nb, 1), kRedBlocks);
-        st->scratch.reserve(nb * sizeof(double));
+        st->scratch.reserve(nb * sizeof(double), st->stream);
         if (st->dtype == QLX_C128)
             k_prob1<double2><<<(unsigned)nb, kThreads, 0, st->stream>>>((const double2*)st->amp, half, q,
                                                                         (double*)st->scratch.ptr);

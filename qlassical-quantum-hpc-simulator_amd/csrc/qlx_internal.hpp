@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -55,7 +56,8 @@ int guard(F&& f) noexcept {
 struct DevBuf {
     void* ptr = nullptr;
     size_t cap = 0;
-    void reserve(size_t bytes);
+    // Grows the buffer; synchronises `stream` first so no queued kernel still reads it.
+    void reserve(size_t bytes, hipStream_t stream);
     void release();
 };
 
@@ -89,7 +91,7 @@ struct qlx_state {
     hipStream_t stream = nullptr;
     qlx::DevBuf scratch;          // reduction partials
     qlx::DevBuf prog;             // uploaded gate descriptors / matrices
-    double* host_pinned = nullptr;  // pinned staging for reductions (small)
+    std::shared_ptr<void> sweep_cache;  // last windowed plan (host plan + device image)
     qlx_exec_stats stats{};
 };
 
@@ -130,6 +132,12 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
                       const double* pool, size_t pool_elems,
                       const qlx_matrix_entry* meta, size_t n_meta,
                       const qlx_exec_opts& opts, qlx_exec_stats& stats);
+
+// Uploads the unitary pool (complex128) to the state's program buffer; returns it.
+const double* upload_pool(qlx_state* st, const double* pool, size_t pool_elems);
+// Launches the one-sweep kernel of a single instruction (BARRIER: nothing).
+void launch_plain_instr(qlx_state* st, const qlx_gate_instr& g, const double* dpool, const qlx_matrix_entry* meta,
+                        const qlx_exec_opts& opts, qlx_exec_stats& stats);
 
 // Dense k-qubit unitary on the device state (k <= 5), matrix already on device
 // as row-major complex128. Used by both executors.

@@ -19,9 +19,10 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 void fail(int code, const std::string& msg) { throw Error{code, msg}; }
 
-void DevBuf::reserve(size_t bytes) {
+void DevBuf::reserve(size_t bytes, hipStream_t stream) {
     if (bytes <= cap) return;
     if (ptr) {
+        QLX_HIP_CHECK(hipStreamSynchronize(stream));
         QLX_HIP_CHECK(hipFree(ptr));
         ptr = nullptr;
         cap = 0;
@@ -242,7 +243,7 @@ static double sum_partials(qlx_state* st, unsigned nb, int per_block) {
 double reduce_norm_sq(qlx_state* st) {
     const uint64_t n = 1ull << st->nq;
     const unsigned nb = red_blocks(n);
-    st->scratch.reserve((size_t)nb * 2 * sizeof(double));
+    st->scratch.reserve((size_t)nb * 2 * sizeof(double), st->stream);
     if (st->dtype == QLX_C128)
         k_norm_partial<<<nb, kRedThreads, 0, st->stream>>>((const double2*)st->amp, n, (double*)st->scratch.ptr);
     else
@@ -317,7 +318,7 @@ static void upload_host_state(qlx_state* st, const double* amps, size_t dim) {
         // stage complex128 on the device in slices, then narrow to complex64
         const size_t slice = size_t(1) << 24;
         DevBuf tmp;
-        tmp.reserve(std::min(dim, slice) * 16);
+        tmp.reserve(std::min(dim, slice) * 16, st->stream);
         for (size_t off = 0; off < dim; off += slice) {
             size_t cnt = std::min(slice, dim - off);
             QLX_HIP_CHECK(hipMemcpyAsync(tmp.ptr, amps + 2 * off, cnt * 16, hipMemcpyHostToDevice, st->stream));
@@ -411,7 +412,7 @@ int qlx_state_to_host(qlx_state* st, double* out, size_t dim) {
         } else {
             const size_t slice = size_t(1) << 24;
             DevBuf tmp;
-            tmp.reserve(std::min(n, slice) * 16);
+            tmp.reserve(std::min(n, slice) * 16, st->stream);
             for (size_t off = 0; off < n; off += slice) {
                 size_t cnt = std::min(slice, n - off);
                 k_widen_c64_to_c128<<<fill_blocks(cnt), 256, 0, st->stream>>>((const float2*)st->amp + off,
@@ -464,7 +465,7 @@ int qlx_state_inner(qlx_state* a, qlx_state* b, double* re, double* im) {
         QLX_HIP_CHECK(hipStreamSynchronize(b->stream));
         const uint64_t n = 1ull << a->nq;
         const unsigned nb = red_blocks(n);
-        a->scratch.reserve((size_t)nb * 2 * sizeof(double));
+        a->scratch.reserve((size_t)nb * 2 * sizeof(double), a->stream);
         dispatch_pair(a, b, [&](auto pa, auto pb) {
             k_inner_partial<<<nb, kRedThreads, 0, a->stream>>>(pa, pb, n, (double*)a->scratch.ptr);
         });
@@ -491,7 +492,7 @@ int qlx_state_max_abs_diff(qlx_state* a, qlx_state* b, double* out) {
         QLX_HIP_CHECK(hipStreamSynchronize(b->stream));
         const uint64_t n = 1ull << a->nq;
         const unsigned nb = red_blocks(n);
-        a->scratch.reserve((size_t)nb * 2 * sizeof(double));
+        a->scratch.reserve((size_t)nb * 2 * sizeof(double), a->stream);
         dispatch_pair(a, b, [&](auto pa, auto pb) {
             k_maxdiff_partial<<<nb, kRedThreads, 0, a->stream>>>(pa, pb, n, (double*)a->scratch.ptr);
         });

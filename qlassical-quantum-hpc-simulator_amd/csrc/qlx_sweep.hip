@@ -1,11 +1,891 @@
-// qlx_sweep.hip — windowed executor (LDS/register window sweeps).
+// qlx_sweep.hip — windowed execution: many gates per HBM sweep.
+//
+// This is the GPU re-targeting of the reference's qubit-windowing idea
+// (include/middleware/middleware.hpp:759-860: windows of <= k_safe active qubits).
+// Instead of moving high qubits into a fixed low window with GLOBAL_SWAP passes,
+// each sweep chooses its own window S of W qubits (always containing the low C
+// qubits, so every HBM access is a contiguous 2^C-amplitude segment). One
+// work-group owns one 2^W-amplitude chunk of the state (all amplitudes that share
+// the bits outside S), keeps it in registers (2^RB amplitudes per work-item,
+// T = 2^LOGT work-items, W = LOGT + RB) and applies every gate of the sweep there:
+//   * a gate whose qubits are "register bits" of the current layout is pure VALU work;
+//   * between phases the chunk is re-laid out through LDS (one write + one read of
+//     the chunk), which moves a new set of qubits into the register bits.
+// The state is read and written once per sweep: algorithmic bytes 2 * sizeof(amp) * 2^N.
+//
+// The host planner (below) builds the sweeps from the gate DAG (gates commute when
+// their qubits are disjoint), the register phases inside each sweep, and the device
+// descriptors. GLOBAL_SWAP is a real qubit permutation here (a SWAP inside a sweep);
+// CCX/CSWAP and dense gates on 3+ qubits run as their own plain kernels between sweeps.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <type_traits>
+#include <vector>
+
 #include "qlx_internal.hpp"
 
 namespace qlx {
 
-void execute_windowed(qlx_state*, const qlx_gate_instr*, size_t, const double*, size_t,
-                      const qlx_matrix_entry*, size_t, const qlx_exec_opts&, qlx_exec_stats&) {
-    fail(QLX_EUNSUPPORTED, "windowed execution is not built yet");
+constexpr int kMaxW = 14;
+constexpr int kMaxLogT = 10;
+constexpr int kMaxR = 32;
+
+enum OpKind : uint8_t { OP_U2 = 1, OP_RX = 2, OP_H = 3, OP_CX = 4, OP_SWAP = 5, OP_U4 = 6 };
+
+struct DevOp {
+    uint8_t kind, b0, b1, pad;
+    uint32_t coef;  // index into the coefficient array (doubles)
+};
+
+struct DevPhase {
+    uint32_t n_ops, op_begin, remap, pad;
+    uint32_t roff[kMaxR];        // local-index contribution of register index i
+    uint8_t tlocal[kMaxLogT];    // local bit held by work-item bit b
+    uint8_t pad2[6];
+};
+
+struct DevSweep {
+    uint32_t n_phases, phase_begin;
+    uint8_t S[kMaxW];            // global qubit of local bit j (ascending)
+    uint8_t ld_tglob[kMaxLogT];  // global qubit at work-item bit b in the first layout
+    uint8_t st_tglob[kMaxLogT];  // ... in the last layout
+    uint64_t ld_roff[kMaxR];     // global offset of register i in the first layout
+    uint64_t st_roff[kMaxR];     // ... in the last layout
+};
+
+// ---------------------------------------------------------------------------------
+// device side
+// ---------------------------------------------------------------------------------
+template <class A>
+struct Real;
+template <>
+struct Real<double2> { using T = double; };
+template <>
+struct Real<float2> { using T = float; };
+
+__device__ __forceinline__ uint64_t dins0(uint64_t idx, uint32_t t) {
+    const uint64_t lo = idx & ((1ull << t) - 1);
+    return lo | ((idx ^ lo) << 1);
+}
+
+// LDS bank swizzle: XOR the low bits of the element index with a fold of its high
+// bits (a bijection on [0, 2^W)), so work-items whose local indices differ only in
+// high bits still land in different banks.
+template <int ELEM>
+__device__ __forceinline__ uint32_t swz(uint32_t l) {
+    constexpr int SH = ELEM == 16 ? 4 : 5;
+    constexpr uint32_t M = (1u << SH) - 1;
+    return l ^ (((l >> SH) ^ (l >> (2 * SH)) ^ (l >> (3 * SH))) & M);
+}
+
+template <int B, class A, int R>
+__device__ __forceinline__ void g_rx(A (&v)[R], typename Real<A>::T c, typename Real<A>::T s) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        if (i & (1 << B)) continue;
+        const A a0 = v[i], a1 = v[i | (1 << B)];
+        v[i] = A{c * a0.x + s * a1.y, c * a0.y + (-s * a1.x)};
+        v[i | (1 << B)] = A{s * a0.y + c * a1.x, -s * a0.x + c * a1.y};
+    }
+}
+
+template <int B, class A, int R>
+__device__ __forceinline__ void g_h(A (&v)[R], typename Real<A>::T r) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        if (i & (1 << B)) continue;
+        const A a0 = v[i], a1 = v[i | (1 << B)];
+        v[i] = A{r * (a0.x + a1.x), r * (a0.y + a1.y)};
+        v[i | (1 << B)] = A{r * (a0.x - a1.x), r * (a0.y - a1.y)};
+    }
+}
+
+template <int B, class A, int R>
+__device__ __forceinline__ void g_u2(A (&v)[R], const double* __restrict__ m) {
+    using T = typename Real<A>::T;
+    const T m0r = (T)m[0], m0i = (T)m[1], m1r = (T)m[2], m1i = (T)m[3];
+    const T m2r = (T)m[4], m2i = (T)m[5], m3r = (T)m[6], m3i = (T)m[7];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        if (i & (1 << B)) continue;
+        const A a0 = v[i], a1 = v[i | (1 << B)];
+        v[i] = A{m0r * a0.x - m0i * a0.y + m1r * a1.x - m1i * a1.y, m0r * a0.y + m0i * a0.x + m1r * a1.y + m1i * a1.x};
+        v[i | (1 << B)] =
+            A{m2r * a0.x - m2i * a0.y + m3r * a1.x - m3i * a1.y, m2r * a0.y + m2i * a0.x + m3r * a1.y + m3i * a1.x};
+    }
+}
+
+template <int BC, int BT, class A, int R>
+__device__ __forceinline__ void g_cx(A (&v)[R]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        if (!(i & (1 << BC)) || (i & (1 << BT))) continue;
+        const A t = v[i];
+        v[i] = v[i | (1 << BT)];
+        v[i | (1 << BT)] = t;
+    }
+}
+
+template <int B0, int B1, class A, int R>
+__device__ __forceinline__ void g_swap(A (&v)[R]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        if (!(i & (1 << B0)) || (i & (1 << B1))) continue;
+        const int j = i ^ (1 << B0) ^ (1 << B1);
+        const A t = v[i];
+        v[i] = v[j];
+        v[j] = t;
+    }
+}
+
+// Dense 4x4 on register bits (B0 = matrix bit 0, B1 = matrix bit 1), row-major complex128.
+template <int B0, int B1, class A, int R>
+__device__ __forceinline__ void g_u4(A (&v)[R], const double* __restrict__ m) {
+    using T = typename Real<A>::T;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        if ((i & (1 << B0)) || (i & (1 << B1))) continue;
+        const int idx[4] = {i, i | (1 << B0), i | (1 << B1), i | (1 << B0) | (1 << B1)};
+        A x[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) x[c] = v[idx[c]];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            T re = T(0), im = T(0);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const T ur = (T)m[2 * (4 * r + c)], ui = (T)m[2 * (4 * r + c) + 1];
+                re += ur * x[c].x - ui * x[c].y;
+                im += ur * x[c].y + ui * x[c].x;
+            }
+            v[idx[r]] = A{re, im};
+        }
+    }
+}
+
+template <int RB, class F>
+__device__ __forceinline__ void with_bit(int b, F&& f) {
+    switch (b) {
+        case 0: f(std::integral_constant<int, 0>{}); break;
+        case 1: if constexpr (RB > 1) f(std::integral_constant<int, 1>{}); break;
+        case 2: if constexpr (RB > 2) f(std::integral_constant<int, 2>{}); break;
+        case 3: if constexpr (RB > 3) f(std::integral_constant<int, 3>{}); break;
+        case 4: if constexpr (RB > 4) f(std::integral_constant<int, 4>{}); break;
+        default: break;
+    }
+}
+
+template <class A, int R, int RB>
+__device__ __forceinline__ void apply_op(A (&v)[R], const DevOp op, const double* __restrict__ coef) {
+    using T = typename Real<A>::T;
+    const double* m = coef + op.coef;
+    switch (op.kind) {
+        case OP_RX: {
+            const T c = (T)m[0], s = (T)m[1];
+            with_bit<RB>(op.b0, [&](auto B) { g_rx<decltype(B)::value>(v, c, s); });
+            break;
+        }
+        case OP_H: {
+            const T r = (T)m[0];
+            with_bit<RB>(op.b0, [&](auto B) { g_h<decltype(B)::value>(v, r); });
+            break;
+        }
+        case OP_U2:
+            with_bit<RB>(op.b0, [&](auto B) { g_u2<decltype(B)::value>(v, m); });
+            break;
+        case OP_CX:
+            with_bit<RB>(op.b0, [&](auto B0) {
+                with_bit<RB>(op.b1, [&](auto B1) {
+                    if constexpr (decltype(B0)::value != decltype(B1)::value)
+                        g_cx<decltype(B0)::value, decltype(B1)::value>(v);
+                });
+            });
+            break;
+        case OP_SWAP:
+            with_bit<RB>(op.b0, [&](auto B0) {
+                with_bit<RB>(op.b1, [&](auto B1) {
+                    if constexpr (decltype(B0)::value != decltype(B1)::value)
+                        g_swap<decltype(B0)::value, decltype(B1)::value>(v);
+                });
+            });
+            break;
+        case OP_U4:
+            with_bit<RB>(op.b0, [&](auto B0) {
+                with_bit<RB>(op.b1, [&](auto B1) {
+                    if constexpr (decltype(B0)::value != decltype(B1)::value)
+                        g_u4<decltype(B0)::value, decltype(B1)::value>(v, m);
+                });
+            });
+            break;
+        default: break;
+    }
+}
+
+template <class A, int LOGT, int RB>
+__global__ void __launch_bounds__(1 << LOGT) k_sweep(A* __restrict__ amp, const DevSweep* __restrict__ sw,
+                                                    const DevPhase* __restrict__ ph, const DevOp* __restrict__ ops,
+                                                    const double* __restrict__ coef) {
+    constexpr int R = 1 << RB;
+    constexpr int W = LOGT + RB;
+    constexpr int ELEM = (int)sizeof(A);
+    extern __shared__ __align__(16) unsigned char smem[];
+    A* lds = reinterpret_cast<A*>(smem);
+    const uint32_t t = threadIdx.x;
+
+    uint64_t base = blockIdx.x;
+#pragma unroll
+    for (int j = 0; j < W; ++j) base = dins0(base, sw->S[j]);
+
+    uint64_t toff = 0;
+#pragma unroll
+    for (int b = 0; b < LOGT; ++b) toff |= (uint64_t)((t >> b) & 1u) << sw->ld_tglob[b];
+
+    A v[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[i] = amp[base | toff | sw->ld_roff[i]];
+
+    const DevPhase* cur = ph + sw->phase_begin;
+    uint32_t lt = 0;
+#pragma unroll
+    for (int b = 0; b < LOGT; ++b) lt |= ((t >> b) & 1u) << cur->tlocal[b];
+
+    const uint32_t nph = sw->n_phases;
+    for (uint32_t p = 0; p < nph; ++p) {
+        const DevPhase* P = ph + sw->phase_begin + p;
+        if (P->remap) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) lds[swz<ELEM>(lt | cur->roff[i])] = v[i];
+            __syncthreads();
+            lt = 0;
+#pragma unroll
+            for (int b = 0; b < LOGT; ++b) lt |= ((t >> b) & 1u) << P->tlocal[b];
+#pragma unroll
+            for (int i = 0; i < R; ++i) v[i] = lds[swz<ELEM>(lt | P->roff[i])];
+            __syncthreads();
+        }
+        cur = P;
+        const uint32_t nops = P->n_ops, ob = P->op_begin;
+        for (uint32_t o = 0; o < nops; ++o) apply_op<A, R, RB>(v, ops[ob + o], coef);
+    }
+
+    toff = 0;
+#pragma unroll
+    for (int b = 0; b < LOGT; ++b) toff |= (uint64_t)((t >> b) & 1u) << sw->st_tglob[b];
+#pragma unroll
+    for (int i = 0; i < R; ++i) amp[base | toff | sw->st_roff[i]] = v[i];
+}
+
+// ---------------------------------------------------------------------------------
+// host side: planning
+// ---------------------------------------------------------------------------------
+struct Geometry {
+    int logt, rb, w, c;
+};
+
+static Geometry geometry_for(int dtype, uint32_t nq, const qlx_exec_opts& o) {
+    Geometry g{};
+    if (dtype == QLX_C128) {
+        g.logt = 8;
+        g.rb = 4;
+    } else {
+        g.logt = 8;
+        g.rb = 5;
+    }
+    g.w = g.logt + g.rb;
+    g.c = o.coalesce_qubits ? (int)o.coalesce_qubits : 3;
+    if (o.window_qubits && (int)o.window_qubits != g.w)
+        fail(QLX_EUNSUPPORTED, "window_qubits " + std::to_string(o.window_qubits) + " not built (this precision uses " +
+                                   std::to_string(g.w) + ")");
+    if (g.c < 0 || g.c > g.logt) fail(QLX_EINVAL, "coalesce_qubits out of range");
+    (void)nq;
+    return g;
+}
+
+struct PGate {
+    int src;            // index in the instruction stream
+    uint8_t type;
+    int nq;             // operand count
+    int q[5];
+    bool sweepable;
+};
+
+struct PlanOp {
+    int src;
+    uint8_t kind;
+    int l0, l1;  // local bits (l1 = -1 for 1q)
+};
+
+struct PlanPhase {
+    std::vector<int> st2loc;  // storage bit -> local bit (size W)
+    std::vector<PlanOp> ops;
+};
+
+struct PlanSweep {
+    std::vector<int> S;  // global qubits, ascending, size W
+    std::vector<PlanPhase> phases;
+};
+
+struct PlanStep {
+    bool sweep;
+    int src;   // plain instruction
+    int sweep_index;
+};
+
+struct SweepPlan {
+    Geometry geo{};
+    std::vector<PlanStep> steps;
+    std::vector<PlanSweep> sweeps;
+    std::vector<int> order;     // execution order of source instructions
+    std::vector<int> order_sweep;
+    // device image
+    std::vector<DevSweep> dsw;
+    std::vector<DevPhase> dph;
+    std::vector<DevOp> dop;
+    std::vector<double> coef;
+    uint64_t phases = 0, gates_in_sweeps = 0;
+};
+
+static bool is_sweepable(const qlx_gate_instr& g, const qlx_matrix_entry* meta) {
+    switch (g.type) {
+        case QLX_H: case QLX_X: case QLX_Y: case QLX_Z: case QLX_S: case QLX_RX: case QLX_RY: case QLX_RZ:
+        case QLX_CX: case QLX_SWAP: case QLX_GLOBAL_SWAP:
+            return true;
+        case QLX_UNITARY:
+        case QLX_FUSED_BLOCK:
+            return meta && instr_arity(g) <= 2;
+        default:
+            return false;
+    }
+}
+
+static int operand_count(const qlx_gate_instr& g) {
+    switch (g.type) {
+        case QLX_CX: case QLX_SWAP: case QLX_GLOBAL_SWAP: return 2;
+        case QLX_CCX: case QLX_CSWAP: return 3;
+        case QLX_UNITARY: case QLX_FUSED_BLOCK: return instr_arity(g);
+        default: return 1;
+    }
+}
+
+// Forward pass: which pending gates can run if exactly the qubits in `in_set` are
+// available (sweepable gates only; a gate needs all predecessors done or runnable).
+static int count_runnable(const std::vector<PGate>& gates, const std::vector<int>& pending,
+                          const std::vector<char>& done, const std::vector<char>& in_set,
+                          std::vector<char>& run, const std::vector<std::vector<int>>& preds) {
+    int cnt = 0;
+    for (int gi : pending) {
+        const PGate& g = gates[gi];
+        bool ok = g.sweepable;
+        for (int k = 0; ok && k < g.nq; ++k) ok = in_set[g.q[k]];
+        for (size_t k = 0; ok && k < preds[gi].size(); ++k) {
+            const int p = preds[gi][k];
+            ok = done[p] || run[p];
+        }
+        run[gi] = ok;
+        cnt += ok;
+    }
+    return cnt;
+}
+
+static void fill_coef(std::vector<double>& coef, const qlx_gate_instr& g, const double* pool,
+                      const qlx_matrix_entry* meta, uint8_t& kind) {
+    auto push_u2 = [&](double m00r, double m00i, double m01r, double m01i, double m10r, double m10i, double m11r,
+                       double m11i) {
+        kind = OP_U2;
+        double m[8] = {m00r, m00i, m01r, m01i, m10r, m10i, m11r, m11i};
+        coef.insert(coef.end(), m, m + 8);
+    };
+    switch (g.type) {
+        case QLX_H:
+            kind = OP_H;
+            coef.push_back(1.0 / std::sqrt(2.0));
+            break;
+        case QLX_RX: {
+            kind = OP_RX;
+            const double half = static_cast<double>(g.payload.param) * 0.5;
+            coef.push_back(std::cos(half));
+            coef.push_back(std::sin(half));
+            break;
+        }
+        case QLX_RY: {
+            const double half = static_cast<double>(g.payload.param) * 0.5;
+            const double c = std::cos(half), s = std::sin(half);
+            push_u2(c, 0, -s, 0, s, 0, c, 0);
+            break;
+        }
+        case QLX_RZ: {
+            const double half = static_cast<double>(g.payload.param) * 0.5;
+            push_u2(std::cos(-half), std::sin(-half), 0, 0, 0, 0, std::cos(half), std::sin(half));
+            break;
+        }
+        case QLX_X: push_u2(0, 0, 1, 0, 1, 0, 0, 0); break;
+        case QLX_Y: push_u2(0, 0, 0, -1, 0, 1, 0, 0); break;
+        case QLX_Z: push_u2(1, 0, 0, 0, 0, 0, -1, 0); break;
+        case QLX_S: push_u2(1, 0, 0, 0, 0, 0, 0, 1); break;
+        case QLX_CX: kind = OP_CX; break;
+        case QLX_SWAP: case QLX_GLOBAL_SWAP: kind = OP_SWAP; break;
+        case QLX_UNITARY:
+        case QLX_FUSED_BLOCK: {
+            const int k = instr_arity(g);
+            const qlx_matrix_entry& e = meta[g.payload.matrix_idx];
+            const double* m = pool + 2 * (size_t)e.offset;
+            coef.insert(coef.end(), m, m + 2 * (size_t)e.num_elements);
+            kind = k == 1 ? OP_U2 : OP_U4;
+            break;
+        }
+        default:
+            fail(QLX_EUNSUPPORTED, "gate cannot run inside a sweep");
+    }
+}
+
+// Greedily chooses `want` qubits from `allowed` (seeded with `seed`) maximising the
+// number of runnable pending gates.
+static std::vector<int> choose_set(const std::vector<PGate>& gates, const std::vector<int>& pending,
+                                   const std::vector<char>& done, const std::vector<std::vector<int>>& preds,
+                                   int nbits, std::vector<char> in_set, int have, int want,
+                                   const std::vector<char>& allowed) {
+    std::vector<char> run(gates.size(), 0);
+    // first-use rank of each bit among pending gates, for tie-breaking
+    std::vector<int> rank(nbits, 1 << 30);
+    int r = 0;
+    for (int gi : pending)
+        for (int k = 0; k < gates[gi].nq; ++k) rank[gates[gi].q[k]] = std::min(rank[gates[gi].q[k]], r++);
+    while (have < want) {
+        int best = -1, best_score = -1, best_rank = 1 << 30;
+        for (int b = 0; b < nbits; ++b) {
+            if (in_set[b] || !allowed[b]) continue;
+            in_set[b] = 1;
+            const int sc = count_runnable(gates, pending, done, in_set, run, preds);
+            in_set[b] = 0;
+            if (sc > best_score || (sc == best_score && rank[b] < best_rank)) {
+                best = b;
+                best_score = sc;
+                best_rank = rank[b];
+            }
+        }
+        if (best < 0) break;
+        in_set[best] = 1;
+        ++have;
+    }
+    std::vector<int> out;
+    for (int b = 0; b < nbits; ++b)
+        if (in_set[b]) out.push_back(b);
+    return out;
+}
+
+// Register phases of one sweep (gates given in local bits).
+static void plan_phases(PlanSweep& sw, const std::vector<PGate>& lg, const Geometry& geo) {
+    const int W = geo.w, RB = geo.rb, LT = geo.logt;
+    const int n = (int)lg.size();
+    std::vector<std::vector<int>> preds(n);
+    {
+        std::vector<int> last(W, -1);
+        for (int i = 0; i < n; ++i) {
+            for (int k = 0; k < lg[i].nq; ++k) {
+                const int b = lg[i].q[k];
+                if (last[b] >= 0) preds[i].push_back(last[b]);
+                last[b] = i;
+            }
+        }
+    }
+    std::vector<char> done(n, 0);
+    std::vector<int> pending(n);
+    for (int i = 0; i < n; ++i) pending[i] = i;
+
+    auto make_layout = [&](const std::vector<int>& regset) {
+        std::vector<int> st2loc(W, -1);
+        std::vector<char> isreg(W, 0);
+        for (int b : regset) isreg[b] = 1;
+        int ti = 0;
+        for (int l = 0; l < W; ++l)
+            if (!isreg[l]) st2loc[ti++] = l;          // work-item bits in ascending local order
+        int ri = 0;
+        for (int l : regset) st2loc[LT + ri++] = l;    // register bits
+        return st2loc;
+    };
+    auto regset_of = [&](const std::vector<int>& st2loc) {
+        std::vector<int> r;
+        for (int s = LT; s < W; ++s) r.push_back(st2loc[s]);
+        return r;
+    };
+
+    // first layout: the low C local bits must stay work-item bits (coalesced load)
+    std::vector<char> allowed(W, 1);
+    for (int l = 0; l < geo.c; ++l) allowed[l] = 0;
+    std::vector<char> none(W, 0);
+    std::vector<int> regset = choose_set(lg, pending, done, preds, W, none, 0, RB, allowed);
+    std::vector<char> all(W, 1);
+    bool first = true;
+    int guard_iters = 0;
+    while (true) {
+        PlanPhase phase;
+        phase.st2loc = make_layout(regset);
+        std::vector<int> loc2st(W);
+        for (int s = 0; s < W; ++s) loc2st[phase.st2loc[s]] = s;
+        std::vector<char> in_set(W, 0);
+        for (int b : regset) in_set[b] = 1;
+        std::vector<char> run(n, 0);
+        count_runnable(lg, pending, done, in_set, run, preds);
+        std::vector<int> still;
+        for (int gi : pending) {
+            if (run[gi]) {
+                PlanOp op{};
+                op.src = lg[gi].src;
+                op.l0 = loc2st[lg[gi].q[0]] - LT;
+                op.l1 = lg[gi].nq > 1 ? loc2st[lg[gi].q[1]] - LT : -1;
+                op.kind = lg[gi].type;  // instruction type; resolved to an OpKind later
+                phase.ops.push_back(op);
+                done[gi] = 1;
+            } else {
+                still.push_back(gi);
+            }
+        }
+        pending.swap(still);
+        if (!phase.ops.empty() || first) sw.phases.push_back(std::move(phase));
+        first = false;
+        if (pending.empty()) break;
+        if (++guard_iters > 4 * n + 8) fail(QLX_EHIP, "sweep phase planner made no progress");
+        // next register set: seed with the first pending gate's bits
+        std::vector<char> seed(W, 0);
+        int have = 0;
+        const PGate& g0 = lg[pending.front()];
+        for (int k = 0; k < g0.nq; ++k)
+            if (!seed[g0.q[k]]) {
+                seed[g0.q[k]] = 1;
+                ++have;
+            }
+        regset = choose_set(lg, pending, done, preds, W, seed, have, RB, all);
+    }
+    // last layout must keep the low C local bits as the low work-item bits (coalesced store)
+    std::vector<int> last = regset_of(sw.phases.back().st2loc);
+    bool low_in_regs = false;
+    for (int l : last)
+        if (l < geo.c) low_in_regs = true;
+    if (low_in_regs) {
+        std::vector<int> rs;
+        for (int l = W - 1; l >= geo.c && (int)rs.size() < RB; --l) rs.push_back(l);
+        std::sort(rs.begin(), rs.end());
+        PlanPhase phase;
+        phase.st2loc = make_layout(rs);
+        sw.phases.push_back(std::move(phase));
+    }
+}
+
+static std::unique_ptr<SweepPlan> build_plan(uint32_t nq, int dtype, const qlx_gate_instr* instrs, size_t n,
+                                             const double* pool, const qlx_matrix_entry* meta,
+                                             const qlx_exec_opts& opts) {
+    auto plan = std::make_unique<SweepPlan>();
+    const Geometry geo = geometry_for(dtype, nq, opts);
+    plan->geo = geo;
+    const int W = geo.w;
+
+    // gate list (BARRIER and no-op GLOBAL_SWAP dropped)
+    std::vector<PGate> gates;
+    for (size_t i = 0; i < n; ++i) {
+        const qlx_gate_instr& g = instrs[i];
+        if (g.type == QLX_BARRIER) continue;
+        if (g.type == QLX_GLOBAL_SWAP && opts.gswap_mode == QLX_GSWAP_NOOP) continue;
+        PGate p{};
+        p.src = (int)i;
+        p.type = g.type;
+        p.nq = operand_count(g);
+        for (int k = 0; k < p.nq; ++k) p.q[k] = instr_qubit(g, k);
+        p.sweepable = is_sweepable(g, meta) && (int)nq >= W;
+        gates.push_back(p);
+    }
+    const int G = (int)gates.size();
+    std::vector<std::vector<int>> preds(G);
+    {
+        std::vector<int> last(nq, -1);
+        for (int i = 0; i < G; ++i)
+            for (int k = 0; k < gates[i].nq; ++k) {
+                const int q = gates[i].q[k];
+                if (last[q] >= 0) preds[i].push_back(last[q]);
+                last[q] = i;
+            }
+    }
+    std::vector<char> done(G, 0);
+    std::vector<int> pending(G);
+    for (int i = 0; i < G; ++i) pending[i] = i;
+
+    auto prune = [&]() {
+        std::vector<int> still;
+        for (int gi : pending)
+            if (!done[gi]) still.push_back(gi);
+        pending.swap(still);
+    };
+
+    while (!pending.empty()) {
+        // 1. plain-only gates whose predecessors are done run as their own kernels
+        bool progress = true;
+        while (progress) {
+            progress = false;
+            for (int gi : pending) {
+                if (done[gi] || gates[gi].sweepable) continue;
+                bool ready = true;
+                for (int p : preds[gi]) ready = ready && done[p];
+                if (!ready) continue;
+                done[gi] = 1;
+                plan->steps.push_back(PlanStep{false, gates[gi].src, -1});
+                plan->order.push_back(gates[gi].src);
+                plan->order_sweep.push_back(-1);
+                progress = true;
+            }
+            prune();
+        }
+        if (pending.empty()) break;
+
+        // 2. choose the window: low C qubits + the first runnable sweepable gate's qubits + greedy
+        std::vector<char> in_set(nq, 0);
+        int have = 0;
+        for (int l = 0; l < geo.c && l < (int)nq; ++l) {
+            in_set[l] = 1;
+            ++have;
+        }
+        int seed_gate = -1;
+        for (int gi : pending) {
+            if (!gates[gi].sweepable) continue;
+            bool ready = true;
+            for (int p : preds[gi]) ready = ready && done[p];
+            if (ready) {
+                seed_gate = gi;
+                break;
+            }
+        }
+        if (seed_gate < 0) fail(QLX_EHIP, "sweep planner: no runnable gate");
+        for (int k = 0; k < gates[seed_gate].nq; ++k)
+            if (!in_set[gates[seed_gate].q[k]]) {
+                in_set[gates[seed_gate].q[k]] = 1;
+                ++have;
+            }
+        std::vector<char> allowed(nq, 1);
+        std::vector<int> S = choose_set(gates, pending, done, preds, (int)nq, in_set, have, W, allowed);
+        if ((int)S.size() != W) fail(QLX_EHIP, "sweep planner: window smaller than W");
+
+        // 3. the sweep's gates, in program order
+        std::vector<char> sset(nq, 0);
+        for (int q : S) sset[q] = 1;
+        std::vector<char> run(G, 0);
+        count_runnable(gates, pending, done, sset, run, preds);
+        std::vector<int> loc(nq, -1);
+        for (int j = 0; j < W; ++j) loc[S[j]] = j;
+        std::vector<PGate> lg;
+        for (int gi : pending) {
+            if (!run[gi]) continue;
+            PGate l = gates[gi];
+            for (int k = 0; k < l.nq; ++k) l.q[k] = loc[l.q[k]];
+            lg.push_back(l);
+            done[gi] = 1;
+        }
+        prune();
+        PlanSweep sw;
+        sw.S = S;
+        plan_phases(sw, lg, geo);
+        const int si = (int)plan->sweeps.size();
+        for (const PlanPhase& ph : sw.phases)
+            for (const PlanOp& op : ph.ops) {
+                plan->order.push_back(op.src);
+                plan->order_sweep.push_back(si);
+            }
+        plan->gates_in_sweeps += lg.size();
+        plan->phases += sw.phases.size();
+        plan->sweeps.push_back(std::move(sw));
+        plan->steps.push_back(PlanStep{true, -1, si});
+    }
+
+    // device image
+    for (const PlanSweep& sw : plan->sweeps) {
+        DevSweep d{};
+        d.n_phases = (uint32_t)sw.phases.size();
+        d.phase_begin = (uint32_t)plan->dph.size();
+        for (int j = 0; j < W; ++j) d.S[j] = (uint8_t)sw.S[j];
+        auto glob_of_local = [&](int l) { return sw.S[l]; };
+        auto fill_ends = [&](const PlanPhase& ph, uint8_t* tglob, uint64_t* roff) {
+            for (int b = 0; b < geo.logt; ++b) tglob[b] = (uint8_t)glob_of_local(ph.st2loc[b]);
+            for (int i = 0; i < (1 << geo.rb); ++i) {
+                uint64_t o = 0;
+                for (int b = 0; b < geo.rb; ++b)
+                    if ((i >> b) & 1) o |= 1ull << glob_of_local(ph.st2loc[geo.logt + b]);
+                roff[i] = o;
+            }
+        };
+        fill_ends(sw.phases.front(), d.ld_tglob, d.ld_roff);
+        fill_ends(sw.phases.back(), d.st_tglob, d.st_roff);
+        plan->dsw.push_back(d);
+        for (size_t p = 0; p < sw.phases.size(); ++p) {
+            const PlanPhase& ph = sw.phases[p];
+            DevPhase dp{};
+            dp.remap = p > 0 ? 1u : 0u;
+            dp.op_begin = (uint32_t)plan->dop.size();
+            dp.n_ops = (uint32_t)ph.ops.size();
+            for (int b = 0; b < geo.logt; ++b) dp.tlocal[b] = (uint8_t)ph.st2loc[b];
+            for (int i = 0; i < (1 << geo.rb); ++i) {
+                uint32_t o = 0;
+                for (int b = 0; b < geo.rb; ++b)
+                    if ((i >> b) & 1) o |= 1u << ph.st2loc[geo.logt + b];
+                dp.roff[i] = o;
+            }
+            plan->dph.push_back(dp);
+            for (const PlanOp& op : ph.ops) {
+                DevOp o{};
+                o.coef = (uint32_t)plan->coef.size();
+                fill_coef(plan->coef, instrs[op.src], pool, meta, o.kind);
+                o.b0 = (uint8_t)op.l0;
+                o.b1 = (uint8_t)(op.l1 < 0 ? 0 : op.l1);
+                if (o.kind == OP_CX || o.kind == OP_SWAP || o.kind == OP_U4) {
+                    if (op.l1 < 0 || op.l0 == op.l1) fail(QLX_EHIP, "sweep planner: bad two-qubit operands");
+                }
+                plan->dop.push_back(o);
+            }
+        }
+    }
+    return plan;
+}
+
+// ---------------------------------------------------------------------------------
+// host side: execution
+// ---------------------------------------------------------------------------------
+static uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+template <class A, int LOGT, int RB>
+static void launch_sweep(qlx_state* st, const char* dev, size_t off_sw, size_t off_ph, size_t off_op, size_t off_cf,
+                         int si) {
+    const uint64_t chunks = 1ull << (st->nq - (LOGT + RB));
+    const size_t lds = (size_t(1) << (LOGT + RB)) * sizeof(A);
+    k_sweep<A, LOGT, RB><<<(unsigned)chunks, 1 << LOGT, lds, st->stream>>>(
+        (A*)st->amp, (const DevSweep*)(dev + off_sw) + si, (const DevPhase*)(dev + off_ph),
+        (const DevOp*)(dev + off_op), (const double*)(dev + off_cf));
+    QLX_HIP_CHECK(hipGetLastError());
+}
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+struct CachedPlan {
+    uint64_t key = 0;
+    std::unique_ptr<SweepPlan> plan;
+    DevBuf dev;
+    size_t off_sw = 0, off_ph = 0, off_op = 0, off_cf = 0;
+};
+
+static CachedPlan& plan_cache(qlx_state* st) {
+    if (!st->sweep_cache) {
+        auto* c = new CachedPlan();
+        st->sweep_cache = std::shared_ptr<void>(c, [](void* p) {
+            auto* cp = static_cast<CachedPlan*>(p);
+            cp->dev.release();
+            delete cp;
+        });
+    }
+    return *static_cast<CachedPlan*>(st->sweep_cache.get());
+}
+
+void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const double* pool, size_t pool_elems,
+                      const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts& opts, qlx_exec_stats& stats) {
+    const Geometry geo = geometry_for(st->dtype, st->nq, opts);
+    if ((int)st->nq < geo.w) {
+        // the state is smaller than one window: every gate runs as its own kernel
+        execute_plain(st, instrs, n, pool, pool_elems, meta, n_meta, opts, stats);
+        return;
+    }
+    uint64_t key = 1469598103934665603ull;
+    key = fnv1a(key, instrs, n * sizeof(qlx_gate_instr));
+    key = fnv1a(key, pool, pool ? pool_elems * 16 : 0);
+    key = fnv1a(key, meta, meta ? n_meta * sizeof(qlx_matrix_entry) : 0);
+    key = fnv1a(key, &opts, sizeof(opts));
+    key = fnv1a(key, &st->nq, sizeof(st->nq));
+    key = fnv1a(key, &st->dtype, sizeof(st->dtype));
+    CachedPlan& cp = plan_cache(st);
+    if (!cp.plan || cp.key != key) {
+        cp.plan = build_plan(st->nq, st->dtype, instrs, n, pool, meta, opts);
+        cp.key = key;
+        SweepPlan& p = *cp.plan;
+        cp.off_sw = 0;
+        cp.off_ph = align_up(cp.off_sw + p.dsw.size() * sizeof(DevSweep), 256);
+        cp.off_op = align_up(cp.off_ph + p.dph.size() * sizeof(DevPhase), 256);
+        cp.off_cf = align_up(cp.off_op + p.dop.size() * sizeof(DevOp), 256);
+        const size_t total = align_up(cp.off_cf + p.coef.size() * sizeof(double), 256) + 256;
+        std::vector<char> img(total, 0);
+        if (!p.dsw.empty()) std::memcpy(img.data() + cp.off_sw, p.dsw.data(), p.dsw.size() * sizeof(DevSweep));
+        if (!p.dph.empty()) std::memcpy(img.data() + cp.off_ph, p.dph.data(), p.dph.size() * sizeof(DevPhase));
+        if (!p.dop.empty()) std::memcpy(img.data() + cp.off_op, p.dop.data(), p.dop.size() * sizeof(DevOp));
+        if (!p.coef.empty()) std::memcpy(img.data() + cp.off_cf, p.coef.data(), p.coef.size() * sizeof(double));
+        QLX_HIP_CHECK(hipStreamSynchronize(st->stream));  // the old image may still be in use
+        cp.dev.reserve(total, st->stream);
+        QLX_HIP_CHECK(hipMemcpyAsync(cp.dev.ptr, img.data(), total, hipMemcpyHostToDevice, st->stream));
+        QLX_HIP_CHECK(hipStreamSynchronize(st->stream));
+    }
+    const SweepPlan& p = *cp.plan;
+    bool need_pool = false;
+    for (const PlanStep& s : p.steps)
+        if (!s.sweep && (instrs[s.src].type == QLX_UNITARY || instrs[s.src].type == QLX_FUSED_BLOCK)) need_pool = true;
+    const double* dpool = need_pool ? upload_pool(st, pool, pool_elems) : nullptr;
+    const char* dev = (const char*)cp.dev.ptr;
+    const double pass = pass_bytes(st);
+    for (const PlanStep& s : p.steps) {
+        if (!s.sweep) {
+            launch_plain_instr(st, instrs[s.src], dpool, meta, opts, stats);
+            continue;
+        }
+        {
+            TimedLaunch tl(st, "k_sweep", pass);
+            if (st->dtype == QLX_C128)
+                launch_sweep<double2, 8, 4>(st, dev, cp.off_sw, cp.off_ph, cp.off_op, cp.off_cf, s.sweep_index);
+            else
+                launch_sweep<float2, 8, 5>(st, dev, cp.off_sw, cp.off_ph, cp.off_op, cp.off_cf, s.sweep_index);
+        }
+        stats.sweeps_full++;
+        stats.algorithmic_bytes += pass;
+    }
+    stats.gates_applied += p.gates_in_sweeps;
 }
 
 }  // namespace qlx
+
+using namespace qlx;
+
+extern "C" int qlx_plan_windowed(uint32_t num_qubits, int dtype, const qlx_gate_instr* instrs, size_t n,
+                                 const double* pool, size_t pool_elems, const qlx_matrix_entry* meta, size_t n_meta,
+                                 const qlx_exec_opts* opts, int64_t* order, int32_t* sweep_of, size_t cap,
+                                 size_t* n_out, qlx_plan_info* info) {
+    return guard([&] {
+        if (n && !instrs) fail(QLX_EINVAL, "null instruction stream");
+        qlx_exec_opts o{};
+        if (opts) o = *opts;
+        for (size_t i = 0; i < n; ++i) {
+            if (instrs[i].type == QLX_MEASURE) fail(QLX_EINVAL, "plans cover MEASURE-free streams");
+            validate_instr(instrs[i], num_qubits, meta, n_meta, pool_elems);
+        }
+        const Geometry geo = geometry_for(dtype, num_qubits, o);
+        std::unique_ptr<SweepPlan> p;
+        if ((int)num_qubits >= geo.w) p = build_plan(num_qubits, dtype, instrs, n, pool, meta, o);
+        const size_t cnt = p ? p->order.size() : 0;
+        if (n_out) *n_out = cnt;
+        for (size_t k = 0; p && k < cnt && k < cap; ++k) {
+            if (order) order[k] = p->order[k];
+            if (sweep_of) sweep_of[k] = p->order_sweep[k];
+        }
+        if (info) {
+            std::memset(info, 0, sizeof(*info));
+            info->window_qubits = (uint32_t)geo.w;
+            info->register_qubits = (uint32_t)geo.rb;
+            info->threads_per_group = 1u << geo.logt;
+            info->coalesce_qubits = (uint32_t)geo.c;
+            if (p) {
+                info->sweeps = p->sweeps.size();
+                info->phases = p->phases;
+                info->plain_steps = p->steps.size() - p->sweeps.size();
+                info->gates_in_sweeps = p->gates_in_sweeps;
+            }
+        }
+    });
+}

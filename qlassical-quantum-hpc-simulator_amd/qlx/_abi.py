@@ -87,6 +87,19 @@ class KernelTiming(C.Structure):
     ]
 
 
+class PlanInfo(C.Structure):
+    _fields_ = [
+        ("sweeps", C.c_uint64),
+        ("phases", C.c_uint64),
+        ("plain_steps", C.c_uint64),
+        ("gates_in_sweeps", C.c_uint64),
+        ("window_qubits", C.c_uint32),
+        ("register_qubits", C.c_uint32),
+        ("threads_per_group", C.c_uint32),
+        ("coalesce_qubits", C.c_uint32),
+    ]
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [
         ("hbm_bytes", C.c_uint64),
@@ -119,6 +132,9 @@ SIGNATURES = [
     ("qlx_state_copy", [C.c_void_p, C.c_void_p]),
     ("qlx_execute", [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
                      C.POINTER(ExecOpts), C.POINTER(C.c_size_t)]),
+    ("qlx_plan_windowed", [C.c_uint32, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,
+                           C.c_size_t, C.POINTER(ExecOpts), C.c_void_p, C.c_void_p, C.c_size_t,
+                           C.POINTER(C.c_size_t), C.POINTER(PlanInfo)]),
     ("qlx_last_exec_stats", [C.c_void_p, C.POINTER(ExecStats)]),
     ("qlx_measure_prob1", [C.c_void_p, C.c_uint32, C.POINTER(C.c_double)]),
     ("qlx_collapse", [C.c_void_p, C.c_uint32, C.c_int, C.c_double]),

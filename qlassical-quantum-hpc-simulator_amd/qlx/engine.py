@@ -38,6 +38,26 @@ def exec_opts(mode: str = "plain", window_qubits: int = 0, coalesce_qubits: int 
     return o
 
 
+def plan_windowed(prog: Program, dtype="c128", opts: Optional[_abi.ExecOpts] = None):
+    """Host-only windowed plan: (order, sweep_of, info) — see qlx_plan_windowed."""
+    L = _abi.lib()
+    o = opts if opts is not None else exec_opts("windowed")
+    ins = np.ascontiguousarray(prog.instrs)
+    pool = np.ascontiguousarray(prog.pool, np.complex128)
+    meta = np.ascontiguousarray(prog.meta)
+    n = C.c_size_t(0)
+    info = _abi.PlanInfo()
+    args = (prog.num_qubits, _dt(dtype), ins.ctypes.data if len(ins) else None, len(ins),
+            pool.ctypes.data if pool.size else None, pool.size, meta.ctypes.data if len(meta) else None, len(meta),
+            C.byref(o))
+    _abi.check(L.qlx_plan_windowed(*args, None, None, 0, C.byref(n), C.byref(info)))
+    order = np.zeros(n.value, np.int64)
+    sweep = np.zeros(n.value, np.int32)
+    _abi.check(L.qlx_plan_windowed(*args, order.ctypes.data, sweep.ctypes.data, n.value, C.byref(n),
+                                   C.byref(info)))
+    return order, sweep, {k: getattr(info, k) for k, _ in info._fields_}
+
+
 def device_info(device: int = 0) -> dict:
     info = _abi.DeviceInfo()
     _abi.check(_abi.lib().qlx_device_query(device, C.byref(info)))

@@ -1,0 +1,117 @@
+"""GPU parity of windowed execution (k_sweep): many gates per HBM sweep, same results.
+
+complex128: max abs error <= 1e-12 against the FP64 oracle running the original
+(unreordered, unwindowed) program. complex64: state infidelity <= 1e-6 against the
+complex128 result.
+"""
+import numpy as np
+import pytest
+
+import bruteforce as bf
+import oracle
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def run(gpu, prog, init=None, dtype="c128", mode="windowed"):
+    n = prog.num_qubits
+    st = gpu.State(n, dtype) if init is None else gpu.State(dtype=dtype, amps=init)
+    assert st.execute(prog, gpu.exec_opts(mode)) == len(prog)
+    st.sync()
+    out = st.to_host()
+    stats = st.last_stats()
+    st.close()
+    return out, stats
+
+
+@pytest.mark.parametrize("n,d,seed", [(12, 10, 1001), (13, 7, 5), (16, 12, 6), (20, 10, 1001)])
+@pytest.mark.parametrize("start", ["zero", "random"])
+def test_layered_windowed_vs_oracle(gpu, n, d, seed, start):
+    p = gpu.layered(n, d, seed)
+    init = None if start == "zero" else bf.random_state(n, np.random.default_rng(seed))
+    got, stats = run(gpu, p, init)
+    want, _ = oracle.execute(n, p.instrs, amps=init)
+    err = np.abs(got - want).max()
+    print(f"windowed {n}q d{d} {start}: sweeps {stats['sweeps_full']} err {err:.2e}")
+    assert err <= TOL
+    _, _, info = gpu.plan_windowed(p)
+    assert stats["sweeps_full"] == info["sweeps"]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_programs_windowed_vs_oracle(gpu, seed):
+    from test_oracle import random_program
+    rng = np.random.default_rng(700 + seed)
+    n = int(rng.integers(12, 17))
+    ins, pool, meta = random_program(n, 150, rng)
+    p = gpu.Program(n, ins, pool if pool is not None else np.zeros(0, complex),
+                    meta if meta is not None else np.zeros(0, gpu.META_DTYPE))
+    init = bf.random_state(n, rng)
+    got, _ = run(gpu, p, init)
+    want, _ = oracle.execute(n, ins, pool, meta, amps=init)
+    assert np.abs(got - want).max() <= TOL
+
+
+def test_global_swap_in_window_is_real_permutation(gpu):
+    n = 14
+    rng = np.random.default_rng(11)
+    c = gpu.Circuit(n)
+    for q in range(n):
+        c.ry(q, float(rng.uniform(0, 3)))
+    p = c.release()
+    gs = np.zeros(3, gpu.GATE_DTYPE)
+    gs["type"] = 0xE1
+    gs["flags_arity"] = 2
+    gs["qubits"] = [[13, 0, 255, 255], [12, 5, 255, 255], [0, 13, 255, 255]]
+    tail = gpu.Circuit(n).cx(0, 5).h(12).release().instrs
+    p.instrs = np.concatenate([p.instrs, gs, tail])
+    got, _ = run(gpu, p)
+    want, _ = oracle.execute(n, p.instrs, gswap="permute")
+    assert np.abs(got - want).max() <= TOL
+
+
+def test_inverse_identity_26q_windowed(gpu):
+    n = 26
+    p = gpu.layered(n, 16, 314)
+    init = bf.random_state(n, np.random.default_rng(3))
+    st = gpu.State(amps=init)
+    o = gpu.exec_opts("windowed")
+    st.execute(p, o)
+    st.execute(gpu.inverse_layered(p), o)
+    got = st.to_host()
+    st.close()
+    assert np.abs(got - init).max() <= TOL
+
+
+def test_closed_form_28q_windowed(gpu):
+    p = gpu.layered(28, 12, 2718)
+    got, stats = run(gpu, p)
+    th = p.instrs["payload"][p.instrs["type"] == 0x10].view(np.float32).astype(np.float64)
+    assert np.abs(got - np.exp(-0.5j * th.sum()) * 2.0 ** -14).max() <= TOL
+
+
+def test_windowed_matches_plain_bitwise_for_permutations(gpu):
+    n = 15
+    rng = np.random.default_rng(4)
+    c = gpu.Circuit(n)
+    for _ in range(200):
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        [lambda: c.cx(a, b), lambda: c.swap(a, b), lambda: c.x(a), lambda: c.y(b), lambda: c.z(a),
+         lambda: c.s(b)][int(rng.integers(6))]()
+    p = c.release()
+    init = bf.random_state(n, rng)
+    got, _ = run(gpu, p, init)
+    want, _ = oracle.execute(n, p.instrs, amps=init)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n,d,seed", [(14, 10, 400001), (20, 20, 400002)])
+def test_c64_windowed_fidelity(gpu, n, d, seed):
+    p = gpu.layered(n, d, seed)
+    init = bf.random_state(n, np.random.default_rng(seed))
+    ref, _ = run(gpu, p, init, "c128", "plain")
+    got, _ = run(gpu, p, init, "c64", "windowed")
+    inf = 1.0 - abs(np.vdot(ref, got)) ** 2
+    print(f"c64 windowed {n}q: infidelity {inf:.2e}, max abs {np.abs(ref - got).max():.2e}")
+    assert inf <= 1e-6

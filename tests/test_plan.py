@@ -1,0 +1,67 @@
+"""Host-only checks of the windowed planner (qlx_plan_windowed): the execution order
+it produces only moves gates past gates on disjoint qubits, so running the reordered
+stream on the CPU oracle reproduces the original program."""
+import numpy as np
+import pytest
+
+import bruteforce as bf
+import oracle
+import qlx
+from test_oracle import random_program
+
+
+def reordered(prog, order):
+    return prog.instrs[order]
+
+
+@pytest.mark.parametrize("n,d,seed", [(12, 6, 1), (14, 10, 2), (17, 8, 3)])
+def test_layered_plan_is_valid_reordering(n, d, seed):
+    p = qlx.layered(n, d, seed)
+    order, sweep, info = qlx.plan_windowed(p)
+    assert sorted(order.tolist()) == list(range(len(p)))
+    assert info["window_qubits"] == 12 and info["sweeps"] >= 1
+    init = bf.random_state(n, np.random.default_rng(seed))
+    a, _ = oracle.execute(n, p.instrs, amps=init)
+    b, _ = oracle.execute(n, reordered(p, order), amps=init)
+    np.testing.assert_allclose(b, a, atol=1e-12)
+    # within a sweep, gates only touch that sweep's W qubits
+    for s in set(sweep.tolist()) - {-1}:
+        qs = set()
+        for g in p.instrs[order[sweep == s]]:
+            k = 1 if g["type"] < 0x20 else 2
+            qs |= set(int(x) for x in g["qubits"][:k])
+        assert len(qs) <= info["window_qubits"]
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_random_program_plan_is_valid_reordering(seed, dtype):
+    rng = np.random.default_rng(900 + seed)
+    n = int(rng.integers(13, 16))
+    ins, pool, meta = random_program(n, 120, rng)
+    prog = qlx.Program(n, ins, pool if pool is not None else np.zeros(0, complex),
+                       meta if meta is not None else np.zeros(0, qlx.META_DTYPE))
+    order, sweep, info = qlx.plan_windowed(prog, dtype)
+    assert sorted(order.tolist()) == list(range(len(ins)))
+    init = bf.random_state(n, rng)
+    a, _ = oracle.execute(n, ins, pool, meta, amps=init)
+    b, _ = oracle.execute(n, ins[order], pool, meta, amps=init)
+    np.testing.assert_allclose(b, a, atol=1e-12)
+    # 3-qubit and dense k>=3 gates run as their own kernels
+    for k, g in zip(sweep, ins[order]):
+        if g["type"] in (0x30, 0x31) or (g["type"] == 0x80 and (g["flags_arity"] & 15) >= 3):
+            assert k == -1
+
+
+def test_config2_plan_shape():
+    p = qlx.layered(30, 20, 1002)
+    order, sweep, info = qlx.plan_windowed(p)
+    print(info)
+    assert info["gates_in_sweeps"] == 920 and info["plain_steps"] == 0
+    # far fewer HBM sweeps than the 775 pass-equivalents of one-kernel-per-gate execution
+    assert info["sweeps"] < 120
+
+
+def test_small_states_plan_nothing():
+    order, sweep, info = qlx.plan_windowed(qlx.layered(8, 3, 1))
+    assert len(order) == 0 and info["sweeps"] == 0
