@@ -36,7 +36,8 @@ constexpr int kMaxR = 32;
 enum OpKind : uint8_t { OP_U2 = 1, OP_RX = 2, OP_H = 3, OP_CX = 4, OP_SWAP = 5, OP_U4 = 6 };
 
 struct DevOp {
-    uint8_t kind, b0, b1, pad;
+    uint16_t key;   // (kind << 8) | (register bit 0 << 4) | register bit 1
+    uint16_t pad;
     uint32_t coef;  // index into the coefficient array (doubles)
 };
 
@@ -166,62 +167,49 @@ __device__ __forceinline__ void g_u4(A (&v)[R], const double* __restrict__ m) {
     }
 }
 
-template <int RB, class F>
-__device__ __forceinline__ void with_bit(int b, F&& f) {
-    switch (b) {
-        case 0: f(std::integral_constant<int, 0>{}); break;
-        case 1: if constexpr (RB > 1) f(std::integral_constant<int, 1>{}); break;
-        case 2: if constexpr (RB > 2) f(std::integral_constant<int, 2>{}); break;
-        case 3: if constexpr (RB > 3) f(std::integral_constant<int, 3>{}); break;
-        case 4: if constexpr (RB > 4) f(std::integral_constant<int, 4>{}); break;
-        default: break;
-    }
-}
+// One flat switch over (kind, bit0, bit1): every case indexes the register array with
+// compile-time constants only, so the array stays in VGPRs. (Nested or lambda-based
+// dispatch made the compiler keep the array in scratch memory.)
+#define QLX_K1(K, B, CALL)                     \
+    case ((K) << 8) | ((B) << 4):              \
+        if constexpr ((B) < RB) { CALL; }       \
+        break;
+#define QLX_K2(K, B0, B1, CALL)                \
+    case ((K) << 8) | ((B0) << 4) | (B1):      \
+        if constexpr ((B0) < RB && (B1) < RB) { CALL; } \
+        break;
+#define QLX_ALL1(K, F) QLX_K1(K, 0, F(0)) QLX_K1(K, 1, F(1)) QLX_K1(K, 2, F(2)) QLX_K1(K, 3, F(3)) QLX_K1(K, 4, F(4))
+#define QLX_ROW2(K, B0, F)                                                                      \
+    QLX_K2(K, B0, (B0 + 1) % 5, F(B0, (B0 + 1) % 5)) QLX_K2(K, B0, (B0 + 2) % 5, F(B0, (B0 + 2) % 5)) \
+    QLX_K2(K, B0, (B0 + 3) % 5, F(B0, (B0 + 3) % 5)) QLX_K2(K, B0, (B0 + 4) % 5, F(B0, (B0 + 4) % 5))
+#define QLX_ALL2(K, F) QLX_ROW2(K, 0, F) QLX_ROW2(K, 1, F) QLX_ROW2(K, 2, F) QLX_ROW2(K, 3, F) QLX_ROW2(K, 4, F)
 
 template <class A, int R, int RB>
 __device__ __forceinline__ void apply_op(A (&v)[R], const DevOp op, const double* __restrict__ coef) {
     using T = typename Real<A>::T;
     const double* m = coef + op.coef;
-    switch (op.kind) {
-        case OP_RX: {
-            const T c = (T)m[0], s = (T)m[1];
-            with_bit<RB>(op.b0, [&](auto B) { g_rx<decltype(B)::value>(v, c, s); });
-            break;
-        }
-        case OP_H: {
-            const T r = (T)m[0];
-            with_bit<RB>(op.b0, [&](auto B) { g_h<decltype(B)::value>(v, r); });
-            break;
-        }
-        case OP_U2:
-            with_bit<RB>(op.b0, [&](auto B) { g_u2<decltype(B)::value>(v, m); });
-            break;
-        case OP_CX:
-            with_bit<RB>(op.b0, [&](auto B0) {
-                with_bit<RB>(op.b1, [&](auto B1) {
-                    if constexpr (decltype(B0)::value != decltype(B1)::value)
-                        g_cx<decltype(B0)::value, decltype(B1)::value>(v);
-                });
-            });
-            break;
-        case OP_SWAP:
-            with_bit<RB>(op.b0, [&](auto B0) {
-                with_bit<RB>(op.b1, [&](auto B1) {
-                    if constexpr (decltype(B0)::value != decltype(B1)::value)
-                        g_swap<decltype(B0)::value, decltype(B1)::value>(v);
-                });
-            });
-            break;
-        case OP_U4:
-            with_bit<RB>(op.b0, [&](auto B0) {
-                with_bit<RB>(op.b1, [&](auto B1) {
-                    if constexpr (decltype(B0)::value != decltype(B1)::value)
-                        g_u4<decltype(B0)::value, decltype(B1)::value>(v, m);
-                });
-            });
-            break;
+    const T c0 = (T)m[0], c1 = (T)m[1];
+#define F_RX(B) g_rx<B>(v, c0, c1)
+#define F_H(B) g_h<B>(v, c0)
+#define F_U2(B) g_u2<B>(v, m)
+#define F_CX(B0, B1) g_cx<B0, B1>(v)
+#define F_SW(B0, B1) g_swap<B0, B1>(v)
+#define F_U4(B0, B1) g_u4<B0, B1>(v, m)
+    switch (op.key) {
+        QLX_ALL1(OP_RX, F_RX)
+        QLX_ALL1(OP_H, F_H)
+        QLX_ALL1(OP_U2, F_U2)
+        QLX_ALL2(OP_CX, F_CX)
+        QLX_ALL2(OP_SWAP, F_SW)
+        QLX_ALL2(OP_U4, F_U4)
         default: break;
     }
+#undef F_RX
+#undef F_H
+#undef F_U2
+#undef F_CX
+#undef F_SW
+#undef F_U4
 }
 
 template <class A, int LOGT, int RB>
@@ -732,12 +720,13 @@ static std::unique_ptr<SweepPlan> build_plan(uint32_t nq, int dtype, const qlx_g
             for (const PlanOp& op : ph.ops) {
                 DevOp o{};
                 o.coef = (uint32_t)plan->coef.size();
-                fill_coef(plan->coef, instrs[op.src], pool, meta, o.kind);
-                o.b0 = (uint8_t)op.l0;
-                o.b1 = (uint8_t)(op.l1 < 0 ? 0 : op.l1);
-                if (o.kind == OP_CX || o.kind == OP_SWAP || o.kind == OP_U4) {
-                    if (op.l1 < 0 || op.l0 == op.l1) fail(QLX_EHIP, "sweep planner: bad two-qubit operands");
-                }
+                uint8_t kind = 0;
+                fill_coef(plan->coef, instrs[op.src], pool, meta, kind);
+                const bool two = kind == OP_CX || kind == OP_SWAP || kind == OP_U4;
+                if (two && (op.l1 < 0 || op.l0 == op.l1)) fail(QLX_EHIP, "sweep planner: bad two-qubit operands");
+                if (op.l0 < 0 || op.l0 >= geo.rb || (two && op.l1 >= geo.rb))
+                    fail(QLX_EHIP, "sweep planner: operand is not a register bit");
+                o.key = (uint16_t)((kind << 8) | (op.l0 << 4) | (two ? op.l1 : 0));
                 plan->dop.push_back(o);
             }
         }
