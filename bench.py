@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--mode", choices=["auto", "plain", "windowed"], default="auto")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one k_copy of the whole state (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -168,6 +170,15 @@ def main():
             "bytes_per_launch": tot_bytes / launches,
         },
     }
+    if args.calib_copy:
+        dst = qlx.State(nq, "c128")
+        dst.timing(True)
+        dst.copy_from(st)
+        t = dst.timing_read()["k_copy"]
+        out["copy_reference"] = {"kernel": "k_copy", "bytes": t[2], "ms": round(t[1], 4),
+                                 "gbps": round(t[2] / (t[1] / 1e3) / 1e9, 1)}
+        log(f"bench: k_copy {out['copy_reference']}")
+        dst.close()
     st.close()
     if not args.no_cpu:
         log("bench: cpu baseline ...")
