@@ -207,6 +207,33 @@ int qlx_timing_enable(qlx_state* st, int on);
 int qlx_timing_reset(qlx_state* st);
 int qlx_timing_read(qlx_state* st, qlx_kernel_timing* out, size_t cap, size_t* n_out);
 
+/* ---- Middleware::transpile_gpu (host C++; reference middleware.hpp:466-481, whose
+ *      GPU branch :474-476 calls the empty stub :866) ----------------------------
+ * Same pass pipeline as transpile_cpu_openmp (middleware.hpp:488-524): DAG build,
+ * pass_gate_fusion (:526-753), pass_qubit_windowing (:759-860), DAG release. */
+typedef struct qlx_transpile_opts {
+    uint32_t enable_gate_fusion;  /* TranspilerConfig::enable_gate_fusion */
+    uint32_t max_fusion_width;    /* 1..5; 0 = 4 (the reference's MAX_GATE_ARITY, middleware.hpp:137) */
+    uint32_t fuse_rotations;      /* also fuse RX/RY/RZ/SWAP/CCX/CSWAP (not fusable in the reference, :442-448) */
+    uint32_t enable_windowing;    /* TranspilerConfig::enable_hw_awareness */
+    uint32_t k_safe;              /* TranspilerConfig::k_safe; 0 = qlx_gpu_safe_qubit_limit */
+    uint32_t numa_penalty;        /* ThresholdCostModel (middleware.hpp:63-119); 0 = 10 */
+    uint32_t swap_penalty;
+    uint32_t restore_layout;      /* append GLOBAL_SWAPs restoring logical qubit order */
+    int32_t  dtype;               /* precision the window is sized for (QLX_C128 / QLX_C64) */
+    uint32_t reserved[7];
+} qlx_transpile_opts;
+typedef struct qlx_program qlx_program;
+int qlx_transpile(uint32_t num_qubits, const qlx_gate_instr* instrs, size_t n,
+                  const double* pool_c128, size_t pool_elems, const qlx_matrix_entry* meta, size_t n_meta,
+                  const qlx_transpile_opts* opts, qlx_program** out);
+int qlx_program_sizes(const qlx_program* p, size_t* n_instr, size_t* pool_elems, size_t* n_meta);
+int qlx_program_copy(const qlx_program* p, qlx_gate_instr* instrs, double* pool_c128, qlx_matrix_entry* meta);
+int qlx_program_destroy(qlx_program* p);
+/* GPU HardwareTopology::get_safe_qubit_limit (reference hw_topology.hpp:47-53): the number
+ * of qubits one work-group's window holds for this precision (device < 0: no query). */
+int qlx_gpu_safe_qubit_limit(int device, int dtype, uint32_t* k_safe);
+
 /* ---- host-side helpers (CPU; no GPU needed) ------------------------------ */
 /* Seeded layered h/rx/cx circuit: H on every qubit, then `depth` layers of
  * RX(theta) on every qubit (theta = (mt19937_64() >> 11) * 2^-53 * 2*pi, stored

@@ -100,6 +100,21 @@ class PlanInfo(C.Structure):
     ]
 
 
+class TranspileOpts(C.Structure):
+    _fields_ = [
+        ("enable_gate_fusion", C.c_uint32),
+        ("max_fusion_width", C.c_uint32),
+        ("fuse_rotations", C.c_uint32),
+        ("enable_windowing", C.c_uint32),
+        ("k_safe", C.c_uint32),
+        ("numa_penalty", C.c_uint32),
+        ("swap_penalty", C.c_uint32),
+        ("restore_layout", C.c_uint32),
+        ("dtype", C.c_int32),
+        ("reserved", C.c_uint32 * 7),
+    ]
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [
         ("hbm_bytes", C.c_uint64),
@@ -142,6 +157,12 @@ SIGNATURES = [
     ("qlx_timing_enable", [C.c_void_p, C.c_int]),
     ("qlx_timing_reset", [C.c_void_p]),
     ("qlx_timing_read", [C.c_void_p, C.POINTER(KernelTiming), C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("qlx_transpile", [C.c_uint32, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
+                       C.POINTER(TranspileOpts), C.POINTER(C.c_void_p)]),
+    ("qlx_program_sizes", [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    ("qlx_program_copy", [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("qlx_program_destroy", [C.c_void_p]),
+    ("qlx_gpu_safe_qubit_limit", [C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
     ("qlx_gen_layered", [C.c_uint32, C.c_uint32, C.c_uint64, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
 ]
 

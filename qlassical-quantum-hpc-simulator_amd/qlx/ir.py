@@ -162,12 +162,13 @@ class Circuit:
     def cswap(self, c, a, b): return self._3q(CSWAP, c, a, b)
     fredkin = cswap
 
-    def unitary(self, qubits, matrix) -> "Circuit":
-        """QuantumCircuit::unitary (frontend.hpp:340-363): arity 1..4 in the reference;
-        arity 5 is accepted here (the fifth qubit is carried in the pad byte)."""
+    def unitary(self, qubits, matrix, max_arity: int = 4) -> "Circuit":
+        """QuantumCircuit::unitary (frontend.hpp:340-363): arity 1..4 as in the reference.
+        max_arity=5 builds the width-5 form the GPU fusion pass emits (fifth qubit in the
+        pad byte)."""
         k = len(qubits)
-        if k == 0 or k > 5:
-            raise ValueError(f"unitary: arity must be 1-5, got {k}")
+        if k == 0 or k > max_arity:
+            raise ValueError(f"unitary: arity must be 1-{max_arity}, got {k}")
         midx = self._prog.store(matrix, k)
         for q in qubits:
             self._validate(q)
@@ -204,6 +205,41 @@ def layered(num_qubits: int, depth: int, seed: int) -> Program:
     arr = np.zeros(n.value, GATE_DTYPE)
     _abi.check(L.qlx_gen_layered(num_qubits, depth, seed, arr.ctypes.data, n.value, C.byref(n)))
     return Program(num_qubits, arr)
+
+
+def transpile_gpu(prog: Program, fusion: bool = True, max_width: int = 4, rotations: bool = True,
+                  windowing: bool = False, k_safe: int = 0, numa_penalty: int = 10, swap_penalty: int = 1,
+                  restore: bool = True, dtype: int = 0) -> Program:
+    """Middleware::transpile_gpu through the library (qlx_transpile)."""
+    L = _abi.lib()
+    o = _abi.TranspileOpts()
+    o.enable_gate_fusion, o.max_fusion_width, o.fuse_rotations = int(fusion), max_width, int(rotations)
+    o.enable_windowing, o.k_safe, o.numa_penalty, o.swap_penalty = int(windowing), k_safe, numa_penalty, swap_penalty
+    o.restore_layout, o.dtype = int(restore), dtype
+    ins = np.ascontiguousarray(prog.instrs)
+    pool = np.ascontiguousarray(prog.pool, np.complex128)
+    meta = np.ascontiguousarray(prog.meta)
+    h = C.c_void_p()
+    _abi.check(L.qlx_transpile(prog.num_qubits, ins.ctypes.data if len(ins) else None, len(ins),
+                               pool.ctypes.data if pool.size else None, pool.size,
+                               meta.ctypes.data if len(meta) else None, len(meta), C.byref(o), C.byref(h)))
+    try:
+        ni, npool, nm = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        _abi.check(L.qlx_program_sizes(h, C.byref(ni), C.byref(npool), C.byref(nm)))
+        out = Program(prog.num_qubits, np.zeros(ni.value, GATE_DTYPE), np.zeros(npool.value, np.complex128),
+                      np.zeros(nm.value, META_DTYPE))
+        _abi.check(L.qlx_program_copy(h, out.instrs.ctypes.data if ni.value else None,
+                                      out.pool.ctypes.data if npool.value else None,
+                                      out.meta.ctypes.data if nm.value else None))
+    finally:
+        L.qlx_program_destroy(h)
+    return out
+
+
+def gpu_safe_qubit_limit(dtype: int = 0, device: int = -1) -> int:
+    k = C.c_uint32()
+    _abi.check(_abi.lib().qlx_gpu_safe_qubit_limit(device, dtype, C.byref(k)))
+    return k.value
 
 
 def inverse_layered(prog: Program) -> Program:

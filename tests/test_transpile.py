@@ -1,0 +1,136 @@
+"""Middleware::transpile_gpu (qlx_transpile): the reference's middleware known answers
+(tests/middleware/test_middleware.cpp) and numerical equivalence on the CPU oracle."""
+import numpy as np
+import pytest
+
+import bruteforce as bf
+import oracle
+import qlx
+from qlx import ir
+from test_oracle import random_program
+
+CPU_STYLE = dict(rotations=False, max_width=4, restore=False)
+
+
+def names(p):
+    return [n for n, _ in p.describe()]
+
+
+# ---- test_middleware.cpp:60-92 (fusion heuristic) ---------------------------------------
+def test_fusion_skips_two_standard_gates():
+    p = qlx.transpile_gpu(qlx.Circuit(2).h(0).x(1).release(), **CPU_STYLE)
+    assert names(p) == ["H", "X"]
+
+
+def test_fusion_three_connected_gates_into_one_block():
+    p = qlx.transpile_gpu(qlx.Circuit(2).h(0).cx(0, 1).x(1).release(), **CPU_STYLE)
+    assert names(p) == ["FUSED_BLOCK"] and ir.arity(p.instrs[0]) == 2
+    want = bf.expand(np.array([[0, 1], [1, 0]]), [1], 2) @ bf.expand(
+        bf.gate_matrix(qlx.Circuit(2).cx(0, 1).release().instrs[0])[0], [0, 1], 2) @ bf.expand(
+        np.array([[1, 1], [1, -1]]) / np.sqrt(2), [0], 2)
+    np.testing.assert_allclose(p.pool.reshape(4, 4), want, atol=1e-15)
+
+
+def test_reference_cannot_fuse_rotations_but_gpu_pass_can():
+    c = qlx.Circuit(2).h(0).rx(0, 0.3).cx(0, 1).rx(1, 1.1).release()
+    assert names(qlx.transpile_gpu(c, **CPU_STYLE)) == ["H", "RX", "CX", "RX"]
+    p = qlx.transpile_gpu(c, rotations=True, max_width=2)
+    assert names(p) == ["FUSED_BLOCK"]
+
+
+# ---- test_middleware.cpp:94-140, 164-235, 296-323 (windowing stream shape) ---------------
+def windowed(k_safe, numa=1, swap=0, restore=False):
+    p = qlx.Circuit(10).x(0).x(1).cx(8, 9).release()
+    return qlx.transpile_gpu(p, fusion=False, windowing=True, k_safe=k_safe, numa_penalty=numa,
+                             swap_penalty=swap, restore=restore)
+
+
+def test_windowing_tlb_hazard_stream():
+    p = windowed(2, 1, 0)
+    d = p.describe()
+    assert [n for n, _ in d] == ["X", "X", "GLOBAL_SWAP", "GLOBAL_SWAP", "CX"]
+    assert d[0][1] == [0] and d[1][1] == [1]
+    for k in (2, 3):
+        assert d[k][1][0] in (8, 9) and d[k][1][1] in (0, 1)
+    assert d[2][1][0] != d[3][1][0] and d[2][1][1] != d[3][1][1]
+    assert set(d[4][1]) == {0, 1}
+
+
+def test_windowing_penalties():
+    assert names(windowed(2, 1, 100)) == ["X", "X", "CX"]                     # :168-189
+    assert names(windowed(2, 10, 1))[2:] == ["GLOBAL_SWAP", "GLOBAL_SWAP", "CX"]  # :191-212
+    assert len(windowed(16, 10, 1)) == 3                                        # :304-312
+    assert len(windowed(2, 10, 1)) == 5                                         # :314-322
+
+
+def test_windowing_restore_appends_inverse_layout():
+    p = windowed(2, 1, 0, restore=True)
+    assert names(p)[:5] == ["X", "X", "GLOBAL_SWAP", "GLOBAL_SWAP", "CX"]
+    assert all(n == "GLOBAL_SWAP" for n in names(p)[5:]) and len(p) > 5
+    # with real swaps + restore the program computes the logical answer |..0011> = index 3
+    s, _ = oracle.execute(10, p.instrs, gswap="permute")
+    assert abs(s[3] - 1) < 1e-12
+    # the reference's own semantics (no-op swap, no restore) give index 1 (a wrong state)
+    s, _ = oracle.execute(10, windowed(2, 1, 0).instrs, gswap="noop")
+    assert abs(s[1] - 1) < 1e-12
+
+
+def test_gpu_safe_qubit_limit():
+    assert qlx.gpu_safe_qubit_limit(0) == 12
+    assert qlx.gpu_safe_qubit_limit(1) == 13
+
+
+# ---- numerical equivalence -------------------------------------------------------------------
+@pytest.mark.parametrize("width", [2, 3, 4, 5])
+@pytest.mark.parametrize("seed", range(4))
+def test_fusion_preserves_the_state(width, seed):
+    rng = np.random.default_rng(seed * 10 + width)
+    n = int(rng.integers(5, 10))
+    ins, pool, meta = random_program(n, 60, rng)
+    prog = qlx.Program(n, ins, pool if pool is not None else np.zeros(0, complex),
+                       meta if meta is not None else np.zeros(0, qlx.META_DTYPE))
+    fused = qlx.transpile_gpu(prog, max_width=width, rotations=True)
+    assert len(fused) <= len(prog)
+    for g in fused.instrs:
+        assert ir.arity(g) <= max(width, 5 if g["type"] == 0x80 else 0) or g["type"] != 0xE0
+    init = bf.random_state(n, rng)
+    a, _ = oracle.execute(n, ins, pool, meta, amps=init)
+    b, _ = oracle.execute(n, fused.instrs, fused.pool, fused.meta, amps=init)
+    np.testing.assert_allclose(b, a, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,d,seed,width", [(8, 6, 1, 5), (10, 10, 1001, 4), (12, 5, 3, 3)])
+def test_layered_fusion_width(n, d, seed, width):
+    p = qlx.layered(n, d, seed)
+    f = qlx.transpile_gpu(p, max_width=width, rotations=True)
+    blocks = [g for g in f.instrs if g["type"] == 0xE0]
+    assert blocks and max(ir.arity(g) for g in blocks) == width
+    assert len(f) < len(p) // 2
+    init = bf.random_state(n, np.random.default_rng(seed))
+    a, _ = oracle.execute(n, p.instrs, amps=init)
+    b, _ = oracle.execute(n, f.instrs, f.pool, f.meta, amps=init)
+    np.testing.assert_allclose(b, a, atol=1e-12)
+    np.testing.assert_allclose(bf.run(n, f.instrs, f.pool, f.meta, psi=init), a, atol=1e-12)
+
+
+@pytest.mark.parametrize("k_safe", [3, 5, 8])
+def test_windowing_with_restore_preserves_the_state(k_safe):
+    n = 10
+    p = qlx.layered(n, 6, 77)
+    w = qlx.transpile_gpu(p, fusion=False, windowing=True, k_safe=k_safe, numa_penalty=10, swap_penalty=1,
+                          restore=True)
+    assert (w.instrs["type"] == 0xE1).sum() > 0
+    init = bf.random_state(n, np.random.default_rng(k_safe))
+    a, _ = oracle.execute(n, p.instrs, amps=init)
+    b, _ = oracle.execute(n, w.instrs, amps=init, gswap="permute")
+    np.testing.assert_allclose(b, a, atol=1e-12)
+
+
+def test_fusion_then_windowing_preserves_the_state():
+    n = 11
+    p = qlx.layered(n, 8, 5)
+    w = qlx.transpile_gpu(p, max_width=3, rotations=True, windowing=True, k_safe=6, restore=True)
+    init = bf.random_state(n, np.random.default_rng(1))
+    a, _ = oracle.execute(n, p.instrs, amps=init)
+    b, _ = oracle.execute(n, w.instrs, w.pool, w.meta, amps=init, gswap="permute")
+    np.testing.assert_allclose(b, a, atol=1e-12)
