@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--depth", type=int, default=20)
     ap.add_argument("--seed", type=int, default=1002)
     ap.add_argument("--mode", choices=["auto", "plain", "windowed"], default="auto")
+    ap.add_argument("--window", type=int, default=0, help="window qubits (0 = default for the precision)")
+    ap.add_argument("--regs", type=int, default=0, help="register qubits per work-item (0 = default)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
@@ -106,7 +108,7 @@ def main():
     st = qlx.State(nq, "c128")
     mode, opts = None, None
     for m in modes:
-        o = qlx.exec_opts(m)
+        o = qlx.exec_opts(m, window_qubits=args.window, register_qubits=args.regs)
         try:
             st.execute(prog.slice(0, 1), o)
             mode, opts = m, o
@@ -157,6 +159,7 @@ def main():
             "workload": f"layered h/rx/cx circuit, {nq} qubits, depth {depth}, seed {args.seed}, complex128 "
                         f"({(16 << nq) / 2**30:.0f} GiB state), {len(prog)} gates; {mode} execution",
             "qubits": nq, "depth": depth, "gates": len(prog), "mode": mode,
+            "window_qubits": args.window or None, "register_qubits": args.regs or None,
             "sweeps_per_step": {"full": stats["sweeps_full"], "half": stats["sweeps_half"],
                                 "quarter": stats["sweeps_quarter"]},
             "circuit_ms": round(ms, 3),

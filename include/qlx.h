@@ -87,7 +87,10 @@ typedef struct qlx_exec_opts {
     uint32_t window_qubits;   /* 0 = default for the precision (c128: 12, c64: 13) */
     uint32_t coalesce_qubits; /* low qubits always kept in the window; 0 = default (3) */
     uint32_t gswap_mode;      /* QLX_GSWAP_PERMUTE | QLX_GSWAP_NOOP */
-    uint32_t reserved[4];
+    uint32_t register_qubits; /* qubits per work-item's registers in a window; 0 = default
+                                 (c128: 4, c64: 5). Built (window, register) pairs:
+                                 c128 (12,4) (12,5) (11,4) (11,3); c64 (13,5) (12,5) (12,4) */
+    uint32_t reserved[3];
 } qlx_exec_opts;
 
 /* Statistics of the last qlx_execute on a state (sweep accounting). */

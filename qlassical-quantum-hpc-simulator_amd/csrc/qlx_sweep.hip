@@ -38,7 +38,8 @@ enum OpKind : uint8_t { OP_U2 = 1, OP_RX = 2, OP_H = 3, OP_CX = 4, OP_SWAP = 5, 
 struct DevOp {
     uint16_t key;   // (kind << 8) | (register bit 0 << 4) | register bit 1
     uint16_t pad;
-    uint32_t coef;  // index into the coefficient array (doubles)
+    uint32_t coef;  // index into the coefficient array (doubles): U2/U4 matrices
+    double c0, c1;  // inline coefficients: RX (cos, sin), H (1/sqrt 2)
 };
 
 struct DevPhase {
@@ -188,7 +189,7 @@ template <class A, int R, int RB>
 __device__ __forceinline__ void apply_op(A (&v)[R], const DevOp op, const double* __restrict__ coef) {
     using T = typename Real<A>::T;
     const double* m = coef + op.coef;
-    const T c0 = (T)m[0], c1 = (T)m[1];
+    const T c0 = (T)op.c0, c1 = (T)op.c1;
 #define F_RX(B) g_rx<B>(v, c0, c1)
 #define F_H(B) g_h<B>(v, c0)
 #define F_U2(B) g_u2<B>(v, m)
@@ -273,20 +274,21 @@ struct Geometry {
     int logt, rb, w, c;
 };
 
+// (window qubits, register qubits) pairs with a compiled k_sweep instantiation.
+static bool geometry_built(int dtype, int w, int rb) {
+    if (dtype == QLX_C128) return (w == 12 && (rb == 4 || rb == 5)) || (w == 11 && (rb == 4 || rb == 3));
+    return (w == 13 && rb == 5) || (w == 12 && (rb == 5 || rb == 4));
+}
+
 static Geometry geometry_for(int dtype, uint32_t nq, const qlx_exec_opts& o) {
     Geometry g{};
-    if (dtype == QLX_C128) {
-        g.logt = 8;
-        g.rb = 4;
-    } else {
-        g.logt = 8;
-        g.rb = 5;
-    }
-    g.w = g.logt + g.rb;
+    g.w = o.window_qubits ? (int)o.window_qubits : (dtype == QLX_C128 ? 12 : 13);
+    g.rb = o.register_qubits ? (int)o.register_qubits : (dtype == QLX_C128 ? 4 : 5);
+    if (!geometry_built(dtype, g.w, g.rb))
+        fail(QLX_EUNSUPPORTED, "window " + std::to_string(g.w) + " with " + std::to_string(g.rb) +
+                                   " register qubits is not built for this precision");
+    g.logt = g.w - g.rb;
     g.c = o.coalesce_qubits ? (int)o.coalesce_qubits : 3;
-    if (o.window_qubits && (int)o.window_qubits != g.w)
-        fail(QLX_EUNSUPPORTED, "window_qubits " + std::to_string(o.window_qubits) + " not built (this precision uses " +
-                                   std::to_string(g.w) + ")");
     if (g.c < 0 || g.c > g.logt) fail(QLX_EINVAL, "coalesce_qubits out of range");
     (void)nq;
     return g;
@@ -727,6 +729,10 @@ static std::unique_ptr<SweepPlan> build_plan(uint32_t nq, int dtype, const qlx_g
                 if (op.l0 < 0 || op.l0 >= geo.rb || (two && op.l1 >= geo.rb))
                     fail(QLX_EHIP, "sweep planner: operand is not a register bit");
                 o.key = (uint16_t)((kind << 8) | (op.l0 << 4) | (two ? op.l1 : 0));
+                if (kind == OP_RX || kind == OP_H) {
+                    o.c0 = plan->coef[o.coef];
+                    o.c1 = kind == OP_RX ? plan->coef[o.coef + 1] : 0.0;
+                }
                 plan->dop.push_back(o);
             }
         }
@@ -827,10 +833,19 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
         }
         {
             TimedLaunch tl(st, "k_sweep", pass);
-            if (st->dtype == QLX_C128)
-                launch_sweep<double2, 8, 4>(st, dev, cp.off_sw, cp.off_ph, cp.off_op, cp.off_cf, s.sweep_index);
-            else
-                launch_sweep<float2, 8, 5>(st, dev, cp.off_sw, cp.off_ph, cp.off_op, cp.off_cf, s.sweep_index);
+            const int key = (st->dtype == QLX_C128 ? 1000 : 0) + geo.w * 10 + geo.rb;
+            const size_t a = cp.off_sw, b = cp.off_ph, c = cp.off_op, d = cp.off_cf;
+            const int si = s.sweep_index;
+            switch (key) {
+                case 1124: launch_sweep<double2, 8, 4>(st, dev, a, b, c, d, si); break;
+                case 1125: launch_sweep<double2, 7, 5>(st, dev, a, b, c, d, si); break;
+                case 1114: launch_sweep<double2, 7, 4>(st, dev, a, b, c, d, si); break;
+                case 1113: launch_sweep<double2, 8, 3>(st, dev, a, b, c, d, si); break;
+                case 135: launch_sweep<float2, 8, 5>(st, dev, a, b, c, d, si); break;
+                case 125: launch_sweep<float2, 7, 5>(st, dev, a, b, c, d, si); break;
+                case 124: launch_sweep<float2, 8, 4>(st, dev, a, b, c, d, si); break;
+                default: fail(QLX_EUNSUPPORTED, "sweep geometry not built");
+            }
         }
         stats.sweeps_full++;
         stats.algorithmic_bytes += pass;

@@ -64,7 +64,8 @@ class ExecOpts(C.Structure):
         ("window_qubits", C.c_uint32),
         ("coalesce_qubits", C.c_uint32),
         ("gswap_mode", C.c_uint32),
-        ("reserved", C.c_uint32 * 4),
+        ("register_qubits", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
     ]
 
 

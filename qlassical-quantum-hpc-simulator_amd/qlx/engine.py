@@ -29,10 +29,11 @@ def _dt(dtype) -> int:
 
 
 def exec_opts(mode: str = "plain", window_qubits: int = 0, coalesce_qubits: int = 0,
-              gswap: str = "permute") -> _abi.ExecOpts:
+              gswap: str = "permute", register_qubits: int = 0) -> _abi.ExecOpts:
     o = _abi.ExecOpts()
     o.mode = {"plain": _abi.QLX_EXEC_PLAIN, "windowed": _abi.QLX_EXEC_WINDOWED}[mode]
     o.window_qubits = window_qubits
+    o.register_qubits = register_qubits
     o.coalesce_qubits = coalesce_qubits
     o.gswap_mode = {"permute": _abi.QLX_GSWAP_PERMUTE, "noop": _abi.QLX_GSWAP_NOOP}[gswap]
     return o
