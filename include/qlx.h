@@ -237,6 +237,42 @@ int qlx_program_destroy(qlx_program* p);
  * of qubits one work-group's window holds for this precision (device < 0: no query). */
 int qlx_gpu_safe_qubit_limit(int device, int dtype, uint32_t* k_safe);
 
+/* ---- batched execution (no reference counterpart: the reference runs one circuit
+ *      per engine) ---------------------------------------------------------------
+ * Many independent circuits, each with its own state in one HBM arena. States start
+ * from |0...0>. Circuits must have >= 10 qubits and contain only gates that run inside
+ * a window sweep (1-qubit gates, CX, SWAP, dense gates on <= 2 qubits). */
+#define QLX_BATCH_PERSISTENT         0  /* one persistent launch over a queue of (circuit, sweep, chunk) tasks */
+#define QLX_BATCH_PER_CIRCUIT        1  /* every circuit as its own stream of one-kernel-per-gate launches */
+#define QLX_BATCH_PER_CIRCUIT_GRAPH  2  /* the same launches captured once into a hipGraph, then replayed */
+#define QLX_BATCH_LEVELS             3  /* one launch per sweep level over all circuits */
+typedef struct qlx_batch qlx_batch;
+typedef struct qlx_batch_info {
+    uint64_t circuits;
+    uint64_t tasks;            /* chunk tasks of one persistent run */
+    uint64_t sweeps;           /* window sweeps over all circuits */
+    uint64_t state_bytes;      /* arena size */
+    uint64_t gates;            /* instructions over all circuits */
+    uint32_t workgroups;       /* persistent grid */
+    uint32_t max_sweeps;       /* longest circuit, in sweeps */
+} qlx_batch_info;
+int qlx_batch_create(int dtype, qlx_batch** out);
+int qlx_batch_add(qlx_batch* b, uint32_t num_qubits, const qlx_gate_instr* instrs, size_t n,
+                  const double* pool_c128, size_t pool_elems, const qlx_matrix_entry* meta, size_t n_meta,
+                  uint32_t* circuit_id);
+/* Plans every circuit, uploads descriptors, allocates the arena. opts: window/register
+ * qubits for circuits at least one window wide (smaller circuits use their own size). */
+int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* info);
+/* Enqueues one run of every circuit from |0...0> (asynchronous). */
+int qlx_batch_run(qlx_batch* b, int mode);
+/* Waits for the last run; *elapsed_ms = its GPU time (HIP events around the run). */
+int qlx_batch_sync(qlx_batch* b, double* elapsed_ms);
+int qlx_batch_state_to_host(qlx_batch* b, uint32_t circuit_id, double* out_c128, size_t dim);
+/* FP64 comparison of one batch state with a reference state of the same size:
+ * 1 - |<ref|psi>|^2 and max |ref_i - psi_i|. */
+int qlx_batch_compare(qlx_batch* b, uint32_t circuit_id, qlx_state* ref, double* infidelity, double* max_abs);
+int qlx_batch_destroy(qlx_batch* b);
+
 /* ---- host-side helpers (CPU; no GPU needed) ------------------------------ */
 /* Seeded layered h/rx/cx circuit: H on every qubit, then `depth` layers of
  * RX(theta) on every qubit (theta = (mt19937_64() >> 11) * 2^-53 * 2*pi, stored

@@ -120,6 +120,9 @@ inline double pass_bytes(const qlx_state* st) {
 // Kernel launch helpers (defined in qlx_state.hip).
 void launch_fill_zero_state(qlx_state* st);
 double reduce_norm_sq(qlx_state* st);
+// FP64 <a|b> and max |a-b| over n amplitudes of (possibly different) precision, on `stream`.
+void compare_amplitudes(const void* a, int dta, const void* b, int dtb, uint64_t n, hipStream_t stream,
+                        DevBuf& scratch, double* re, double* im, double* maxd);
 
 // Plain (one sweep per instruction) executor, qlx_gates.hip.
 void execute_plain(qlx_state* st, const qlx_gate_instr* instrs, size_t n,

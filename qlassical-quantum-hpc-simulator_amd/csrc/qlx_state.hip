@@ -339,6 +339,34 @@ static void upload_host_state(qlx_state* st, const double* amps, size_t dim) {
     QLX_HIP_CHECK(hipStreamSynchronize(st->stream));
 }
 
+void compare_amplitudes(const void* a, int dta, const void* b, int dtb, uint64_t n, hipStream_t stream,
+                        DevBuf& scratch, double* re, double* im, double* maxd) {
+    const unsigned nb = red_blocks(n);
+    scratch.reserve((size_t)nb * 3 * sizeof(double), stream);
+    double* part = (double*)scratch.ptr;
+    auto go = [&](auto pa, auto pb) {
+        k_inner_partial<<<nb, kRedThreads, 0, stream>>>(pa, pb, n, part);
+        k_maxdiff_partial<<<nb, kRedThreads, 0, stream>>>(pa, pb, n, part + 2 * nb);
+    };
+    if (dta == QLX_C128 && dtb == QLX_C128) go((const double2*)a, (const double2*)b);
+    else if (dta == QLX_C128) go((const double2*)a, (const float2*)b);
+    else if (dtb == QLX_C128) go((const float2*)a, (const double2*)b);
+    else go((const float2*)a, (const float2*)b);
+    QLX_HIP_CHECK(hipGetLastError());
+    std::vector<double> h(3 * nb);
+    QLX_HIP_CHECK(hipMemcpyAsync(h.data(), part, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, stream));
+    QLX_HIP_CHECK(hipStreamSynchronize(stream));
+    double r = 0, i = 0, m = 0;
+    for (unsigned k = 0; k < nb; ++k) {
+        r += h[2 * k];
+        i += h[2 * k + 1];
+        m = std::fmax(m, h[2 * nb + k]);
+    }
+    *re = r;
+    *im = i;
+    *maxd = m;
+}
+
 template <class F>
 static void dispatch_pair(qlx_state* a, qlx_state* b, F&& f) {
     if (a->dtype == QLX_C128 && b->dtype == QLX_C128) f((const double2*)a->amp, (const double2*)b->amp);

@@ -25,255 +25,21 @@
 #include <type_traits>
 #include <vector>
 
-#include "qlx_internal.hpp"
+#include "qlx_sweep_core.hpp"
 
 namespace qlx {
-
-constexpr int kMaxW = 14;
-constexpr int kMaxLogT = 10;
-constexpr int kMaxR = 32;
-
-enum OpKind : uint8_t { OP_U2 = 1, OP_RX = 2, OP_H = 3, OP_CX = 4, OP_SWAP = 5, OP_U4 = 6 };
-
-struct DevOp {
-    uint16_t key;   // (kind << 8) | (register bit 0 << 4) | register bit 1
-    uint16_t pad;
-    uint32_t coef;  // index into the coefficient array (doubles): U2/U4 matrices
-    double c0, c1;  // inline coefficients: RX (cos, sin), H (1/sqrt 2)
-};
-
-struct DevPhase {
-    uint32_t n_ops, op_begin, remap, pad;
-    uint32_t roff[kMaxR];        // local-index contribution of register index i
-    uint8_t tlocal[kMaxLogT];    // local bit held by work-item bit b
-    uint8_t pad2[6];
-};
-
-struct DevSweep {
-    uint32_t n_phases, phase_begin;
-    uint8_t S[kMaxW];            // global qubit of local bit j (ascending)
-    uint8_t ld_tglob[kMaxLogT];  // global qubit at work-item bit b in the first layout
-    uint8_t st_tglob[kMaxLogT];  // ... in the last layout
-    uint64_t ld_roff[kMaxR];     // global offset of register i in the first layout
-    uint64_t st_roff[kMaxR];     // ... in the last layout
-};
-
-// ---------------------------------------------------------------------------------
-// device side
-// ---------------------------------------------------------------------------------
-template <class A>
-struct Real;
-template <>
-struct Real<double2> { using T = double; };
-template <>
-struct Real<float2> { using T = float; };
-
-__device__ __forceinline__ uint64_t dins0(uint64_t idx, uint32_t t) {
-    const uint64_t lo = idx & ((1ull << t) - 1);
-    return lo | ((idx ^ lo) << 1);
-}
-
-// LDS bank swizzle: XOR the low bits of the element index with a fold of its high
-// bits (a bijection on [0, 2^W)), so work-items whose local indices differ only in
-// high bits still land in different banks.
-template <int ELEM>
-__device__ __forceinline__ uint32_t swz(uint32_t l) {
-    constexpr int SH = ELEM == 16 ? 4 : 5;
-    constexpr uint32_t M = (1u << SH) - 1;
-    return l ^ (((l >> SH) ^ (l >> (2 * SH)) ^ (l >> (3 * SH))) & M);
-}
-
-template <int B, class A, int R>
-__device__ __forceinline__ void g_rx(A (&v)[R], typename Real<A>::T c, typename Real<A>::T s) {
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        if (i & (1 << B)) continue;
-        const A a0 = v[i], a1 = v[i | (1 << B)];
-        v[i] = A{c * a0.x + s * a1.y, c * a0.y + (-s * a1.x)};
-        v[i | (1 << B)] = A{s * a0.y + c * a1.x, -s * a0.x + c * a1.y};
-    }
-}
-
-template <int B, class A, int R>
-__device__ __forceinline__ void g_h(A (&v)[R], typename Real<A>::T r) {
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        if (i & (1 << B)) continue;
-        const A a0 = v[i], a1 = v[i | (1 << B)];
-        v[i] = A{r * (a0.x + a1.x), r * (a0.y + a1.y)};
-        v[i | (1 << B)] = A{r * (a0.x - a1.x), r * (a0.y - a1.y)};
-    }
-}
-
-template <int B, class A, int R>
-__device__ __forceinline__ void g_u2(A (&v)[R], const double* __restrict__ m) {
-    using T = typename Real<A>::T;
-    const T m0r = (T)m[0], m0i = (T)m[1], m1r = (T)m[2], m1i = (T)m[3];
-    const T m2r = (T)m[4], m2i = (T)m[5], m3r = (T)m[6], m3i = (T)m[7];
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        if (i & (1 << B)) continue;
-        const A a0 = v[i], a1 = v[i | (1 << B)];
-        v[i] = A{m0r * a0.x - m0i * a0.y + m1r * a1.x - m1i * a1.y, m0r * a0.y + m0i * a0.x + m1r * a1.y + m1i * a1.x};
-        v[i | (1 << B)] =
-            A{m2r * a0.x - m2i * a0.y + m3r * a1.x - m3i * a1.y, m2r * a0.y + m2i * a0.x + m3r * a1.y + m3i * a1.x};
-    }
-}
-
-template <int BC, int BT, class A, int R>
-__device__ __forceinline__ void g_cx(A (&v)[R]) {
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        if (!(i & (1 << BC)) || (i & (1 << BT))) continue;
-        const A t = v[i];
-        v[i] = v[i | (1 << BT)];
-        v[i | (1 << BT)] = t;
-    }
-}
-
-template <int B0, int B1, class A, int R>
-__device__ __forceinline__ void g_swap(A (&v)[R]) {
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        if (!(i & (1 << B0)) || (i & (1 << B1))) continue;
-        const int j = i ^ (1 << B0) ^ (1 << B1);
-        const A t = v[i];
-        v[i] = v[j];
-        v[j] = t;
-    }
-}
-
-// Dense 4x4 on register bits (B0 = matrix bit 0, B1 = matrix bit 1), row-major complex128.
-template <int B0, int B1, class A, int R>
-__device__ __forceinline__ void g_u4(A (&v)[R], const double* __restrict__ m) {
-    using T = typename Real<A>::T;
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        if ((i & (1 << B0)) || (i & (1 << B1))) continue;
-        const int idx[4] = {i, i | (1 << B0), i | (1 << B1), i | (1 << B0) | (1 << B1)};
-        A x[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) x[c] = v[idx[c]];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            T re = T(0), im = T(0);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const T ur = (T)m[2 * (4 * r + c)], ui = (T)m[2 * (4 * r + c) + 1];
-                re += ur * x[c].x - ui * x[c].y;
-                im += ur * x[c].y + ui * x[c].x;
-            }
-            v[idx[r]] = A{re, im};
-        }
-    }
-}
-
-// One flat switch over (kind, bit0, bit1): every case indexes the register array with
-// compile-time constants only, so the array stays in VGPRs. (Nested or lambda-based
-// dispatch made the compiler keep the array in scratch memory.)
-#define QLX_K1(K, B, CALL)                     \
-    case ((K) << 8) | ((B) << 4):              \
-        if constexpr ((B) < RB) { CALL; }       \
-        break;
-#define QLX_K2(K, B0, B1, CALL)                \
-    case ((K) << 8) | ((B0) << 4) | (B1):      \
-        if constexpr ((B0) < RB && (B1) < RB) { CALL; } \
-        break;
-#define QLX_ALL1(K, F) QLX_K1(K, 0, F(0)) QLX_K1(K, 1, F(1)) QLX_K1(K, 2, F(2)) QLX_K1(K, 3, F(3)) QLX_K1(K, 4, F(4))
-#define QLX_ROW2(K, B0, F)                                                                      \
-    QLX_K2(K, B0, (B0 + 1) % 5, F(B0, (B0 + 1) % 5)) QLX_K2(K, B0, (B0 + 2) % 5, F(B0, (B0 + 2) % 5)) \
-    QLX_K2(K, B0, (B0 + 3) % 5, F(B0, (B0 + 3) % 5)) QLX_K2(K, B0, (B0 + 4) % 5, F(B0, (B0 + 4) % 5))
-#define QLX_ALL2(K, F) QLX_ROW2(K, 0, F) QLX_ROW2(K, 1, F) QLX_ROW2(K, 2, F) QLX_ROW2(K, 3, F) QLX_ROW2(K, 4, F)
-
-template <class A, int R, int RB>
-__device__ __forceinline__ void apply_op(A (&v)[R], const DevOp op, const double* __restrict__ coef) {
-    using T = typename Real<A>::T;
-    const double* m = coef + op.coef;
-    const T c0 = (T)op.c0, c1 = (T)op.c1;
-#define F_RX(B) g_rx<B>(v, c0, c1)
-#define F_H(B) g_h<B>(v, c0)
-#define F_U2(B) g_u2<B>(v, m)
-#define F_CX(B0, B1) g_cx<B0, B1>(v)
-#define F_SW(B0, B1) g_swap<B0, B1>(v)
-#define F_U4(B0, B1) g_u4<B0, B1>(v, m)
-    switch (op.key) {
-        QLX_ALL1(OP_RX, F_RX)
-        QLX_ALL1(OP_H, F_H)
-        QLX_ALL1(OP_U2, F_U2)
-        QLX_ALL2(OP_CX, F_CX)
-        QLX_ALL2(OP_SWAP, F_SW)
-        QLX_ALL2(OP_U4, F_U4)
-        default: break;
-    }
-#undef F_RX
-#undef F_H
-#undef F_U2
-#undef F_CX
-#undef F_SW
-#undef F_U4
-}
 
 template <class A, int LOGT, int RB>
 __global__ void __launch_bounds__(1 << LOGT) k_sweep(A* __restrict__ amp, const DevSweep* __restrict__ sw,
                                                     const DevPhase* __restrict__ ph, const DevOp* __restrict__ ops,
                                                     const double* __restrict__ coef) {
-    constexpr int R = 1 << RB;
-    constexpr int W = LOGT + RB;
-    constexpr int ELEM = (int)sizeof(A);
     extern __shared__ __align__(16) unsigned char smem[];
-    A* lds = reinterpret_cast<A*>(smem);
-    const uint32_t t = threadIdx.x;
-
-    uint64_t base = blockIdx.x;
-#pragma unroll
-    for (int j = 0; j < W; ++j) base = dins0(base, sw->S[j]);
-
-    uint64_t toff = 0;
-#pragma unroll
-    for (int b = 0; b < LOGT; ++b) toff |= (uint64_t)((t >> b) & 1u) << sw->ld_tglob[b];
-
-    A v[R];
-#pragma unroll
-    for (int i = 0; i < R; ++i) v[i] = amp[base | toff | sw->ld_roff[i]];
-
-    const DevPhase* cur = ph + sw->phase_begin;
-    uint32_t lt = 0;
-#pragma unroll
-    for (int b = 0; b < LOGT; ++b) lt |= ((t >> b) & 1u) << cur->tlocal[b];
-
-    const uint32_t nph = sw->n_phases;
-    for (uint32_t p = 0; p < nph; ++p) {
-        const DevPhase* P = ph + sw->phase_begin + p;
-        if (P->remap) {
-#pragma unroll
-            for (int i = 0; i < R; ++i) lds[swz<ELEM>(lt | cur->roff[i])] = v[i];
-            __syncthreads();
-            lt = 0;
-#pragma unroll
-            for (int b = 0; b < LOGT; ++b) lt |= ((t >> b) & 1u) << P->tlocal[b];
-#pragma unroll
-            for (int i = 0; i < R; ++i) v[i] = lds[swz<ELEM>(lt | P->roff[i])];
-            __syncthreads();
-        }
-        cur = P;
-        const uint32_t nops = P->n_ops, ob = P->op_begin;
-        for (uint32_t o = 0; o < nops; ++o) apply_op<A, R, RB>(v, ops[ob + o], coef);
-    }
-
-    toff = 0;
-#pragma unroll
-    for (int b = 0; b < LOGT; ++b) toff |= (uint64_t)((t >> b) & 1u) << sw->st_tglob[b];
-#pragma unroll
-    for (int i = 0; i < R; ++i) amp[base | toff | sw->st_roff[i]] = v[i];
+    run_chunk<A, LOGT, RB>(amp, sw, blockIdx.x, ph, ops, coef, reinterpret_cast<A*>(smem), false);
 }
 
 // ---------------------------------------------------------------------------------
 // host side: planning
 // ---------------------------------------------------------------------------------
-struct Geometry {
-    int logt, rb, w, c;
-};
-
 // (window qubits, register qubits) pairs with a compiled k_sweep instantiation.
 static bool geometry_built(int dtype, int w, int rb) {
     if (dtype == QLX_C128) return (w == 12 && (rb == 4 || rb == 5)) || (w == 11 && (rb == 4 || rb == 3));
@@ -300,42 +66,6 @@ struct PGate {
     int nq;             // operand count
     int q[5];
     bool sweepable;
-};
-
-struct PlanOp {
-    int src;
-    uint8_t kind;
-    int l0, l1;  // local bits (l1 = -1 for 1q)
-};
-
-struct PlanPhase {
-    std::vector<int> st2loc;  // storage bit -> local bit (size W)
-    std::vector<PlanOp> ops;
-};
-
-struct PlanSweep {
-    std::vector<int> S;  // global qubits, ascending, size W
-    std::vector<PlanPhase> phases;
-};
-
-struct PlanStep {
-    bool sweep;
-    int src;   // plain instruction
-    int sweep_index;
-};
-
-struct SweepPlan {
-    Geometry geo{};
-    std::vector<PlanStep> steps;
-    std::vector<PlanSweep> sweeps;
-    std::vector<int> order;     // execution order of source instructions
-    std::vector<int> order_sweep;
-    // device image
-    std::vector<DevSweep> dsw;
-    std::vector<DevPhase> dph;
-    std::vector<DevOp> dop;
-    std::vector<double> coef;
-    uint64_t phases = 0, gates_in_sweeps = 0;
 };
 
 static bool is_sweepable(const qlx_gate_instr& g, const qlx_matrix_entry* meta) {
@@ -564,11 +294,11 @@ static void plan_phases(PlanSweep& sw, const std::vector<PGate>& lg, const Geome
     }
 }
 
-static std::unique_ptr<SweepPlan> build_plan(uint32_t nq, int dtype, const qlx_gate_instr* instrs, size_t n,
-                                             const double* pool, const qlx_matrix_entry* meta,
-                                             const qlx_exec_opts& opts) {
+std::unique_ptr<SweepPlan> build_plan_geo(uint32_t nq, const Geometry& geo, const qlx_gate_instr* instrs, size_t n,
+                                          const double* pool, const qlx_matrix_entry* meta,
+                                          const qlx_exec_opts& opts) {
     auto plan = std::make_unique<SweepPlan>();
-    const Geometry geo = geometry_for(dtype, nq, opts);
+    if ((int)nq < geo.w) fail(QLX_EINVAL, "window larger than the state");
     plan->geo = geo;
     const int W = geo.w;
 
@@ -738,6 +468,12 @@ static std::unique_ptr<SweepPlan> build_plan(uint32_t nq, int dtype, const qlx_g
         }
     }
     return plan;
+}
+
+static std::unique_ptr<SweepPlan> build_plan(uint32_t nq, int dtype, const qlx_gate_instr* instrs, size_t n,
+                                             const double* pool, const qlx_matrix_entry* meta,
+                                             const qlx_exec_opts& opts) {
+    return build_plan_geo(nq, geometry_for(dtype, nq, opts), instrs, n, pool, meta, opts);
 }
 
 // ---------------------------------------------------------------------------------

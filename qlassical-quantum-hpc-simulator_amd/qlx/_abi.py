@@ -116,6 +116,24 @@ class TranspileOpts(C.Structure):
     ]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [
+        ("circuits", C.c_uint64),
+        ("tasks", C.c_uint64),
+        ("sweeps", C.c_uint64),
+        ("state_bytes", C.c_uint64),
+        ("gates", C.c_uint64),
+        ("workgroups", C.c_uint32),
+        ("max_sweeps", C.c_uint32),
+    ]
+
+
+QLX_BATCH_PERSISTENT = 0
+QLX_BATCH_PER_CIRCUIT = 1
+QLX_BATCH_PER_CIRCUIT_GRAPH = 2
+QLX_BATCH_LEVELS = 3
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [
         ("hbm_bytes", C.c_uint64),
@@ -164,6 +182,15 @@ SIGNATURES = [
     ("qlx_program_copy", [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("qlx_program_destroy", [C.c_void_p]),
     ("qlx_gpu_safe_qubit_limit", [C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
+    ("qlx_batch_create", [C.c_int, C.POINTER(C.c_void_p)]),
+    ("qlx_batch_add", [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,
+                       C.c_size_t, C.POINTER(C.c_uint32)]),
+    ("qlx_batch_prepare", [C.c_void_p, C.POINTER(ExecOpts), C.POINTER(BatchInfo)]),
+    ("qlx_batch_run", [C.c_void_p, C.c_int]),
+    ("qlx_batch_sync", [C.c_void_p, C.POINTER(C.c_double)]),
+    ("qlx_batch_state_to_host", [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t]),
+    ("qlx_batch_compare", [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("qlx_batch_destroy", [C.c_void_p]),
     ("qlx_gen_layered", [C.c_uint32, C.c_uint32, C.c_uint64, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
 ]
 

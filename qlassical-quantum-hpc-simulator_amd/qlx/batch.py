@@ -1,0 +1,86 @@
+"""Batched execution of many independent circuits (qlx_batch_*), and the seeded
+workload generators of the batching and serving configurations."""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Iterable, List, Optional, Tuple
+
+import numpy as np
+
+from . import _abi
+from .engine import State, _dt, exec_opts
+from .ir import MT19937_64, Program, layered
+
+MODES = {"persistent": _abi.QLX_BATCH_PERSISTENT, "per_circuit": _abi.QLX_BATCH_PER_CIRCUIT,
+         "per_circuit_graph": _abi.QLX_BATCH_PER_CIRCUIT_GRAPH, "levels": _abi.QLX_BATCH_LEVELS}
+
+
+def batch_workload(count: int = 4096, qubits=(10, 20), depths=(5, 40), shape_seed: int = 4004,
+                   circuit_seed_base: int = 400000) -> List[Tuple[int, int, int]]:
+    """(num_qubits, depth, seed) of each circuit: N and D drawn uniformly from an
+    mt19937_64(shape_seed) stream as lo + rng() % (hi - lo + 1); circuit i uses seed base + i."""
+    rng = MT19937_64(shape_seed)
+    out = []
+    for i in range(count):
+        n = qubits[0] + rng() % (qubits[1] - qubits[0] + 1)
+        d = depths[0] + rng() % (depths[1] - depths[0] + 1)
+        out.append((int(n), int(d), circuit_seed_base + i))
+    return out
+
+
+class Batch:
+    """A set of circuits whose states live in one HBM arena."""
+
+    def __init__(self, programs: Iterable[Program], dtype="c64", window_qubits: int = 0, register_qubits: int = 0):
+        L = _abi.lib()
+        h = C.c_void_p()
+        _abi.check(L.qlx_batch_create(_dt(dtype), C.byref(h)))
+        self._h = h
+        self.dtype = _dt(dtype)
+        self.programs = []
+        try:
+            for p in programs:
+                ins = np.ascontiguousarray(p.instrs)
+                pool = np.ascontiguousarray(p.pool, np.complex128)
+                meta = np.ascontiguousarray(p.meta)
+                cid = C.c_uint32()
+                _abi.check(L.qlx_batch_add(h, p.num_qubits, ins.ctypes.data if len(ins) else None, len(ins),
+                                           pool.ctypes.data if pool.size else None, pool.size,
+                                           meta.ctypes.data if len(meta) else None, len(meta), C.byref(cid)))
+                self.programs.append(p)
+            o = exec_opts("windowed", window_qubits=window_qubits, register_qubits=register_qubits)
+            info = _abi.BatchInfo()
+            _abi.check(L.qlx_batch_prepare(h, C.byref(o), C.byref(info)))
+            self.info = {k: getattr(info, k) for k, _ in info._fields_}
+        except Exception:
+            self.close()
+            raise
+
+    def run(self, mode: str = "persistent") -> None:
+        _abi.check(_abi.lib().qlx_batch_run(self._h, MODES[mode]))
+
+    def sync(self) -> float:
+        ms = C.c_double()
+        _abi.check(_abi.lib().qlx_batch_sync(self._h, C.byref(ms)))
+        return ms.value
+
+    def state(self, i: int) -> np.ndarray:
+        out = np.empty(1 << self.programs[i].num_qubits, np.complex128)
+        _abi.check(_abi.lib().qlx_batch_state_to_host(self._h, i, out.ctypes.data, out.size))
+        return out
+
+    def compare(self, i: int, ref: State) -> Tuple[float, float]:
+        inf, m = C.c_double(), C.c_double()
+        _abi.check(_abi.lib().qlx_batch_compare(self._h, i, ref.handle, C.byref(inf), C.byref(m)))
+        return inf.value, m.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _abi.check(_abi.lib().qlx_batch_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

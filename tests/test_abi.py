@@ -102,3 +102,14 @@ def test_measure_draw_matches_oracle_engine():
         _, m = oracle.execute(1, c.instrs, seed=seed)
         r = qlx.MT19937_64(seed).uniform01()
         assert m[0] == (1 if r < 0.5 else 0)
+
+
+def test_batch_workload_shapes():
+    w = qlx.batch_workload(4096)
+    ns = [n for n, _, _ in w]
+    ds = [d for _, d, _ in w]
+    assert min(ns) == 10 and max(ns) == 20 and min(ds) == 5 and max(ds) == 40
+    assert w[0][2] == 400000 and w[-1][2] == 404095
+    # the draw is the mt19937_64 stream of the stated seed
+    g = qlx.MT19937_64(4004)
+    assert w[0][0] == 10 + g() % 11 and w[0][1] == 5 + g() % 36
