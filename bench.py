@@ -75,6 +75,59 @@ def cpu_baseline(prog, nq, budget_s):
     }
 
 
+def run_batch(args):
+    """4096 independent layered circuits (N ~ U[10,20], D ~ U[5,40], mt19937_64(4004) shapes,
+    seeds 400000+i), complex64: one persistent launch vs per-circuit plain launches."""
+    import numpy as np
+
+    import qlx
+    work = qlx.batch_workload(args.batch_circuits)
+    progs = [qlx.layered(n, d, s) for n, d, s in work]
+    gates = sum(len(p) for p in progs)
+    t0 = time.perf_counter()
+    b = qlx.Batch(progs, "c64", window_qubits=args.window, register_qubits=args.regs)
+    prep_s = time.perf_counter() - t0
+    log(f"batch: {len(progs)} circuits, {gates} gates, info {b.info}, prepare {prep_s:.2f} s")
+    res = {}
+    for mode in args.batch_modes.split(","):
+        for _ in range(args.warmup):
+            b.run(mode)
+            b.sync()
+        times = []
+        for _ in range(args.steps):
+            b.run(mode)
+            times.append(b.sync())
+        ms = float(np.median(times))
+        res[mode] = {"ms": round(ms, 3), "circuits_per_s": round(len(progs) / (ms / 1e3), 1)}
+        log(f"batch: {mode} {res[mode]}")
+    # fidelity of every state against the complex128 plain path (itself pinned to the CPU oracle)
+    b.run("persistent")
+    b.sync()
+    worst, worst_abs = 0.0, 0.0
+    check = range(len(progs)) if args.batch_check_all else range(0, len(progs), max(1, len(progs) // 256))
+    for i in check:
+        ref = qlx.State(progs[i].num_qubits, "c128")
+        ref.execute(progs[i])
+        ref.sync()
+        inf, mx = b.compare(i, ref)
+        ref.close()
+        worst, worst_abs = max(worst, inf), max(worst_abs, mx)
+    b.close()
+    best = res.get("persistent", {}).get("circuits_per_s")
+    base = res.get("per_circuit", {}).get("circuits_per_s")
+    out = {
+        "metric": "batched circuits/s (4096 layered circuits, complex64)", "value": best, "unit": "circuits/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res.get("persistent", {}).get("ms"), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(progs)} layered h/rx/cx circuits, N~U[10,20], D~U[5,40], complex64",
+                   "gates": gates, "modes": res, "speedup_vs_per_circuit_plain": round(best / base, 2) if best and base else None,
+                   "checked": len(list(check)), "max_infidelity_vs_c128": worst, "max_abs_vs_c128": worst_abs,
+                   "batch_info": b.info if False else None, "prepare_s": round(prep_s, 2)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +141,10 @@ def main():
     ap.add_argument("--regs", type=int, default=0, help="register qubits per work-item (0 = default)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", choices=["circuit30", "batch"], default="circuit30")
+    ap.add_argument("--batch-circuits", type=int, default=4096)
+    ap.add_argument("--batch-modes", default="persistent,levels,per_circuit,per_circuit_graph")
+    ap.add_argument("--batch-check-all", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one k_copy of the whole state (HBM counter calibration)")
     args = ap.parse_args()
@@ -97,6 +154,10 @@ def main():
               "(a state shards by its top log2(G) qubits; see the design notes). Use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "batch":
+        run_batch(args)
+        return
 
     import numpy as np
 
