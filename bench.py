@@ -123,7 +123,37 @@ def run_batch(args):
         "config": {"workload": f"{len(progs)} layered h/rx/cx circuits, N~U[10,20], D~U[5,40], complex64",
                    "gates": gates, "modes": res, "speedup_vs_per_circuit_plain": round(best / base, 2) if best and base else None,
                    "checked": len(list(check)), "max_infidelity_vs_c128": worst, "max_abs_vs_c128": worst_abs,
-                   "batch_info": b.info if False else None, "prepare_s": round(prep_s, 2)},
+                   "prepare_s": round(prep_s, 2)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_serve(args):
+    """Poisson stream of layered circuits (N ~ U[12,28], D ~ U[5,40], mt19937_64(5005) shapes,
+    seeds 500000+j), complex64, HBM-budgeted continuous batching. The rate is 80% of the
+    capacity measured with every request arriving at t = 0."""
+    import numpy as np
+
+    import qlx
+    work = qlx.serve_workload(args.serve_requests)
+    progs = [qlx.layered(n, d, s) for n, d, s in work]
+    budget = int(args.serve_budget_gb * (1 << 30))
+    cap = qlx.run_server(progs, np.zeros(len(progs)), budget, "c64")
+    rate = 0.8 * cap["circuits_per_s"]
+    log(f"serve: saturated capacity {cap['circuits_per_s']:.1f} circuits/s; running at {rate:.1f}/s")
+    arr = qlx.poisson_arrivals(len(progs), rate, seed=5006)
+    res = qlx.run_server(progs, arr, budget, "c64", check_every=max(1, len(progs) // 40))
+    out = {
+        "metric": "served circuits/s + p99 latency (Poisson layered circuits, complex64)",
+        "value": round(res["circuits_per_s"], 2), "unit": "circuits/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(res["wall_s"] * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(progs)} requests, N~U[12,28], D~U[5,40], Poisson rate {rate:.1f}/s "
+                               f"(80% of saturated {cap['circuits_per_s']:.1f}/s), HBM budget {args.serve_budget_gb} GiB",
+                   "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
+                   "mean_ms": round(res["mean_ms"], 3), "saturated_circuits_per_s": round(cap["circuits_per_s"], 2),
+                   "saturated_p99_ms": round(cap["p99_ms"], 3), "spot_checks": len(res["checks"]),
+                   "max_infidelity_vs_c128": res["max_infidelity"], "max_abs_vs_c128": res["max_abs"]},
     }
     print(json.dumps(out), flush=True)
 
@@ -141,7 +171,9 @@ def main():
     ap.add_argument("--regs", type=int, default=0, help="register qubits per work-item (0 = default)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", choices=["circuit30", "batch"], default="circuit30")
+    ap.add_argument("--config", choices=["circuit30", "batch", "serve"], default="circuit30")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--batch-modes", default="persistent,levels,per_circuit,per_circuit_graph")
     ap.add_argument("--batch-check-all", action="store_true")
@@ -157,6 +189,9 @@ def main():
 
     if args.config == "batch":
         run_batch(args)
+        return
+    if args.config == "serve":
+        run_serve(args)
         return
 
     import numpy as np

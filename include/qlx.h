@@ -273,6 +273,26 @@ int qlx_batch_state_to_host(qlx_batch* b, uint32_t circuit_id, double* out_c128,
 int qlx_batch_compare(qlx_batch* b, uint32_t circuit_id, qlx_state* ref, double* infidelity, double* max_abs);
 int qlx_batch_destroy(qlx_batch* b);
 
+/* ---- serving (no reference counterpart) -----------------------------------------
+ * A continuous-batching request loop: submitted circuits (>= 10 qubits; 1-qubit gates,
+ * CX, SWAP, GLOBAL_SWAP) are admitted FIFO while their states fit the HBM budget, grouped
+ * by qubit count, and every launch of a group advances all of its active requests by one
+ * window sweep; requests join and leave between launches. Results stay in HBM until
+ * released. qlx_server_pump never blocks on device work. */
+typedef struct qlx_server qlx_server;
+int qlx_server_create(int dtype, uint64_t hbm_budget_bytes, qlx_server** out);
+int qlx_server_submit(qlx_server* s, uint32_t num_qubits, const qlx_gate_instr* instrs, size_t n,
+                      uint64_t* request_id);
+/* Harvests finished launches, admits queued requests, launches the next sweep of every
+ * group with room (two launches in flight per group). */
+int qlx_server_pump(qlx_server* s, uint32_t* queued, uint32_t* active, uint32_t* inflight);
+/* Requests completed since the last call, with submit-to-completion latency in seconds. */
+int qlx_server_take_completed(qlx_server* s, uint64_t* ids, double* latency_s, size_t cap, size_t* n_out);
+int qlx_server_compare(qlx_server* s, uint64_t id, qlx_state* ref, double* infidelity, double* max_abs);
+int qlx_server_state_to_host(qlx_server* s, uint64_t id, double* out_c128, size_t dim);
+int qlx_server_release(qlx_server* s, uint64_t id);
+int qlx_server_destroy(qlx_server* s);
+
 /* ---- host-side helpers (CPU; no GPU needed) ------------------------------ */
 /* Seeded layered h/rx/cx circuit: H on every qubit, then `depth` layers of
  * RX(theta) on every qubit (theta = (mt19937_64() >> 11) * 2^-53 * 2*pi, stored

@@ -1,0 +1,468 @@
+// qlx_serve.hip — a continuous-batching circuit server on one GPU.
+//
+// Requests (MEASURE-free circuits of >= 10 qubits made of window-sweep gates) are queued
+// on submit and admitted FIFO while their state fits an HBM byte budget (first-fit
+// allocator over one arena). An admitted request's window-sweep plan is uploaded once and
+// its state stays in HBM until the client releases it. Requests are grouped by qubit
+// count; each group owns a HIP stream, and one launch (`k_serve_step`) advances every
+// active request of the group by one sweep (all its chunks as independent work-groups).
+// New requests join their group's next launch, finished ones leave: iteration-level
+// (continuous) batching. Groups run on different streams, so their work overlaps on the
+// device. Up to two launches per group are in flight; a request's sweeps are ordered by
+// its group's stream.
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "qlx_sweep_core.hpp"
+
+namespace qlx {
+
+struct ServeTask {
+    uint64_t amp_off;    // elements into the arena
+    uint64_t block_off;  // bytes into the descriptor pool (this request's image)
+    uint32_t sweep;      // sweep index within the request
+    uint32_t chunk;
+    uint32_t rb;
+    uint32_t init;       // first sweep: synthesise |0...0>
+    uint32_t off_ph, off_op, off_cf, pad;  // byte offsets of phases/ops/coefs inside the image
+};
+
+template <class A>
+__global__ void __launch_bounds__(256) k_serve_step(A* __restrict__ arena, const ServeTask* __restrict__ tasks,
+                                                   const char* __restrict__ pool) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    A* lds = reinterpret_cast<A*>(smem);
+    const ServeTask tk = tasks[blockIdx.x];
+    const char* img = pool + tk.block_off;
+    const DevSweep* sw = reinterpret_cast<const DevSweep*>(img) + tk.sweep;
+    const DevPhase* ph = reinterpret_cast<const DevPhase*>(img + tk.off_ph);
+    const DevOp* ops = reinterpret_cast<const DevOp*>(img + tk.off_op);
+    const double* coef = reinterpret_cast<const double*>(img + tk.off_cf);
+    A* amp = arena + tk.amp_off;
+    const bool init = tk.init != 0;
+    if constexpr (sizeof(A) == 8) {
+        switch (tk.rb) {
+            case 2: run_chunk<A, 8, 2>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
+            case 3: run_chunk<A, 8, 3>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
+            case 4: run_chunk<A, 8, 4>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
+            default: run_chunk<A, 8, 5>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
+        }
+    } else {
+        switch (tk.rb) {
+            case 2: run_chunk<A, 8, 2>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
+            case 3: run_chunk<A, 8, 3>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
+            default: run_chunk<A, 8, 4>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
+        }
+    }
+}
+
+// First-fit allocator over [0, size) with 256-byte granularity.
+class RangeAlloc {
+public:
+    void reset(uint64_t size) {
+        free_.clear();
+        free_[0] = size;
+        size_ = size;
+    }
+    bool alloc(uint64_t n, uint64_t& off) {
+        n = (n + 255) & ~uint64_t(255);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                off = it->first;
+                const uint64_t rest = it->second - n;
+                free_.erase(it);
+                if (rest) free_[off + n] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t n) {
+        n = (n + 255) & ~uint64_t(255);
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+
+private:
+    std::map<uint64_t, uint64_t> free_;
+    uint64_t size_ = 0;
+};
+
+}  // namespace qlx
+
+using namespace qlx;
+
+struct qlx_server {
+    int dtype = QLX_C64;
+    size_t amp = 8;
+    DevBuf arena, pool;
+    RangeAlloc arena_alloc, pool_alloc;
+    uint64_t budget = 0, pool_bytes = 0, next_id = 1;
+    std::chrono::steady_clock::time_point t0;
+    enum Status { QUEUED, ACTIVE, DONE };
+    struct Req {
+        uint64_t id;
+        uint32_t nq;
+        std::vector<qlx_gate_instr> instrs;
+        Status status = QUEUED;
+        std::unique_ptr<SweepPlan> plan;
+        std::vector<char> image;  // descriptor image (kept until the upload is complete)
+        uint64_t amp_off = 0, state_bytes = 0, block_off = 0, block_bytes = 0;
+        uint32_t off_ph = 0, off_op = 0, off_cf = 0;
+        uint32_t next_sweep = 0, n_sweeps = 0, rb = 0, log_chunks = 0;
+        double submit_s = 0, admit_s = 0, done_s = 0;
+    };
+    std::map<uint64_t, Req> reqs;
+    std::deque<uint64_t> queue;
+    struct Step {
+        hipEvent_t ev;
+        int slot;
+        std::vector<uint64_t> finishing;  // requests whose last sweep is in this launch
+    };
+    struct Group {
+        hipStream_t stream = nullptr;
+        std::vector<uint64_t> active;
+        std::deque<Step> inflight;
+        ServeTask* host_tasks[2] = {nullptr, nullptr};
+        DevBuf dev_tasks[2];
+        size_t cap[2] = {0, 0};
+        int next_slot = 0;
+    };
+    std::map<uint32_t, Group> groups;
+    std::vector<uint64_t> completed;
+    std::vector<hipEvent_t> free_events;
+    DevBuf scratch;
+    hipStream_t aux = nullptr;
+};
+
+namespace {
+
+using Group = qlx_server::Group;
+using Req = qlx_server::Req;
+
+double now_s(const qlx_server* s) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+}
+
+hipEvent_t get_event(qlx_server* s) {
+    if (!s->free_events.empty()) {
+        hipEvent_t e = s->free_events.back();
+        s->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    QLX_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return e;
+}
+
+size_t align_to(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+void destroy_server(qlx_server* s) {
+    if (!s) return;
+    for (auto& kv : s->groups) {
+        Group& g = kv.second;
+        if (g.stream) (void)hipStreamSynchronize(g.stream);
+        for (auto& st : g.inflight) (void)hipEventDestroy(st.ev);
+        for (int k = 0; k < 2; ++k) {
+            if (g.host_tasks[k]) (void)hipHostFree(g.host_tasks[k]);
+            g.dev_tasks[k].release();
+        }
+        if (g.stream) (void)hipStreamDestroy(g.stream);
+    }
+    for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
+    if (s->aux) (void)hipStreamDestroy(s->aux);
+    s->arena.release();
+    s->pool.release();
+    s->scratch.release();
+    delete s;
+}
+
+// Admission: FIFO while the head request's state and descriptors fit.
+void admit(qlx_server* s) {
+    while (!s->queue.empty()) {
+        Req& r = s->reqs.at(s->queue.front());
+        uint64_t aoff = 0, boff = 0;
+        if (!s->arena_alloc.alloc(r.state_bytes, aoff)) return;
+        if (!s->pool_alloc.alloc(r.block_bytes, boff)) {
+            s->arena_alloc.release(aoff, r.state_bytes);
+            return;
+        }
+        r.amp_off = aoff / s->amp;
+        r.block_off = boff;
+        Group& g = s->groups[r.nq];
+        if (!g.stream) QLX_HIP_CHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+        // upload on the auxiliary stream (never waits for the groups' launches); it is
+        // complete before the request's first launch is enqueued
+        QLX_HIP_CHECK(hipMemcpyAsync((char*)s->pool.ptr + boff, r.image.data(), r.image.size(),
+                                     hipMemcpyHostToDevice, s->aux));
+        QLX_HIP_CHECK(hipStreamSynchronize(s->aux));
+        r.image.clear();
+        r.image.shrink_to_fit();
+        r.status = qlx_server::ACTIVE;
+        r.admit_s = now_s(s);
+        g.active.push_back(r.id);
+        s->queue.pop_front();
+    }
+}
+
+void harvest(qlx_server* s) {
+    for (auto& kv : s->groups) {
+        Group& g = kv.second;
+        while (!g.inflight.empty()) {
+            const hipError_t q = hipEventQuery(g.inflight.front().ev);
+            if (q == hipErrorNotReady) break;
+            if (q != hipSuccess) QLX_HIP_CHECK(q);
+            qlx_server::Step st = std::move(g.inflight.front());
+            g.inflight.pop_front();
+            s->free_events.push_back(st.ev);
+            const double t = now_s(s);
+            for (uint64_t id : st.finishing) {
+                Req& r = s->reqs.at(id);
+                r.status = qlx_server::DONE;
+                r.done_s = t;
+                s->pool_alloc.release(r.block_off, r.block_bytes);  // descriptors no longer read
+                s->completed.push_back(id);
+            }
+        }
+    }
+}
+
+void launch_steps(qlx_server* s) {
+    for (auto& kv : s->groups) {
+        Group& g = kv.second;
+        if (g.inflight.size() >= 2) continue;
+        std::vector<ServeTask> tasks;
+        std::vector<uint64_t> finishing, still;
+        for (uint64_t id : g.active) {
+            Req& r = s->reqs.at(id);
+            const uint32_t nch = 1u << r.log_chunks;
+            for (uint32_t k = 0; k < nch; ++k) {
+                ServeTask t{};
+                t.amp_off = r.amp_off;
+                t.block_off = r.block_off;
+                t.sweep = r.next_sweep;
+                t.chunk = k;
+                t.rb = r.rb;
+                t.init = r.next_sweep == 0;
+                t.off_ph = r.off_ph;
+                t.off_op = r.off_op;
+                t.off_cf = r.off_cf;
+                tasks.push_back(t);
+            }
+            if (++r.next_sweep == r.n_sweeps) finishing.push_back(id);
+            else still.push_back(id);
+        }
+        if (tasks.empty()) continue;
+        g.active.swap(still);
+        const int slot = g.next_slot;
+        g.next_slot ^= 1;
+        if (g.cap[slot] < tasks.size()) {
+            if (g.host_tasks[slot]) QLX_HIP_CHECK(hipHostFree(g.host_tasks[slot]));
+            g.host_tasks[slot] = nullptr;
+            const size_t cap = std::max<size_t>(tasks.size() * 2, 256);
+            QLX_HIP_CHECK(hipHostMalloc((void**)&g.host_tasks[slot], cap * sizeof(ServeTask), hipHostMallocDefault));
+            g.dev_tasks[slot].reserve(cap * sizeof(ServeTask), g.stream);
+            g.cap[slot] = cap;
+        }
+        std::memcpy(g.host_tasks[slot], tasks.data(), tasks.size() * sizeof(ServeTask));
+        QLX_HIP_CHECK(hipMemcpyAsync(g.dev_tasks[slot].ptr, g.host_tasks[slot], tasks.size() * sizeof(ServeTask),
+                                     hipMemcpyHostToDevice, g.stream));
+        const size_t lds = s->dtype == QLX_C64 ? (size_t(1) << 13) * 8 : (size_t(1) << 12) * 16;
+        if (s->dtype == QLX_C64)
+            k_serve_step<float2><<<(unsigned)tasks.size(), 256, lds, g.stream>>>(
+                (float2*)s->arena.ptr, (const ServeTask*)g.dev_tasks[slot].ptr, (const char*)s->pool.ptr);
+        else
+            k_serve_step<double2><<<(unsigned)tasks.size(), 256, lds, g.stream>>>(
+                (double2*)s->arena.ptr, (const ServeTask*)g.dev_tasks[slot].ptr, (const char*)s->pool.ptr);
+        QLX_HIP_CHECK(hipGetLastError());
+        qlx_server::Step st;
+        st.ev = get_event(s);
+        st.slot = slot;
+        st.finishing = std::move(finishing);
+        QLX_HIP_CHECK(hipEventRecord(st.ev, g.stream));
+        g.inflight.push_back(std::move(st));
+    }
+}
+
+Req& get_req(qlx_server* s, uint64_t id) {
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) fail(QLX_ERANGE, "unknown request id");
+    return it->second;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qlx_server_create(int dtype, uint64_t hbm_budget_bytes, qlx_server** out) {
+    return guard([&] {
+        if (!out) fail(QLX_EINVAL, "null output");
+        if (dtype != QLX_C64 && dtype != QLX_C128) fail(QLX_EINVAL, "bad dtype");
+        if (hbm_budget_bytes < (1u << 20)) fail(QLX_EINVAL, "HBM budget too small");
+        auto* s = new qlx_server();
+        try {
+            s->dtype = dtype;
+            s->amp = amp_size(dtype);
+            s->budget = hbm_budget_bytes;
+            s->arena.reserve(hbm_budget_bytes, nullptr);
+            s->arena_alloc.reset(hbm_budget_bytes);
+            s->pool_bytes = 256ull << 20;
+            s->pool.reserve(s->pool_bytes, nullptr);
+            s->pool_alloc.reset(s->pool_bytes);
+            QLX_HIP_CHECK(hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
+            s->t0 = std::chrono::steady_clock::now();
+        } catch (...) {
+            destroy_server(s);
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int qlx_server_submit(qlx_server* s, uint32_t nq, const qlx_gate_instr* instrs, size_t n, uint64_t* request_id) {
+    return guard([&] {
+        if (!s) fail(QLX_EINVAL, "null server");
+        if (nq < 10 || nq > 34) fail(QLX_EUNSUPPORTED, "served circuits need 10..34 qubits");
+        if (!instrs || n == 0) fail(QLX_EINVAL, "empty circuit");
+        Req r;
+        r.id = s->next_id++;
+        r.nq = nq;
+        r.state_bytes = (1ull << nq) * s->amp;
+        if (r.state_bytes > s->budget) fail(QLX_ENOMEM, "state larger than the HBM budget");
+        for (size_t i = 0; i < n; ++i) {
+            const qlx_gate_instr& g = instrs[i];
+            if (g.type == QLX_MEASURE || g.type == QLX_CCX || g.type == QLX_CSWAP || g.type == QLX_UNITARY ||
+                g.type == QLX_FUSED_BLOCK)
+                fail(QLX_EUNSUPPORTED, "served circuits take 1-qubit gates, CX, SWAP and GLOBAL_SWAP");
+            validate_instr(g, nq, nullptr, 0, 0);
+        }
+        r.instrs.assign(instrs, instrs + n);
+        qlx_exec_opts o{};
+        Geometry geo{};
+        geo.logt = 8;
+        geo.w = std::min<int>((int)nq, s->dtype == QLX_C64 ? 13 : 12);
+        geo.rb = geo.w - 8;
+        geo.c = 3;
+        r.plan = build_plan_geo(nq, geo, r.instrs.data(), n, nullptr, nullptr, o);
+        const SweepPlan& p = *r.plan;
+        r.n_sweeps = (uint32_t)p.dsw.size();
+        r.rb = (uint32_t)geo.rb;
+        r.log_chunks = nq - (uint32_t)geo.w;
+        const size_t off_ph = align_to(p.dsw.size() * sizeof(DevSweep), 256);
+        const size_t off_op = align_to(off_ph + p.dph.size() * sizeof(DevPhase), 256);
+        const size_t off_cf = align_to(off_op + p.dop.size() * sizeof(DevOp), 256);
+        const size_t total = align_to(off_cf + p.coef.size() * sizeof(double), 256) + 256;
+        r.image.assign(total, 0);
+        std::memcpy(r.image.data(), p.dsw.data(), p.dsw.size() * sizeof(DevSweep));
+        std::memcpy(r.image.data() + off_ph, p.dph.data(), p.dph.size() * sizeof(DevPhase));
+        if (!p.dop.empty()) std::memcpy(r.image.data() + off_op, p.dop.data(), p.dop.size() * sizeof(DevOp));
+        if (!p.coef.empty()) std::memcpy(r.image.data() + off_cf, p.coef.data(), p.coef.size() * sizeof(double));
+        r.off_ph = (uint32_t)off_ph;
+        r.off_op = (uint32_t)off_op;
+        r.off_cf = (uint32_t)off_cf;
+        r.block_bytes = total;
+        if (total > s->pool_bytes) fail(QLX_ENOMEM, "descriptor image larger than the pool");
+        r.submit_s = now_s(s);
+        const uint64_t id = r.id;
+        s->reqs.emplace(id, std::move(r));
+        s->queue.push_back(id);
+        if (request_id) *request_id = id;
+    });
+}
+
+int qlx_server_pump(qlx_server* s, uint32_t* queued, uint32_t* active, uint32_t* inflight) {
+    return guard([&] {
+        if (!s) fail(QLX_EINVAL, "null server");
+        harvest(s);
+        admit(s);
+        launch_steps(s);
+        uint32_t a = 0, f = 0;
+        for (auto& kv : s->groups) {
+            a += (uint32_t)kv.second.active.size();
+            f += (uint32_t)kv.second.inflight.size();
+        }
+        if (queued) *queued = (uint32_t)s->queue.size();
+        if (active) *active = a;
+        if (inflight) *inflight = f;
+    });
+}
+
+int qlx_server_take_completed(qlx_server* s, uint64_t* ids, double* latency_s, size_t cap, size_t* n_out) {
+    return guard([&] {
+        if (!s) fail(QLX_EINVAL, "null server");
+        const size_t n = std::min(cap, s->completed.size());
+        for (size_t i = 0; i < n; ++i) {
+            const Req& r = s->reqs.at(s->completed[i]);
+            if (ids) ids[i] = r.id;
+            if (latency_s) latency_s[i] = r.done_s - r.submit_s;
+        }
+        s->completed.erase(s->completed.begin(), s->completed.begin() + (long)n);
+        if (n_out) *n_out = n;
+    });
+}
+
+int qlx_server_compare(qlx_server* s, uint64_t id, qlx_state* ref, double* infidelity, double* max_abs) {
+    return guard([&] {
+        if (!s) fail(QLX_EINVAL, "null server");
+        Req& r = get_req(s, id);
+        if (r.status != qlx_server::DONE) fail(QLX_EINVAL, "request not completed");
+        if (!ref || !ref->amp || ref->nq != r.nq) fail(QLX_EINVAL, "reference state does not match");
+        QLX_HIP_CHECK(hipStreamSynchronize(ref->stream));
+        double re = 0, im = 0, m = 0;
+        compare_amplitudes(ref->amp, ref->dtype, (char*)s->arena.ptr + r.amp_off * s->amp, s->dtype, 1ull << r.nq,
+                           s->aux, s->scratch, &re, &im, &m);
+        if (infidelity) *infidelity = 1.0 - (re * re + im * im);
+        if (max_abs) *max_abs = m;
+    });
+}
+
+int qlx_server_state_to_host(qlx_server* s, uint64_t id, double* out, size_t dim) {
+    return guard([&] {
+        if (!s) fail(QLX_EINVAL, "null server");
+        Req& r = get_req(s, id);
+        if (r.status != qlx_server::DONE) fail(QLX_EINVAL, "request not completed");
+        const size_t n = size_t(1) << r.nq;
+        if (!out || dim < n) fail(QLX_EINVAL, "output buffer too small");
+        const char* src = (const char*)s->arena.ptr + r.amp_off * s->amp;
+        if (s->dtype == QLX_C128) {
+            QLX_HIP_CHECK(hipMemcpy(out, src, n * 16, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> tmp(2 * n);
+            QLX_HIP_CHECK(hipMemcpy(tmp.data(), src, n * 8, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < 2 * n; ++i) out[i] = tmp[i];
+        }
+    });
+}
+
+int qlx_server_release(qlx_server* s, uint64_t id) {
+    return guard([&] {
+        if (!s) fail(QLX_EINVAL, "null server");
+        Req& r = get_req(s, id);
+        if (r.status != qlx_server::DONE) fail(QLX_EINVAL, "request not completed");
+        s->arena_alloc.release(r.amp_off * s->amp, r.state_bytes);
+        s->reqs.erase(id);
+    });
+}
+
+int qlx_server_destroy(qlx_server* s) {
+    return guard([&] { destroy_server(s); });
+}
+
+}  // extern "C"
