@@ -215,12 +215,27 @@ def lib() -> C.CDLL:
             f"libqlx.so is not built at {LIB_PATH}; build it with "
             "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc, gfx950)")
     h = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+    missing = []
     for name, args in SIGNATURES:
-        fn = getattr(h, name)
+        try:
+            fn = getattr(h, name)
+        except AttributeError:
+            missing.append(name)
+            continue
         fn.argtypes = args
         fn.restype = C.c_char_p if name in ("qlx_last_error", "qlx_version") else C.c_int
+    if missing:
+        # an older build: everything it has works; calling a missing entry point raises
+        for name in missing:
+            setattr(h, name, _missing(name))
     _lib = h
     return h
+
+
+def _missing(name):
+    def call(*_a, **_k):
+        raise ImportError(f"{name} is not in {LIB_PATH}: rebuild the library")
+    return call
 
 
 def check(rc: int) -> None:
