@@ -75,6 +75,89 @@ def cpu_baseline(prog, nq, budget_s):
     }
 
 
+def run_precision(args):
+    """The 32-qubit depth-30 layered circuit (seed 1003) as a complex64 state (32 GiB),
+    its width-5 fused blocks (matrices formed in FP64 on the GPU) applied as MFMA
+    contractions, checked against the complex128 state (64 GiB) of the unfused circuit
+    run on the GPU in FP64 (windowed path, pinned to the CPU oracle at smaller sizes):
+    max |amplitude error| and infidelity 1 - |<psi64|psi128>|^2, FP64 reductions on the GPU."""
+    import numpy as np
+
+    import qlx
+
+    nq, depth, seed = args.qubits or 32, args.depth or 30, args.seed or 1003
+    prog = qlx.layered(nq, depth, seed)
+    t0 = time.perf_counter()
+    fused = qlx.transpile_gpu(prog, max_width=5, rotations=True, on_device=True)
+    t_fuse = time.perf_counter() - t0
+    kinds = {int(k): int(v) for k, v in zip(*np.unique(fused.meta["num_qubits"], return_counts=True))}
+    log(f"precision: {nq}q depth {depth}: {len(prog)} gates -> {len(fused)} instructions "
+        f"(fused widths {kinds}), fusion {t_fuse:.2f} s")
+    variants = [v for v in args.dense_variants.split(",") if v]
+    res = {}
+    st = qlx.State(nq, "c64")
+    for v in variants:
+        o = qlx.exec_opts("plain", dense=v)
+        st.reset()
+        for _ in range(args.warmup):
+            st.execute(fused, o)
+        st.sync()
+        st.timing(True)
+        st.timing_reset()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st.execute(fused, o)
+        st.sync()
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        timing = st.timing_read()
+        st.timing(False)
+        stats = st.last_stats()
+        res[v] = {"ms": ms, "timing": timing, "bytes": stats["algorithmic_bytes"]}
+        log(f"precision: {v}: {ms:.1f} ms/step, kernels {timing}")
+    # accuracy of one run from |0...0> against the FP64 state
+    ref = qlx.State(nq, "c128")
+    ref.execute(prog, qlx.exec_opts("windowed"))
+    ref.sync()
+    acc = {}
+    for v in variants:
+        st.reset()
+        st.execute(fused, qlx.exec_opts("plain", dense=v))
+        st.sync()
+        ip = st.inner(ref)
+        n64 = st.norm_sq()
+        acc[v] = {"max_abs_err": st.max_abs_diff(ref), "infidelity": 1.0 - abs(ip) ** 2 / n64,
+                  "norm_sq_minus_1": n64 - 1.0}
+        log(f"precision: {v}: {acc[v]}")
+    ref.close()
+    st.close()
+    main_v = variants[0]
+    r = res[main_v]
+    gbps = r["bytes"] / (r["ms"] / 1e3) / 1e9
+    name = "k_dense_mfma" if "k_dense_mfma" in r["timing"] else max(r["timing"].items(), key=lambda kv: kv[1][1])[0]
+    launches, tot_ms, tot_bytes = r["timing"][name]
+    avg_ms = tot_ms / launches
+    achieved = (tot_bytes / launches) / (avg_ms / 1e3)
+    out = {
+        "metric": METRIC, "value": round(gbps, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(r["ms"], 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": f"layered h/rx/cx circuit, {nq} qubits, depth {depth}, seed {seed}, complex64 "
+                        f"({(8 << nq) / 2**30:.0f} GiB state), {len(prog)} gates fused (width <= 5, FP64 "
+                        f"products on the GPU) into {len(fused)} instructions; dense gates as {main_v}",
+            "fused_widths": kinds, "fusion_s": round(t_fuse, 3),
+            "accuracy_vs_fp64": acc,
+            "variants_ms": {v: round(res[v]["ms"], 3) for v in variants},
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": name, "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4), "traffic": None,
+            "launches": launches, "avg_ms": round(avg_ms, 4), "bytes_per_launch": tot_bytes / launches,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_batch(args):
     """4096 independent layered circuits (N ~ U[10,20], D ~ U[5,40], mt19937_64(4004) shapes,
     seeds 400000+i), complex64: one persistent launch vs per-circuit plain launches."""
@@ -104,8 +187,10 @@ def run_batch(args):
     b.run("persistent")
     b.sync()
     worst, worst_abs = 0.0, 0.0
-    check = range(len(progs)) if args.batch_check_all else range(0, len(progs), max(1, len(progs) // 256))
+    check = range(0, len(progs), max(1, len(progs) // 256)) if args.batch_check_sample else range(len(progs))
     for i in check:
+        if i % 512 == 0:
+            log(f"batch: checking circuit {i} against complex128")
         ref = qlx.State(progs[i].num_qubits, "c128")
         ref.execute(progs[i])
         ref.sync()
@@ -163,20 +248,23 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--qubits", type=int, default=30)
-    ap.add_argument("--depth", type=int, default=20)
-    ap.add_argument("--seed", type=int, default=1002)
+    ap.add_argument("--qubits", type=int, default=None, help="circuit30: 30; precision: 32")
+    ap.add_argument("--depth", type=int, default=None, help="circuit30: 20; precision: 30")
+    ap.add_argument("--seed", type=int, default=None, help="circuit30: 1002; precision: 1003")
     ap.add_argument("--mode", choices=["auto", "plain", "windowed"], default="auto")
     ap.add_argument("--window", type=int, default=0, help="window qubits (0 = default for the precision)")
     ap.add_argument("--regs", type=int, default=0, help="register qubits per work-item (0 = default)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", choices=["circuit30", "batch", "serve"], default="circuit30")
+    ap.add_argument("--config", choices=["circuit30", "precision", "batch", "serve"], default="circuit30")
+    ap.add_argument("--dense-variants", default="bf16x3,f32,valu",
+                    help="precision config: dense-gate arithmetic forms to time (first one is reported)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--batch-modes", default="persistent,levels,per_circuit,per_circuit_graph")
-    ap.add_argument("--batch-check-all", action="store_true")
+    ap.add_argument("--batch-check-sample", action="store_true",
+                    help="check every 16th state against complex128 instead of all of them")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one k_copy of the whole state (HBM counter calibration)")
     args = ap.parse_args()
@@ -187,6 +275,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.config == "precision":
+        run_precision(args)
+        return
     if args.config == "batch":
         run_batch(args)
         return
@@ -198,7 +289,8 @@ def main():
 
     import qlx
 
-    nq, depth = args.qubits, args.depth
+    nq, depth = args.qubits or 30, args.depth or 20
+    args.seed = args.seed or 1002
     prog = qlx.layered(nq, depth, args.seed)
     modes = ["windowed", "plain"] if args.mode == "auto" else [args.mode]
     st = qlx.State(nq, "c128")

@@ -90,8 +90,18 @@ typedef struct qlx_exec_opts {
     uint32_t register_qubits; /* qubits per work-item's registers in a window; 0 = default
                                  (c128: 4, c64: 5). Built (window, register) pairs:
                                  c128 (12,4) (12,5) (11,4) (11,3); c64 (13,5) (12,5) (12,4) */
-    uint32_t reserved[3];
+    uint32_t dense_kernel;    /* QLX_DENSE_*: arithmetic of UNITARY / FUSED_BLOCK gates */
+    uint32_t reserved[2];
 } qlx_exec_opts;
+
+/* Dense-gate arithmetic (qlx_exec_opts::dense_kernel). The MFMA forms apply to 4- and
+ * 5-qubit gates on complex64 states; every other dense gate runs on the vector units
+ * (FP64 for complex128, FP32 for complex64). The matrices stay FP64 in the pool and are
+ * converted on the device. */
+#define QLX_DENSE_AUTO        0  /* complex64 4/5-qubit gates: MFMA BF16X3 */
+#define QLX_DENSE_VALU        1  /* vector units only */
+#define QLX_DENSE_MFMA_BF16X3 2  /* BF16 hi/lo split, three products accumulated in FP32 */
+#define QLX_DENSE_MFMA_F32    3  /* FP32 MFMA (v_mfma_f32_32x32x2_f32) */
 
 /* Statistics of the last qlx_execute on a state (sweep accounting). */
 typedef struct qlx_exec_stats {
@@ -224,7 +234,9 @@ typedef struct qlx_transpile_opts {
     uint32_t swap_penalty;
     uint32_t restore_layout;      /* append GLOBAL_SWAPs restoring logical qubit order */
     int32_t  dtype;               /* precision the window is sized for (QLX_C128 / QLX_C64) */
-    uint32_t reserved[7];
+    uint32_t build_on_device;     /* form the fused-block matrices in FP64 on the current GPU
+                                     (the host decides the blocks); 0 = on the host */
+    uint32_t reserved[6];
 } qlx_transpile_opts;
 typedef struct qlx_program qlx_program;
 int qlx_transpile(uint32_t num_qubits, const qlx_gate_instr* instrs, size_t n,
@@ -249,7 +261,7 @@ int qlx_gpu_safe_qubit_limit(int device, int dtype, uint32_t* k_safe);
 typedef struct qlx_batch qlx_batch;
 typedef struct qlx_batch_info {
     uint64_t circuits;
-    uint64_t tasks;            /* chunk tasks of one persistent run */
+    uint64_t tasks;            /* (circuit, sweep, chunk) tasks of one run */
     uint64_t sweeps;           /* window sweeps over all circuits */
     uint64_t state_bytes;      /* arena size */
     uint64_t gates;            /* instructions over all circuits */

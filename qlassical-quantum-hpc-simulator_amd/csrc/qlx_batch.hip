@@ -6,16 +6,15 @@
 // whole small circuit is one on-chip task: its state is synthesised as |0...0> in
 // registers, every gate runs there, and it is written to HBM once.
 //
-// QLX_BATCH_PERSISTENT: one launch of `k_batch` with one work-group per LDS slot. Work
-// comes from a queue of (circuit, sweep, chunk) tasks ordered by sweep level across
-// circuits. A task of sweep s waits until its circuit's counter shows every chunk of
-// sweeps < s done (thread 0 spins with acquire loads; a bounded spin sets an error flag
-// instead of hanging); after a task the work-group publishes its stores (release fence)
-// and bumps the counter. A work-group only ever waits for tasks that come earlier in the
-// queue, all of which were taken by running work-groups, so the queue always drains.
-// QLX_BATCH_LEVELS: the same tasks, one launch per sweep level (kernel boundaries order
-// the levels; no device-side waiting). QLX_BATCH_PER_CIRCUIT(_GRAPH): the baseline — each
-// circuit's gates as one-kernel-per-gate launches (optionally captured once into a hipGraph).
+// QLX_BATCH_PERSISTENT: one launch of `k_batch` with one work-group per LDS slot of the
+// device. A work-group takes whole circuits from a queue (largest amount of work first,
+// so the long circuits start early) and runs every sweep of its circuit itself: chunk
+// after chunk, with a work-group barrier between sweeps. No work-group ever waits for
+// another; the launch ends when the queue is empty.
+// QLX_BATCH_LEVELS: the (circuit, sweep, chunk) tasks, one launch per sweep level
+// (kernel boundaries order the levels; chunks of a level run in parallel).
+// QLX_BATCH_PER_CIRCUIT(_GRAPH): the baseline — each circuit's gates as
+// one-kernel-per-gate launches (optionally captured once into a hipGraph).
 
 #include <algorithm>
 #include <cstring>
@@ -35,75 +34,55 @@ struct BatchCircuitDev {
 };
 
 struct BatchTaskDev {
-    uint32_t circuit, sweep, chunk, need;  // need: chunks of this circuit done before this sweep
+    uint32_t circuit, sweep, chunk, pad;
 };
 
-constexpr uint32_t kSpinLimit = 1u << 22;
-
 template <class A>
-__device__ __forceinline__ void run_task_chunk(A* arena, const BatchCircuitDev& c, const BatchTaskDev& tk,
+__device__ __forceinline__ void run_task_chunk(A* arena, const BatchCircuitDev& c, uint32_t sweep, uint32_t chunk,
                                                const DevSweep* sw, const DevPhase* ph, const DevOp* ops,
                                                const double* coef, A* lds) {
-    const DevSweep* s = sw + c.sweep_begin + tk.sweep;
+    const DevSweep* s = sw + c.sweep_begin + sweep;
     A* amp = arena + c.amp_off;
-    const bool init = tk.sweep == 0;
+    const bool init = sweep == 0;
     if constexpr (sizeof(A) == 8) {
         switch (c.rb) {
-            case 2: run_chunk<A, 8, 2>(amp, s, tk.chunk, ph, ops, coef, lds, init); break;
-            case 3: run_chunk<A, 8, 3>(amp, s, tk.chunk, ph, ops, coef, lds, init); break;
-            case 4: run_chunk<A, 8, 4>(amp, s, tk.chunk, ph, ops, coef, lds, init); break;
-            default: run_chunk<A, 8, 5>(amp, s, tk.chunk, ph, ops, coef, lds, init); break;
+            case 2: run_chunk<A, 8, 2>(amp, s, chunk, ph, ops, coef, lds, init); break;
+            case 3: run_chunk<A, 8, 3>(amp, s, chunk, ph, ops, coef, lds, init); break;
+            case 4: run_chunk<A, 8, 4>(amp, s, chunk, ph, ops, coef, lds, init); break;
+            default: run_chunk<A, 8, 5>(amp, s, chunk, ph, ops, coef, lds, init); break;
         }
     } else {
         switch (c.rb) {
-            case 2: run_chunk<A, 8, 2>(amp, s, tk.chunk, ph, ops, coef, lds, init); break;
-            case 3: run_chunk<A, 8, 3>(amp, s, tk.chunk, ph, ops, coef, lds, init); break;
-            default: run_chunk<A, 8, 4>(amp, s, tk.chunk, ph, ops, coef, lds, init); break;
+            case 2: run_chunk<A, 8, 2>(amp, s, chunk, ph, ops, coef, lds, init); break;
+            case 3: run_chunk<A, 8, 3>(amp, s, chunk, ph, ops, coef, lds, init); break;
+            default: run_chunk<A, 8, 4>(amp, s, chunk, ph, ops, coef, lds, init); break;
         }
     }
 }
 
+// ctl[0]: next queue position (zeroed before the launch).
 template <class A>
 __global__ void __launch_bounds__(256) k_batch(A* __restrict__ arena, const BatchCircuitDev* __restrict__ circ,
-                                              const BatchTaskDev* __restrict__ tasks, uint32_t n_tasks,
+                                              const uint32_t* __restrict__ queue, uint32_t n_circ,
                                               const DevSweep* __restrict__ sw, const DevPhase* __restrict__ ph,
                                               const DevOp* __restrict__ ops, const double* __restrict__ coef,
                                               uint32_t* __restrict__ ctl) {
     extern __shared__ __align__(16) unsigned char smem[];
     A* lds = reinterpret_cast<A*>(smem);
-    __shared__ uint32_t s_task, s_ok;
-    uint32_t* next = ctl;
-    uint32_t* err = ctl + 1;
-    uint32_t* done = ctl + 2;
+    __shared__ uint32_t s_job;
     for (;;) {
-        if (threadIdx.x == 0) s_task = atomicAdd(next, 1u);
+        if (threadIdx.x == 0) s_job = atomicAdd(ctl, 1u);
         __syncthreads();
-        const uint32_t ti = s_task;
-        if (ti >= n_tasks) break;
-        const BatchTaskDev tk = tasks[ti];
-        if (threadIdx.x == 0) {
-            uint32_t ok = 1;
-            if (tk.need) {
-                uint32_t it = 0;
-                while (__hip_atomic_load(done + tk.circuit, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < tk.need) {
-                    if (++it > kSpinLimit) {
-                        atomicOr(err, 1u);
-                        ok = 0;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(2);
-                }
-            }
-            s_ok = ok;
+        const uint32_t j = s_job;
+        __syncthreads();  // every work-item holds j before s_job is written again
+        if (j >= n_circ) break;
+        const BatchCircuitDev c = circ[queue[j]];
+        const uint32_t nch = 1u << c.log_chunks;
+        for (uint32_t s = 0; s < c.n_sweeps; ++s) {
+            for (uint32_t k = 0; k < nch; ++k) run_task_chunk<A>(arena, c, s, k, sw, ph, ops, coef, lds);
+            // the sweep's stores are visible to the whole work-group before the next sweep reads
+            __syncthreads();
         }
-        __syncthreads();
-        if (s_ok) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            run_task_chunk<A>(arena, circ[tk.circuit], tk, sw, ph, ops, coef, lds);
-        }
-        __threadfence();  // release: this work-item's stores are visible device-wide
-        __syncthreads();
-        if (threadIdx.x == 0) atomicAdd(done + tk.circuit, 1u);
     }
 }
 
@@ -115,7 +94,7 @@ __global__ void __launch_bounds__(256) k_batch_level(A* __restrict__ arena, cons
                                                     const DevOp* __restrict__ ops, const double* __restrict__ coef) {
     extern __shared__ __align__(16) unsigned char smem[];
     const BatchTaskDev tk = tasks[first + blockIdx.x];
-    run_task_chunk<A>(arena, circ[tk.circuit], tk, sw, ph, ops, coef, reinterpret_cast<A*>(smem));
+    run_task_chunk<A>(arena, circ[tk.circuit], tk.sweep, tk.chunk, sw, ph, ops, coef, reinterpret_cast<A*>(smem));
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -143,8 +122,9 @@ struct qlx_batch {
     bool prepared = false;
     // device
     DevBuf arena, desc, pool, ctl, scratch;
-    size_t off_circ = 0, off_task = 0, off_sw = 0, off_ph = 0, off_op = 0, off_cf = 0;
+    size_t off_circ = 0, off_task = 0, off_queue = 0, off_sw = 0, off_ph = 0, off_op = 0, off_cf = 0;
     std::vector<BatchTaskDev> tasks;
+    std::vector<uint32_t> queue;  // circuits in persistent-queue order
     std::vector<uint32_t> level_first, level_count;
     uint32_t grid = 0;
     size_t lds_bytes = 0;
@@ -193,14 +173,13 @@ Geometry batch_geometry(int dtype, uint32_t nq, const qlx_exec_opts& o) {
 }
 
 void launch_persistent(qlx_batch* b) {
-    const uint32_t nctl = 2 + (uint32_t)b->circs.size();
-    k_zero_u32<<<std::max(1u, std::min(1024u, (nctl + 255) / 256)), 256, 0, b->stream>>>((uint32_t*)b->ctl.ptr, nctl);
+    k_zero_u32<<<1, 64, 0, b->stream>>>((uint32_t*)b->ctl.ptr, 1u);
     const char* d = (const char*)b->desc.ptr;
-    const uint32_t nt = (uint32_t)b->tasks.size();
+    const uint32_t nc = (uint32_t)b->queue.size();
     auto go = [&](auto* arena) {
         using A = std::remove_pointer_t<decltype(arena)>;
         k_batch<A><<<b->grid, 256, b->lds_bytes, b->stream>>>(
-            arena, (const BatchCircuitDev*)(d + b->off_circ), (const BatchTaskDev*)(d + b->off_task), nt,
+            arena, (const BatchCircuitDev*)(d + b->off_circ), (const uint32_t*)(d + b->off_queue), nc,
             (const DevSweep*)(d + b->off_sw), (const DevPhase*)(d + b->off_ph), (const DevOp*)(d + b->off_op),
             (const double*)(d + b->off_cf), (uint32_t*)b->ctl.ptr);
     };
@@ -345,14 +324,20 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
                 const BatchCircuitDev& d = cdev[ci];
                 if (L >= d.n_sweeps) continue;
                 const uint32_t nch = 1u << d.log_chunks;
-                for (uint32_t k = 0; k < nch; ++k) b->tasks.push_back(BatchTaskDev{(uint32_t)ci, L, k, L * nch});
+                for (uint32_t k = 0; k < nch; ++k) b->tasks.push_back(BatchTaskDev{(uint32_t)ci, L, k, 0});
             }
             b->level_count.push_back((uint32_t)b->tasks.size() - b->level_first.back());
         }
+        // persistent queue: most chunk-sweeps first (longest-processing-time order)
+        b->queue.resize(cdev.size());
+        for (size_t ci = 0; ci < cdev.size(); ++ci) b->queue[ci] = (uint32_t)ci;
+        auto work = [&](uint32_t ci) { return (uint64_t)cdev[ci].n_sweeps << cdev[ci].log_chunks; };
+        std::stable_sort(b->queue.begin(), b->queue.end(), [&](uint32_t x, uint32_t y) { return work(x) > work(y); });
         // descriptor image
         b->off_circ = 0;
         b->off_task = align_up(cdev.size() * sizeof(BatchCircuitDev), 256);
-        b->off_sw = align_up(b->off_task + b->tasks.size() * sizeof(BatchTaskDev), 256);
+        b->off_queue = align_up(b->off_task + b->tasks.size() * sizeof(BatchTaskDev), 256);
+        b->off_sw = align_up(b->off_queue + b->queue.size() * sizeof(uint32_t), 256);
         b->off_ph = align_up(b->off_sw + dsw.size() * sizeof(DevSweep), 256);
         b->off_op = align_up(b->off_ph + dph.size() * sizeof(DevPhase), 256);
         b->off_cf = align_up(b->off_op + dop.size() * sizeof(DevOp), 256);
@@ -360,6 +345,7 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
         std::vector<char> img(total, 0);
         std::memcpy(img.data() + b->off_circ, cdev.data(), cdev.size() * sizeof(BatchCircuitDev));
         std::memcpy(img.data() + b->off_task, b->tasks.data(), b->tasks.size() * sizeof(BatchTaskDev));
+        std::memcpy(img.data() + b->off_queue, b->queue.data(), b->queue.size() * sizeof(uint32_t));
         std::memcpy(img.data() + b->off_sw, dsw.data(), dsw.size() * sizeof(DevSweep));
         std::memcpy(img.data() + b->off_ph, dph.data(), dph.size() * sizeof(DevPhase));
         if (!dop.empty()) std::memcpy(img.data() + b->off_op, dop.data(), dop.size() * sizeof(DevOp));
@@ -367,7 +353,7 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
         b->desc.reserve(total, b->stream);
         QLX_HIP_CHECK(hipMemcpyAsync(b->desc.ptr, img.data(), total, hipMemcpyHostToDevice, b->stream));
         b->arena.reserve(amp_total * amp, b->stream);
-        b->ctl.reserve((2 + cdev.size()) * sizeof(uint32_t), b->stream);
+        b->ctl.reserve(sizeof(uint32_t), b->stream);
         if (pool_total) {
             std::vector<double> pool_img(2 * pool_total);
             for (const auto& c : b->circs)
@@ -398,7 +384,7 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
         QLX_HIP_CHECK(hipGetDevice(&dev));
         QLX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         b->grid = (uint32_t)std::max(1, per_cu) * (uint32_t)cus;
-        b->grid = std::min<uint32_t>(b->grid, std::max<uint32_t>(1, (uint32_t)b->tasks.size()));
+        b->grid = std::min<uint32_t>(b->grid, (uint32_t)b->queue.size());
         b->prepared = true;
         b->info = qlx_batch_info{};
         b->info.circuits = b->circs.size();
@@ -459,11 +445,6 @@ int qlx_batch_sync(qlx_batch* b, double* elapsed_ms) {
         if (!b) fail(QLX_EINVAL, "null batch");
         QLX_HIP_CHECK(hipStreamSynchronize(b->stream));
         QLX_HIP_CHECK(hipGetLastError());
-        if (b->prepared && b->ctl.ptr) {
-            uint32_t err = 0;
-            QLX_HIP_CHECK(hipMemcpy(&err, (uint32_t*)b->ctl.ptr + 1, sizeof(err), hipMemcpyDeviceToHost));
-            if (err) fail(QLX_EHIP, "batched run: a task waited past the spin limit");
-        }
         if (elapsed_ms) {
             float ms = 0.f;
             QLX_HIP_CHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));

@@ -474,9 +474,16 @@ void launch_plain_instr(qlx_state* st, const qlx_gate_instr& g, const double* dp
             const int k = instr_arity(g);
             uint8_t qs[5];
             for (int j = 0; j < k; ++j) qs[j] = (uint8_t)instr_qubit(g, j);
-            const qlx_matrix_entry& e = meta[g.payload.matrix_idx];
-            TimedLaunch tl(st, "k_dense", pass);
-            launch_dense(st, dpool + 2 * (size_t)e.offset, qs, k);
+            const uint32_t mi = g.payload.matrix_idx;
+            const qlx_matrix_entry& e = meta[mi];
+            if (mi < st->frag_index.size() && st->frag_index[mi] >= 0) {
+                TimedLaunch tl(st, "k_dense_mfma", pass);
+                launch_dense_mfma(st, (const unsigned char*)st->frags.ptr + st->frag_index[mi], qs, k,
+                                  st->frag_variant);
+            } else {
+                TimedLaunch tl(st, "k_dense", pass);
+                launch_dense(st, dpool + 2 * (size_t)e.offset, qs, k);
+            }
             account(1.0);
             break;
         }
@@ -488,12 +495,13 @@ void launch_plain_instr(qlx_state* st, const qlx_gate_instr& g, const double* dp
 }
 
 void execute_plain(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const double* pool,
-                   size_t pool_elems, const qlx_matrix_entry* meta, size_t,
+                   size_t pool_elems, const qlx_matrix_entry* meta, size_t n_meta,
                    const qlx_exec_opts& opts, qlx_exec_stats& stats) {
     bool need_pool = false;
     for (size_t i = 0; i < n; ++i)
         if (instrs[i].type == QLX_UNITARY || instrs[i].type == QLX_FUSED_BLOCK) need_pool = true;
     const double* dpool = need_pool ? upload_pool(st, pool, pool_elems) : nullptr;
+    prepare_dense(st, instrs, n, dpool, meta, n_meta, opts);
     for (size_t i = 0; i < n; ++i) launch_plain_instr(st, instrs[i], dpool, meta, opts, stats);
 }
 

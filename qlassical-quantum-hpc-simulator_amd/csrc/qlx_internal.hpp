@@ -92,6 +92,10 @@ struct qlx_state {
     qlx::DevBuf scratch;          // reduction partials
     qlx::DevBuf prog;             // uploaded gate descriptors / matrices
     std::shared_ptr<void> sweep_cache;  // last windowed plan (host plan + device image)
+    qlx::DevBuf frags;            // MFMA operand fragments of the current program's dense gates
+    qlx::DevBuf frag_desc;        // their build descriptors
+    std::vector<int64_t> frag_index;  // per matrix_idx: byte offset into frags, or -1
+    int frag_variant = 0;         // QLX_DENSE_MFMA_* the fragments were built for
     qlx_exec_stats stats{};
 };
 
@@ -145,6 +149,40 @@ void launch_plain_instr(qlx_state* st, const qlx_gate_instr& g, const double* dp
 // Dense k-qubit unitary on the device state (k <= 5), matrix already on device
 // as row-major complex128. Used by both executors.
 void launch_dense(qlx_state* st, const double* d_matrix, const uint8_t* qubits, int k);
+
+// Matrix-core path for 4/5-qubit dense gates on complex64 states (qlx_mfma.hip).
+struct MfmaSplitJob {
+    uint64_t pool_off;  // complex elements into the device pool
+    uint64_t frag_off;  // bytes into the fragment buffer
+    int k;              // 4 or 5
+    int variant;        // QLX_DENSE_MFMA_BF16X3 | QLX_DENSE_MFMA_F32
+};
+size_t mfma_fragment_bytes(int k, int variant);
+// Builds the A-operand fragments of each job from the FP64 pool (synchronous).
+void mfma_split(const double* d_pool, const MfmaSplitJob* jobs, size_t n, void* d_frags, DevBuf& desc_buf,
+                hipStream_t stream);
+void launch_dense_mfma(qlx_state* st, const void* d_frag, const uint8_t* qubits, int k, int variant);
+// Resolves opts.dense_kernel for the program's dense gates and builds the MFMA
+// fragments they need (st->frag_index); call after upload_pool.
+void prepare_dense(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const double* dpool,
+                   const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts& opts);
+
+// FP64 fused-block products on the GPU (qlx_fusion.hip). A recipe lists a block's
+// members in program order; member i's matrix (2^a x 2^a complex128, row-major, bit j
+// <-> operand j) starts at complex element mat_off of member_mats and operand j sits on
+// block-local bit pos[j]. `out` receives the blocks' 4^k-element matrices back to back
+// (interleaved re/im), in recipe order.
+struct FusedMember {
+    uint32_t mat_off;
+    int a;
+    int pos[5];
+};
+struct FusedRecipe {
+    int k;
+    std::vector<FusedMember> members;
+};
+void build_fused_on_device(const std::vector<FusedRecipe>& recipes, const std::vector<double>& member_mats,
+                           std::vector<double>& out);
 
 // Instruction helpers shared by the executors.
 inline int instr_arity(const qlx_gate_instr& g) {

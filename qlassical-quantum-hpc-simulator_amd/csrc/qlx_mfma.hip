@@ -1,0 +1,311 @@
+// qlx_mfma.hip — dense 4/5-qubit unitaries on complex64 states as matrix-core (MFMA)
+// contractions on gfx950.
+//
+// The reference applies a UNITARY / FUSED_BLOCK gate group by group as a dense
+// matrix-vector product (include/backend/cpu_openmp_engine.hpp:539-606: local index
+// bit i <-> qubits[i], v_out = U * v_in). Here 32 such groups form the columns of one
+// real GEMM per wave:
+//
+//     D[2^(K+1) x 32] = A[2^(K+1) x 2^(K+1)] * X[2^(K+1) x 32]
+//
+// with the real embedding A[2o+0][2l+0] = Re U[o][l], A[2o+0][2l+1] = -Im U[o][l],
+// A[2o+1][2l+0] = Im U[o][l], A[2o+1][2l+1] = Re U[o][l], and X[2l+c][j] the real (c=0)
+// and imaginary (c=1) part of amplitude l of group j. Two arithmetic forms:
+//   * BF16X3: v_mfma_f32_32x32x16_bf16 on a hi/lo split of both operands
+//     (a = a_hi + a_lo, each bf16), three products lo*hi + hi*lo + hi*hi accumulated
+//     in FP32 (the lo*lo term, <= 2^-18 relative, is dropped);
+//   * F32:    v_mfma_f32_32x32x2_f32 on the FP32 operands directly.
+// A is built once per gate from the FP64 matrix on the device (k_mfma_split) and held
+// in VGPRs for the whole launch; X is read from HBM straight into MFMA operand
+// registers and the result is written back in place (a wave reads its 32 groups
+// completely before writing them; waves own disjoint groups).
+//
+// MFMA operand layout used (wave64; CDNA3/CDNA4 32x32 forms): lane l holds row/column
+// l % 32 of A/B and the K-slice selected by l / 32; the 32x32 FP32 result sits in 16
+// VGPRs with element (row 8*(v/4) + 4*(l/32) + v%4, column l%32) in VGPR v of lane l.
+// Only the pairing of A and B slices matters for K, so the K order below is chosen for
+// the loads: lane half h of column j holds amplitudes [h*2^(K-1), (h+1)*2^(K-1)) of
+// group j, real and imaginary parts interleaved as in memory.
+
+#include <algorithm>
+
+#include "qlx_internal.hpp"
+
+namespace qlx {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kWaves = 4;
+constexpr int kBlock = 64 * kWaves;
+
+__device__ inline uint64_t ins0(uint64_t idx, uint32_t t) {
+    const uint64_t lo = idx & ((1ull << t) - 1);
+    return lo | ((idx ^ lo) << 1);
+}
+
+// Real-embedded element (row, col) of the complex D x D row-major matrix U.
+__device__ inline double embed(const double* __restrict__ U, int D, int row, int col) {
+    const int o = row >> 1, l = col >> 1;
+    const double ur = U[2 * (o * D + l)], ui = U[2 * (o * D + l) + 1];
+    if ((row & 1) == 0) return (col & 1) ? -ui : ur;
+    return (col & 1) ? ur : ui;
+}
+
+// BF16X3: the column of X (and of A) that slot s of K-step t of lane half h carries.
+__host__ __device__ constexpr int kcol_bf16(int K, int t, int h, int s) {
+    return 2 * ((h << (K - 1)) + 4 * t + (s >> 1)) + (s & 1);
+}
+// F32: K-step t of lane half h (one value per lane).
+__host__ __device__ constexpr int kcol_f32(int K, int t, int h) {
+    return 2 * ((h << (K - 1)) + (t >> 1)) + (t & 1);
+}
+
+struct SplitDesc {
+    uint64_t pool_off;   // complex elements into the device pool
+    uint64_t frag_off;   // bytes into the fragment buffer
+    uint32_t k;          // 4 or 5
+    uint32_t f32;        // 1: FP32 fragments, 0: BF16 hi/lo fragments
+};
+
+// One work-group per matrix: writes the A-operand fragments of every (M-tile, K-step, lane).
+__global__ void __launch_bounds__(kBlock) k_mfma_split(const double* __restrict__ pool,
+                                                      const SplitDesc* __restrict__ descs,
+                                                      unsigned char* __restrict__ frags) {
+    const SplitDesc d = descs[blockIdx.x];
+    const int K = (int)d.k, D = 1 << K;
+    const double* U = pool + 2 * d.pool_off;
+    const int MT = (2 * D) / 32;
+    if (d.f32) {
+        const int T = D;  // 2D / 2 K-steps
+        float* out = (float*)(frags + d.frag_off);
+        for (int e = threadIdx.x; e < MT * T * 64; e += kBlock) {
+            const int lane = e & 63, t = (e >> 6) % T, m = (e >> 6) / T;
+            out[e] = (float)embed(U, D, 32 * m + (lane & 31), kcol_f32(K, t, lane >> 5));
+        }
+    } else {
+        const int T = (2 * D) / 16;
+        bf16x8* hi = (bf16x8*)(frags + d.frag_off);
+        bf16x8* lo = hi + MT * T * 64;
+        for (int e = threadIdx.x; e < MT * T * 64; e += kBlock) {
+            const int lane = e & 63, t = (e >> 6) % T, m = (e >> 6) / T;
+            bf16x8 vh, vl;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const double v = embed(U, D, 32 * m + (lane & 31), kcol_bf16(K, t, lane >> 5, s));
+                const __bf16 h = (__bf16)(float)v;
+                vh[s] = h;
+                vl[s] = (__bf16)(float)(v - (double)(float)h);
+            }
+            hi[e] = vh;
+            lo[e] = vl;
+        }
+    }
+}
+
+struct MfmaArgs {
+    uint32_t sorted[5];   // target qubits ascending (zero-hole insertion order)
+    uint32_t in_off[16];  // offset of local amplitude r < 2^(K-1) (bit K-1 of r clear)
+    uint32_t out_off[16]; // offset of output slot (m, a, b): local 16m + 4a + b (bit 1 clear)
+    uint32_t hbit_in;     // 1 << qubits[K-1]: lane half 1 reads the upper half of a group
+    uint32_t hbit_out;    // 1 << qubits[1]:   lane half 1 writes locals with bit 1 set
+};
+
+template <int K, bool F32>
+__global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp, uint64_t ngroups,
+                                                      const unsigned char* __restrict__ frags, MfmaArgs ma) {
+    constexpr int D = 1 << K;          // amplitudes per group
+    constexpr int HALF = D / 2;        // amplitudes per lane
+    constexpr int MT = (2 * D) / 32;   // 32-row tiles of the real embedding
+    const int lane = threadIdx.x & 63;
+    const int h = lane >> 5, j = lane & 31;
+    const uint64_t hin = h ? ma.hbit_in : 0u, hout = h ? ma.hbit_out : 0u;
+
+    constexpr int TB = (2 * D) / 16;   // BF16X3 K-steps
+    constexpr int TF = D;              // F32 K-steps
+    bf16x8 ah[F32 ? 1 : MT][F32 ? 1 : TB], al[F32 ? 1 : MT][F32 ? 1 : TB];
+    float af[F32 ? MT : 1][F32 ? TF : 1];
+    if constexpr (F32) {
+        const float* f = (const float*)frags;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int t = 0; t < TF; ++t) af[m][t] = f[(m * TF + t) * 64 + lane];
+    } else {
+        const bf16x8* fh = (const bf16x8*)frags;
+        const bf16x8* fl = fh + MT * TB * 64;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int t = 0; t < TB; ++t) {
+                ah[m][t] = fh[(m * TB + t) * 64 + lane];
+                al[m][t] = fl[(m * TB + t) * 64 + lane];
+            }
+    }
+
+    const uint64_t ntiles = (ngroups + 31) / 32;
+    const uint64_t wave0 = (uint64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * kWaves;
+    for (uint64_t tile = wave0; tile < ntiles; tile += nwaves) {
+        const uint64_t g = tile * 32 + j;
+        const bool valid = g < ngroups;
+        uint64_t base = g;
+#pragma unroll
+        for (int q = 0; q < K; ++q) base = ins0(base, ma.sorted[q]);
+        float2 x[HALF];
+#pragma unroll
+        for (int r = 0; r < HALF; ++r) x[r] = valid ? amp[base + hin + ma.in_off[r]] : float2{0.f, 0.f};
+
+        f32x16 acc[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[m][v] = 0.f;
+
+        if constexpr (F32) {
+#pragma unroll
+            for (int t = 0; t < TF; ++t) {
+                const float xv = (t & 1) ? x[t >> 1].y : x[t >> 1].x;
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[m][t], xv, acc[m], 0, 0, 0);
+            }
+        } else {
+            bf16x8 xh[TB], xl[TB];
+#pragma unroll
+            for (int t = 0; t < TB; ++t)
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    const float v = (s & 1) ? x[4 * t + (s >> 1)].y : x[4 * t + (s >> 1)].x;
+                    const __bf16 hv = (__bf16)v;
+                    xh[t][s] = hv;
+                    xl[t][s] = (__bf16)(v - (float)hv);
+                }
+            // small terms first
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[m][t], xh[t], acc[m], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m][t], xl[t], acc[m], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m][t], xh[t], acc[m], 0, 0, 0);
+            }
+        }
+        if (valid) {
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+                        amp[base + hout + ma.out_off[m * 8 + a * 2 + b]] =
+                            float2{acc[m][4 * a + 2 * b], acc[m][4 * a + 2 * b + 1]};
+        }
+    }
+}
+
+size_t frag_bytes(int k, bool f32) {
+    const size_t D = size_t{1} << k, MT = 2 * D / 32;
+    return f32 ? MT * D * 64 * sizeof(float) : 2 * MT * (2 * D / 16) * 64 * 16;
+}
+
+}  // namespace
+
+size_t mfma_fragment_bytes(int k, int variant) {
+    if (k != 4 && k != 5) fail(QLX_EUNSUPPORTED, "MFMA dense gates take 4 or 5 qubits");
+    return frag_bytes(k, variant == QLX_DENSE_MFMA_F32);
+}
+
+void mfma_split(const double* d_pool, const MfmaSplitJob* jobs, size_t n, void* d_frags, DevBuf& desc_buf,
+                hipStream_t stream) {
+    if (!n) return;
+    std::vector<SplitDesc> h(n);
+    for (size_t i = 0; i < n; ++i) {
+        if (jobs[i].k != 4 && jobs[i].k != 5) fail(QLX_EUNSUPPORTED, "MFMA dense gates take 4 or 5 qubits");
+        h[i] = SplitDesc{jobs[i].pool_off, jobs[i].frag_off, (uint32_t)jobs[i].k,
+                         jobs[i].variant == QLX_DENSE_MFMA_F32 ? 1u : 0u};
+    }
+    desc_buf.reserve(n * sizeof(SplitDesc), stream);
+    QLX_HIP_CHECK(hipMemcpyAsync(desc_buf.ptr, h.data(), n * sizeof(SplitDesc), hipMemcpyHostToDevice, stream));
+    k_mfma_split<<<(unsigned)n, kBlock, 0, stream>>>(d_pool, (const SplitDesc*)desc_buf.ptr,
+                                                     (unsigned char*)d_frags);
+    QLX_HIP_CHECK(hipGetLastError());
+    // the descriptors live in pageable host memory
+    QLX_HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void launch_dense_mfma(qlx_state* st, const void* d_frag, const uint8_t* qubits, int k, int variant) {
+    if (st->dtype != QLX_C64) fail(QLX_EUNSUPPORTED, "MFMA dense gates run on complex64 states");
+    if (k != 4 && k != 5) fail(QLX_EUNSUPPORTED, "MFMA dense gates take 4 or 5 qubits");
+    if ((uint32_t)k > st->nq) fail(QLX_ERANGE, "gate arity exceeds the qubit count");
+    MfmaArgs ma{};
+    auto off = [&](int loc) {
+        uint32_t o = 0;
+        for (int i = 0; i < k; ++i)
+            if ((loc >> i) & 1) o |= 1u << qubits[i];
+        return o;
+    };
+    const int half = 1 << (k - 1);
+    for (int r = 0; r < half; ++r) ma.in_off[r] = off(r);
+    const int mt = (1 << (k + 1)) / 32;
+    for (int m = 0; m < mt; ++m)
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 2; ++b) ma.out_off[m * 8 + a * 2 + b] = off(16 * m + 4 * a + b);
+    ma.hbit_in = 1u << qubits[k - 1];
+    ma.hbit_out = 1u << qubits[1];
+    uint32_t s[5];
+    for (int i = 0; i < k; ++i) s[i] = qubits[i];
+    std::sort(s, s + k);
+    for (int i = 0; i < k; ++i) ma.sorted[i] = s[i];
+
+    const uint64_t ngroups = 1ull << (st->nq - k);
+    const uint64_t tiles = (ngroups + 31) / 32;
+    // enough resident waves to keep HBM busy; every wave loops over its tiles
+    const uint64_t want = (tiles + kWaves - 1) / kWaves;
+    const unsigned grid = (unsigned)std::min<uint64_t>(want, 256ull * 8);
+    float2* amp = (float2*)st->amp;
+    const auto* f = (const unsigned char*)d_frag;
+    const bool f32 = variant == QLX_DENSE_MFMA_F32;
+    if (k == 5) {
+        if (f32) k_dense_mfma<5, true><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+        else k_dense_mfma<5, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+    } else {
+        if (f32) k_dense_mfma<4, true><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+        else k_dense_mfma<4, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+    }
+    QLX_HIP_CHECK(hipGetLastError());
+}
+
+void prepare_dense(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const double* dpool,
+                   const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts& opts) {
+    st->frag_index.clear();
+    const uint32_t dk = opts.dense_kernel;
+    if (dk > QLX_DENSE_MFMA_F32) fail(QLX_EINVAL, "unknown dense_kernel");
+    if (st->dtype != QLX_C64 || dk == QLX_DENSE_VALU || !dpool || !meta) return;
+    const int variant = dk == QLX_DENSE_MFMA_F32 ? QLX_DENSE_MFMA_F32 : QLX_DENSE_MFMA_BF16X3;
+    std::vector<MfmaSplitJob> jobs;
+    size_t total = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const qlx_gate_instr& g = instrs[i];
+        if (g.type != QLX_UNITARY && g.type != QLX_FUSED_BLOCK) continue;
+        const int k = instr_arity(g);
+        const uint32_t mi = g.payload.matrix_idx;
+        if ((k != 4 && k != 5) || (uint32_t)k > st->nq || mi >= n_meta || meta[mi].num_qubits != k) continue;
+        if (st->frag_index.empty()) st->frag_index.assign(n_meta, -1);
+        if (st->frag_index[mi] >= 0) continue;
+        st->frag_index[mi] = (int64_t)total;
+        jobs.push_back(MfmaSplitJob{meta[mi].offset, total, k, variant});
+        total += (mfma_fragment_bytes(k, variant) + 255) & ~size_t{255};
+    }
+    if (jobs.empty()) return;
+    st->frags.reserve(total, st->stream);
+    mfma_split(dpool, jobs.data(), jobs.size(), st->frags.ptr, st->frag_desc, st->stream);
+    st->frag_variant = variant;
+}
+
+}  // namespace qlx

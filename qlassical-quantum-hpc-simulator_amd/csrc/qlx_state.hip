@@ -294,6 +294,8 @@ static void destroy_state(qlx_state* st) {
     if (st->amp) (void)hipFree(st->amp);
     st->scratch.release();
     st->prog.release();
+    st->frags.release();
+    st->frag_desc.release();
     for (auto& r : st->pending) {
         (void)hipEventDestroy(r.e0);
         (void)hipEventDestroy(r.e1);

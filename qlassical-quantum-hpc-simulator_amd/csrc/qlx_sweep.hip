@@ -560,6 +560,7 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
     for (const PlanStep& s : p.steps)
         if (!s.sweep && (instrs[s.src].type == QLX_UNITARY || instrs[s.src].type == QLX_FUSED_BLOCK)) need_pool = true;
     const double* dpool = need_pool ? upload_pool(st, pool, pool_elems) : nullptr;
+    prepare_dense(st, instrs, n, dpool, meta, n_meta, opts);
     const char* dev = (const char*)cp.dev.ptr;
     const double pass = pass_bytes(st);
     for (const PlanStep& s : p.steps) {

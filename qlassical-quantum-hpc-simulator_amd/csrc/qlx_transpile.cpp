@@ -218,8 +218,15 @@ std::vector<qlx_gate_instr> release_dag(Dag&& dag) {
 
 bool in_list(const std::vector<uint32_t>& v, uint32_t x) { return std::find(v.begin(), v.end(), x) != v.end(); }
 
+// Blocks whose matrices are formed on the device after the pass (build_on_device).
+struct DeviceFusion {
+    std::vector<FusedRecipe> recipes;
+    std::vector<uint32_t> pool_offset;  // where each block's matrix goes in the pool
+    std::vector<double> member_mats;
+};
+
 // ---- pass_gate_fusion (middleware.hpp:526-753) ------------------------------------------------
-void gate_fusion(Dag& dag, Pool& pool, int max_width, bool rotations) {
+void gate_fusion(Dag& dag, Pool& pool, int max_width, bool rotations, DeviceFusion* dev) {
     auto& N = dag.nodes;
     for (size_t i = 0; i < N.size(); ++i) {
         if (N[i].fused || !fusable(N[i].instr, rotations, max_width)) continue;
@@ -267,12 +274,39 @@ void gate_fusion(Dag& dag, Pool& pool, int max_width, bool rotations) {
             for (int k = 0; k < node_arity(N[b].instr); ++k) q.push_back(node_qubit(N[b].instr, k));
             return q;
         };
-        Mat M = gate_matrix(N[block[0]].instr, pool);
-        std::vector<int> cq = operands(block[0]);
-        for (size_t k = 1; k < block.size(); ++k) {
-            std::vector<int> oq;
-            M = fuse(M, cq, gate_matrix(N[block[k]].instr, pool), operands(block[k]), oq);
-            cq = oq;
+        Mat M;
+        std::vector<int> cq;
+        if (dev) {
+            // the union of the members' qubits, sorted (fuse() builds the same order)
+            for (uint32_t b : block)
+                for (int q : operands(b))
+                    if (std::find(cq.begin(), cq.end(), q) == cq.end()) cq.push_back(q);
+            std::sort(cq.begin(), cq.end());
+            FusedRecipe rc;
+            rc.k = (int)cq.size();
+            for (uint32_t b : block) {
+                const Mat g = gate_matrix(N[b].instr, pool);
+                FusedMember fm{};
+                fm.mat_off = (uint32_t)(dev->member_mats.size() / 2);
+                const std::vector<int> oq = operands(b);
+                fm.a = (int)oq.size();
+                for (int j = 0; j < fm.a; ++j) fm.pos[j] = (int)(std::find(cq.begin(), cq.end(), oq[j]) - cq.begin());
+                for (const cd& z : g) {
+                    dev->member_mats.push_back(z.real());
+                    dev->member_mats.push_back(z.imag());
+                }
+                rc.members.push_back(fm);
+            }
+            dev->recipes.push_back(std::move(rc));
+            M.assign(size_t{1} << (2 * cq.size()), cd(0.0, 0.0));  // filled in after the pass
+        } else {
+            M = gate_matrix(N[block[0]].instr, pool);
+            cq = operands(block[0]);
+            for (size_t k = 1; k < block.size(); ++k) {
+                std::vector<int> oq;
+                M = fuse(M, cq, gate_matrix(N[block[k]].instr, pool), operands(block[k]), oq);
+                cq = oq;
+            }
         }
         // stitch the merged node's edges (middleware.hpp:665-703)
         const int sz = (int)cq.size();
@@ -306,6 +340,7 @@ void gate_fusion(Dag& dag, Pool& pool, int max_width, bool rotations) {
             bout[u] = last;
         }
         const uint32_t midx = pool.store(M, sz);
+        if (dev) dev->pool_offset.push_back(pool.meta[midx].offset);
         Node& A = N[block[0]];
         A.instr.type = QLX_FUSED_BLOCK;
         A.instr.flags_arity = (uint8_t)((A.instr.flags_arity & 0xF0) | (sz & 0x0F));
@@ -492,7 +527,21 @@ int qlx_transpile(uint32_t nq, const qlx_gate_instr* instrs, size_t n, const dou
         prog->pool.data.assign(reinterpret_cast<const cd*>(pool), reinterpret_cast<const cd*>(pool) + pool_elems);
         if (meta) prog->pool.meta.assign(meta, meta + n_meta);
         Dag dag = build_dag(instrs, n, nq);
-        if (o.enable_gate_fusion) gate_fusion(dag, prog->pool, width, o.fuse_rotations != 0);
+        if (o.enable_gate_fusion) {
+            DeviceFusion dev;
+            gate_fusion(dag, prog->pool, width, o.fuse_rotations != 0, o.build_on_device ? &dev : nullptr);
+            if (!dev.recipes.empty()) {
+                std::vector<double> mats;
+                build_fused_on_device(dev.recipes, dev.member_mats, mats);
+                size_t at = 0;
+                for (size_t b = 0; b < dev.recipes.size(); ++b) {
+                    const size_t ne = size_t{1} << (2 * dev.recipes[b].k);
+                    for (size_t e = 0; e < ne; ++e)
+                        prog->pool.data[dev.pool_offset[b] + e] = cd(mats[2 * (at + e)], mats[2 * (at + e) + 1]);
+                    at += ne;
+                }
+            }
+        }
         if (o.enable_windowing) {
             uint32_t ks = o.k_safe;
             if (!ks) {

@@ -25,6 +25,10 @@ QLX_EXEC_PLAIN = 0
 QLX_EXEC_WINDOWED = 1
 QLX_GSWAP_PERMUTE = 0
 QLX_GSWAP_NOOP = 1
+QLX_DENSE_AUTO = 0
+QLX_DENSE_VALU = 1
+QLX_DENSE_MFMA_BF16X3 = 2
+QLX_DENSE_MFMA_F32 = 3
 
 
 class QlxError(RuntimeError):
@@ -65,7 +69,8 @@ class ExecOpts(C.Structure):
         ("coalesce_qubits", C.c_uint32),
         ("gswap_mode", C.c_uint32),
         ("register_qubits", C.c_uint32),
-        ("reserved", C.c_uint32 * 3),
+        ("dense_kernel", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
     ]
 
 
@@ -112,7 +117,8 @@ class TranspileOpts(C.Structure):
         ("swap_penalty", C.c_uint32),
         ("restore_layout", C.c_uint32),
         ("dtype", C.c_int32),
-        ("reserved", C.c_uint32 * 7),
+        ("build_on_device", C.c_uint32),
+        ("reserved", C.c_uint32 * 6),
     ]
 
 

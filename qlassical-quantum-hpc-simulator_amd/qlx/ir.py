@@ -209,10 +209,12 @@ def layered(num_qubits: int, depth: int, seed: int) -> Program:
 
 def transpile_gpu(prog: Program, fusion: bool = True, max_width: int = 4, rotations: bool = True,
                   windowing: bool = False, k_safe: int = 0, numa_penalty: int = 10, swap_penalty: int = 1,
-                  restore: bool = True, dtype: int = 0) -> Program:
-    """Middleware::transpile_gpu through the library (qlx_transpile)."""
+                  restore: bool = True, dtype: int = 0, on_device: bool = False) -> Program:
+    """Middleware::transpile_gpu through the library (qlx_transpile). on_device: form the
+    fused-block matrices in FP64 on the current GPU (the blocks are chosen on the host)."""
     L = _abi.lib()
     o = _abi.TranspileOpts()
+    o.build_on_device = int(on_device)
     o.enable_gate_fusion, o.max_fusion_width, o.fuse_rotations = int(fusion), max_width, int(rotations)
     o.enable_windowing, o.k_safe, o.numa_penalty, o.swap_penalty = int(windowing), k_safe, numa_penalty, swap_penalty
     o.restore_layout, o.dtype = int(restore), dtype

@@ -1,0 +1,153 @@
+"""Dense 4/5-qubit gates on complex64 states through the matrix cores (k_dense_mfma),
+and FP64 fused-block construction on the device.
+
+Reference: apply_unitary (include/backend/cpu_openmp_engine.hpp:539-606) and
+pass_gate_fusion / FusionMath (include/middleware/middleware.hpp:526-753, 241-344).
+The complex64 results are compared with the complex128 vector-unit result of the same
+program on the same state (pinned to the CPU oracle by test_gpu_parity.py), and whole
+circuits with the CPU oracle of the unfused circuit.
+
+Tolerances. BF16X3 represents each operand as hi + lo (bf16 + bf16, 16 significand
+bits: relative error <= 2^-16) and drops lo*lo, so one product is off by a few 2^-16;
+one gate therefore moves the state by a relative error of order 1e-5 (stated bound:
+max |diff| <= 2e-4 * max |amp|, infidelity <= 1e-8 per gate). FP32 MFMA is plain
+FP32 arithmetic: max |diff| <= 2e-6 * max |amp|, infidelity <= 1e-11 per gate.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"bf16x3": (2e-4, 1e-8), "f32": (2e-6, 1e-11), "valu": (2e-6, 1e-11)}
+
+
+def rand_unitary(rng, k):
+    d = 1 << k
+    z = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+    q, r = np.linalg.qr(z)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def rand_state(rng, n):
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return v / np.linalg.norm(v)
+
+
+def infidelity(a, b):
+    return 1.0 - abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+
+
+def run_c64(qlx, prog, psi, dense, mode="plain"):
+    st = qlx.State(amps=psi, dtype="c64")
+    st.timing(True)
+    st.execute(prog, qlx.exec_opts(mode, dense=dense))
+    t = st.timing_read()
+    out = st.to_host()
+    st.close()
+    return out, t
+
+
+def run_c128(qlx, prog, psi):
+    st = qlx.State(amps=psi, dtype="c128")
+    st.execute(prog)
+    out = st.to_host()
+    st.close()
+    return out
+
+
+CASES = [
+    (5, (0, 1, 2, 3, 4)),        # one group: 31 of 32 lanes idle
+    (6, (5, 0, 3, 1, 4)),
+    (9, (8, 7, 6, 5, 4)),        # descending, partial tile
+    (12, (2, 9, 0, 11, 5)),
+    (14, (13, 0, 7, 1, 12)),
+    (16, (3, 4, 5, 6, 7)),
+    (4, (3, 1, 0, 2)),
+    (10, (9, 0, 5, 2)),
+    (15, (14, 13, 1, 0)),
+]
+
+
+@pytest.mark.parametrize("dense", ["bf16x3", "f32"])
+@pytest.mark.parametrize("n,qubits", CASES)
+def test_dense_gate_matches_c128(gpu, dense, n, qubits):
+    rng = np.random.default_rng(1000 * n + len(qubits))
+    k = len(qubits)
+    u = rand_unitary(rng, k)
+    prog = gpu.Circuit(n).unitary(list(qubits), u, max_arity=5).release()
+    psi = rand_state(rng, n)
+    got, t = run_c64(gpu, prog, psi, dense)
+    assert "k_dense_mfma" in t and "k_dense" not in t, t
+    want = run_c128(gpu, prog, psi)
+    # independent check of the c128 path: numpy brute force at small n
+    if n <= 10:
+        ref, _ = oracle.execute(n, prog.instrs, prog.pool, prog.meta, amps=psi)
+        assert np.abs(want - ref).max() <= 1e-12
+    tol_abs, tol_inf = TOL[dense]
+    assert np.abs(got - want).max() <= tol_abs * np.abs(want).max(), (dense, n, qubits)
+    assert infidelity(got, want) <= tol_inf
+
+
+def test_dense_mfma_repeated_gates_keep_norm(gpu):
+    """200 random 5-qubit gates: error grows like a random walk, the norm stays 1."""
+    rng = np.random.default_rng(77)
+    n = 14
+    c = gpu.Circuit(n)
+    for _ in range(200):
+        qs = [int(x) for x in rng.choice(n, 5, replace=False)]
+        c.unitary(qs, rand_unitary(rng, 5), max_arity=5)
+    prog = c.release()
+    psi = rand_state(rng, n)
+    want = run_c128(gpu, prog, psi)
+    for dense, bound in [("bf16x3", 1e-6), ("f32", 1e-9)]:
+        got, _ = run_c64(gpu, prog, psi, dense)
+        assert infidelity(got, want) <= bound, dense
+        assert abs(np.vdot(got, got).real - 1.0) <= 1e-4
+
+
+@pytest.mark.parametrize("dense", ["bf16x3", "f32", "valu"])
+@pytest.mark.parametrize("mode", ["plain", "windowed"])
+def test_fused_layered_circuit_c64_vs_oracle(gpu, dense, mode):
+    n, d = 16, 12
+    prog = gpu.layered(n, d, 1003)
+    fused = gpu.transpile_gpu(prog, max_width=5, rotations=True)
+    assert (fused.meta["num_qubits"] == 5).any()
+    rng = np.random.default_rng(5)
+    psi = rand_state(rng, n)
+    want, _ = oracle.execute(n, prog.instrs, amps=psi)
+    got, t = run_c64(gpu, fused, psi, dense, mode)
+    assert ("k_dense_mfma" in t) == (dense != "valu"), t
+    bound = 1e-6 if dense == "bf16x3" else 1e-9
+    assert infidelity(got, want) <= bound
+    assert np.abs(got - want).max() <= 50 * TOL[dense][0] * np.abs(want).max()
+
+
+def test_device_built_fusion_matches_host(gpu):
+    for n, d, seed, w in [(12, 8, 1, 5), (10, 6, 2, 4), (20, 4, 3, 5)]:
+        prog = gpu.layered(n, d, seed)
+        host = gpu.transpile_gpu(prog, max_width=w, rotations=True)
+        dev = gpu.transpile_gpu(prog, max_width=w, rotations=True, on_device=True)
+        assert host.instrs.tobytes() == dev.instrs.tobytes()
+        assert host.meta.tobytes() == dev.meta.tobytes()
+        assert np.abs(host.pool - dev.pool).max() <= 1e-13
+    # dense members and the reference's own gate set (no rotations)
+    rng = np.random.default_rng(9)
+    c = gpu.Circuit(6).h(0).cx(0, 1).unitary([2, 4], rand_unitary(rng, 2)).cx(1, 2).s(4).swap(3, 5).z(2)
+    prog = c.release()
+    for rot in (False, True):
+        host = gpu.transpile_gpu(prog, max_width=5, rotations=rot)
+        dev = gpu.transpile_gpu(prog, max_width=5, rotations=rot, on_device=True)
+        assert host.instrs.tobytes() == dev.instrs.tobytes()
+        assert np.abs(host.pool - dev.pool).max() <= 1e-13
+
+
+def test_dense_kernel_choice_rejected_when_unknown(gpu):
+    st = gpu.State(6, "c64")
+    o = gpu.exec_opts("plain")
+    o.dense_kernel = 9
+    prog = gpu.Circuit(6).unitary([0, 1, 2, 3], np.eye(16)).release()
+    with pytest.raises(gpu.QlxError):
+        st.execute(prog, o)
+    st.close()
