@@ -32,6 +32,7 @@ def _load(path: str) -> C.CDLL:
     if path == REF:
         L.ref_gen_layered.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.c_void_p, C.c_size_t,
                                       C.POINTER(C.c_size_t)]
+        L.ref_gen_bench.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
         L.ref_build.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                 C.POINTER(C.c_size_t)]
         L.ref_insert_zero_bit.argtypes = [C.c_uint64, C.c_uint8]
@@ -162,6 +163,17 @@ def ref_gen_layered(n: int, depth: int, seed: int) -> np.ndarray:
     L.ref_gen_layered(n, depth, seed, None, 0, C.byref(cnt))
     out = np.zeros(cnt.value, GATE_DTYPE)
     L.ref_gen_layered(n, depth, seed, out.ctypes.data, cnt.value, C.byref(cnt))
+    return out
+
+
+def ref_gen_bench(kind: str, n: int, depth: int = 0) -> np.ndarray:
+    """The reference's make_hea_circuit(n, depth) / make_qft_circuit(n) instruction bytes."""
+    L = ref()
+    k = {"hea": 0, "qft": 1}[kind]
+    cnt = C.c_size_t(0)
+    L.ref_gen_bench(k, n, depth, None, 0, C.byref(cnt))
+    out = np.zeros(cnt.value, GATE_DTYPE)
+    L.ref_gen_bench(k, n, depth, out.ctypes.data, cnt.value, C.byref(cnt))
     return out
 
 

@@ -8,10 +8,12 @@
 //   UnitaryPool::store      include/frontend/frontend.hpp:132-150
 //   insert_zero_bit         include/backend/state_vector_utils.hpp:23-28
 //   CPUStateVector          include/backend/cpu_state_vector.hpp:54-115
+//   make_hea/qft_circuit    include/frontend/circuit_generators.hpp:22-68
 // The engine (cpu_openmp_engine.hpp) and middleware need Eigen 3.4 and are not built.
 
 #include <backend/cpu_state_vector.hpp>
 #include <backend/state_vector_utils.hpp>
+#include <frontend/circuit_generators.hpp>
 #include <frontend/frontend.hpp>
 
 #include <cmath>
@@ -32,6 +34,19 @@ int ref_gen_layered(uint32_t n, uint32_t depth, uint64_t seed, void* out, size_t
         }
         for (uint32_t q = l % 2; q + 1 < n; q += 2) qc.cx((int16_t)q, (int16_t)(q + 1));
     }
+    qlassical::IRModule m = std::move(qc).release();
+    *n_out = m.gate_stream.size();
+    if (out) {
+        if (cap < m.gate_stream.size()) return -1;
+        std::memcpy(out, m.gate_stream.data(), m.gate_stream.size() * sizeof(qlassical::GateInstr));
+    }
+    return 0;
+}
+
+// The reference's own benchmark generators (circuit_generators.hpp): kind 0 = HEA, 1 = QFT.
+int ref_gen_bench(int kind, uint32_t n, uint32_t depth, void* out, size_t cap, size_t* n_out) {
+    qlassical::QuantumCircuit qc = kind == 0 ? qlassical::frontend::make_hea_circuit(n, depth)
+                                             : qlassical::frontend::make_qft_circuit(n);
     qlassical::IRModule m = std::move(qc).release();
     *n_out = m.gate_stream.size();
     if (out) {

@@ -207,6 +207,35 @@ def layered(num_qubits: int, depth: int, seed: int) -> Program:
     return Program(num_qubits, arr)
 
 
+def hea(num_qubits: int, depth: int) -> Program:
+    """make_hea_circuit (reference include/frontend/circuit_generators.hpp:22-48): per layer
+    RY(q, theta) and RZ(q, 1.5 theta) on every qubit, then CX(q, q+1) along the chain;
+    theta = 0.1234 + 0.05 * layer, accumulated in double as the reference does."""
+    c = Circuit(num_qubits)
+    theta = 0.1234
+    for _ in range(depth):
+        for q in range(num_qubits):
+            c.ry(q, theta)
+            c.rz(q, theta * 1.5)
+        for q in range(num_qubits - 1):
+            c.cx(q, q + 1)
+        theta += 0.05
+    return c.release()
+
+
+def qft(num_qubits: int) -> Program:
+    """make_qft_circuit (reference circuit_generators.hpp:50-68): H(i) then CX(j, i) for
+    every j > i, for i = 0..n-1; then SWAP(i, n-1-i)."""
+    c = Circuit(num_qubits)
+    for i in range(num_qubits):
+        c.h(i)
+        for j in range(i + 1, num_qubits):
+            c.cx(j, i)
+    for i in range(num_qubits // 2):
+        c.swap(i, num_qubits - 1 - i)
+    return c.release()
+
+
 def transpile_gpu(prog: Program, fusion: bool = True, max_width: int = 4, rotations: bool = True,
                   windowing: bool = False, k_safe: int = 0, numa_penalty: int = 10, swap_penalty: int = 1,
                   restore: bool = True, dtype: int = 0, on_device: bool = False) -> Program:

@@ -115,3 +115,12 @@ def test_batch_workload_shapes():
     # the draw is the mt19937_64 stream of the stated seed
     g = qlx.MT19937_64(4004)
     assert w[0][0] == 10 + g() % 11 and w[0][1] == 5 + g() % 36
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference tree not built here")
+@pytest.mark.parametrize("kind,n,d", [("hea", 2, 1), ("hea", 5, 3), ("hea", 12, 4), ("hea", 20, 7),
+                                      ("qft", 1, 0), ("qft", 2, 0), ("qft", 5, 0), ("qft", 16, 0)])
+def test_benchmark_generators_match_reference(kind, n, d):
+    """make_hea_circuit / make_qft_circuit (circuit_generators.hpp:22-68): identical bytes."""
+    mine = qlx.hea(n, d) if kind == "hea" else qlx.qft(n)
+    assert mine.instrs.tobytes() == oracle.ref_gen_bench(kind, n, d).tobytes()

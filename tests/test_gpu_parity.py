@@ -109,7 +109,7 @@ def test_dense_unitary_vs_oracle(gpu, k):
     for n in (k, k + 3, 14):
         for _ in range(3):
             qs = [int(x) for x in rng.choice(n, k, replace=False)]
-            c = gpu.Circuit(n).unitary(qs, bf.random_unitary(k, rng))
+            c = gpu.Circuit(n).unitary(qs, bf.random_unitary(k, rng), max_arity=5)
             p = c.release()
             init = bf.random_state(n, rng)
             got, _ = gpu_run(gpu, p, init)
