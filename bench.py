@@ -257,7 +257,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", choices=["circuit30", "precision", "batch", "serve"], default="circuit30")
-    ap.add_argument("--dense-variants", default="bf16x3,f32,valu",
+    ap.add_argument("--dense-variants", default="bf16,f32,valu",
                     help="precision config: dense-gate arithmetic forms to time (first one is reported)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)

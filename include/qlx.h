@@ -98,9 +98,10 @@ typedef struct qlx_exec_opts {
  * 5-qubit gates on complex64 states; every other dense gate runs on the vector units
  * (FP64 for complex128, FP32 for complex64). The matrices stay FP64 in the pool and are
  * converted on the device. */
-#define QLX_DENSE_AUTO        0  /* complex64 4/5-qubit gates: MFMA BF16X3 */
+#define QLX_DENSE_AUTO        0  /* complex64 4/5-qubit gates: MFMA BF16 */
 #define QLX_DENSE_VALU        1  /* vector units only */
-#define QLX_DENSE_MFMA_BF16X3 2  /* BF16 hi/lo split, three products accumulated in FP32 */
+#define QLX_DENSE_MFMA_BF16   2  /* BF16 split operands (matrix hi+mid+lo, state hi+lo), four
+                                    products accumulated in FP32 */
 #define QLX_DENSE_MFMA_F32    3  /* FP32 MFMA (v_mfma_f32_32x32x2_f32) */
 
 /* Statistics of the last qlx_execute on a state (sweep accounting). */
@@ -280,8 +281,9 @@ int qlx_batch_run(qlx_batch* b, int mode);
 /* Waits for the last run; *elapsed_ms = its GPU time (HIP events around the run). */
 int qlx_batch_sync(qlx_batch* b, double* elapsed_ms);
 int qlx_batch_state_to_host(qlx_batch* b, uint32_t circuit_id, double* out_c128, size_t dim);
-/* FP64 comparison of one batch state with a reference state of the same size:
- * 1 - |<ref|psi>|^2 and max |ref_i - psi_i|. */
+/* FP64 comparison (on the GPU) of one batch state with a reference state of the same
+ * size: the state infidelity 1 - |<ref|psi>|^2 / (<ref|ref> <psi|psi>) and
+ * max |ref_i - psi_i|. */
 int qlx_batch_compare(qlx_batch* b, uint32_t circuit_id, qlx_state* ref, double* infidelity, double* max_abs);
 int qlx_batch_destroy(qlx_batch* b);
 
@@ -300,6 +302,7 @@ int qlx_server_submit(qlx_server* s, uint32_t num_qubits, const qlx_gate_instr* 
 int qlx_server_pump(qlx_server* s, uint32_t* queued, uint32_t* active, uint32_t* inflight);
 /* Requests completed since the last call, with submit-to-completion latency in seconds. */
 int qlx_server_take_completed(qlx_server* s, uint64_t* ids, double* latency_s, size_t cap, size_t* n_out);
+/* As qlx_batch_compare, for a completed request. */
 int qlx_server_compare(qlx_server* s, uint64_t id, qlx_state* ref, double* infidelity, double* max_abs);
 int qlx_server_state_to_host(qlx_server* s, uint64_t id, double* out_c128, size_t dim);
 int qlx_server_release(qlx_server* s, uint64_t id);

@@ -480,9 +480,10 @@ int qlx_batch_compare(qlx_batch* b, uint32_t id, qlx_state* ref, double* infidel
         if (ref->nq != v.nq) fail(QLX_EINVAL, "reference state has a different qubit count");
         QLX_HIP_CHECK(hipStreamSynchronize(b->stream));
         QLX_HIP_CHECK(hipStreamSynchronize(ref->stream));
-        double re = 0, im = 0, m = 0;
-        compare_amplitudes(ref->amp, ref->dtype, v.amp, v.dtype, 1ull << v.nq, b->stream, b->scratch, &re, &im, &m);
-        if (infidelity) *infidelity = 1.0 - (re * re + im * im);
+        double re = 0, im = 0, m = 0, nb = 0;
+        compare_amplitudes(ref->amp, ref->dtype, v.amp, v.dtype, 1ull << v.nq, b->stream, b->scratch, &re, &im, &m,
+                           &nb);
+        if (infidelity) *infidelity = infidelity_from(re, im, reduce_norm_sq(ref), nb);
         if (max_abs) *max_abs = m;
     });
 }

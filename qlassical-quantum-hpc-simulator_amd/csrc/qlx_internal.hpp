@@ -124,9 +124,13 @@ inline double pass_bytes(const qlx_state* st) {
 // Kernel launch helpers (defined in qlx_state.hip).
 void launch_fill_zero_state(qlx_state* st);
 double reduce_norm_sq(qlx_state* st);
-// FP64 <a|b> and max |a-b| over n amplitudes of (possibly different) precision, on `stream`.
+// FP64 <a|b>, max |a-b| and <b|b> over n amplitudes of (possibly different) precision, on `stream`.
 void compare_amplitudes(const void* a, int dta, const void* b, int dtb, uint64_t n, hipStream_t stream,
-                        DevBuf& scratch, double* re, double* im, double* maxd);
+                        DevBuf& scratch, double* re, double* im, double* maxd, double* norm_b = nullptr);
+// State infidelity 1 - |<ref|psi>|^2 / (<ref|ref> <psi|psi>) from the FP64 sums.
+inline double infidelity_from(double re, double im, double norm_ref, double norm_psi) {
+    return 1.0 - (re * re + im * im) / (norm_ref * norm_psi);
+}
 
 // Plain (one sweep per instruction) executor, qlx_gates.hip.
 void execute_plain(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
@@ -155,7 +159,7 @@ struct MfmaSplitJob {
     uint64_t pool_off;  // complex elements into the device pool
     uint64_t frag_off;  // bytes into the fragment buffer
     int k;              // 4 or 5
-    int variant;        // QLX_DENSE_MFMA_BF16X3 | QLX_DENSE_MFMA_F32
+    int variant;        // QLX_DENSE_MFMA_BF16 | QLX_DENSE_MFMA_F32
 };
 size_t mfma_fragment_bytes(int k, int variant);
 // Builds the A-operand fragments of each job from the FP64 pool (synchronous).

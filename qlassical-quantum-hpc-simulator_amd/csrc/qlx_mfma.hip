@@ -11,9 +11,14 @@
 // with the real embedding A[2o+0][2l+0] = Re U[o][l], A[2o+0][2l+1] = -Im U[o][l],
 // A[2o+1][2l+0] = Im U[o][l], A[2o+1][2l+1] = Re U[o][l], and X[2l+c][j] the real (c=0)
 // and imaginary (c=1) part of amplitude l of group j. Two arithmetic forms:
-//   * BF16X3: v_mfma_f32_32x32x16_bf16 on a hi/lo split of both operands
-//     (a = a_hi + a_lo, each bf16), three products lo*hi + hi*lo + hi*hi accumulated
-//     in FP32 (the lo*lo term, <= 2^-18 relative, is dropped);
+//   * BF16: v_mfma_f32_32x32x16_bf16 on bf16 splits of both operands. The matrix,
+//     split once per gate, keeps three terms (a = a_hi + a_mid + a_lo: 24 significand
+//     bits, as FP32): its error is the same for every group, so a coarser matrix
+//     would add up coherently over the gates and drift the norm (two terms: ~4e-7
+//     per gate). The state keeps two (x = x_hi + x_lo: 16 bits), an error that differs
+//     from amplitude to amplitude. Four products, small terms first, accumulated in
+//     FP32: a_lo x_hi + a_mid x_hi + a_hi x_lo + a_hi x_hi (dropped: a_mid x_lo and
+//     below, <= 2^-24 relative);
 //   * F32:    v_mfma_f32_32x32x2_f32 on the FP32 operands directly.
 // A is built once per gate from the FP64 matrix on the device (k_mfma_split) and held
 // in VGPRs for the whole launch; X is read from HBM straight into MFMA operand
@@ -53,7 +58,7 @@ __device__ inline double embed(const double* __restrict__ U, int D, int row, int
     return (col & 1) ? ur : ui;
 }
 
-// BF16X3: the column of X (and of A) that slot s of K-step t of lane half h carries.
+// BF16: the column of X (and of A) that slot s of K-step t of lane half h carries.
 __host__ __device__ constexpr int kcol_bf16(int K, int t, int h, int s) {
     return 2 * ((h << (K - 1)) + 4 * t + (s >> 1)) + (s & 1);
 }
@@ -66,7 +71,7 @@ struct SplitDesc {
     uint64_t pool_off;   // complex elements into the device pool
     uint64_t frag_off;   // bytes into the fragment buffer
     uint32_t k;          // 4 or 5
-    uint32_t f32;        // 1: FP32 fragments, 0: BF16 hi/lo fragments
+    uint32_t f32;        // 1: FP32 fragments, 0: BF16 hi/mid/lo fragments
 };
 
 // One work-group per matrix: writes the A-operand fragments of every (M-tile, K-step, lane).
@@ -87,18 +92,24 @@ __global__ void __launch_bounds__(kBlock) k_mfma_split(const double* __restrict_
     } else {
         const int T = (2 * D) / 16;
         bf16x8* hi = (bf16x8*)(frags + d.frag_off);
-        bf16x8* lo = hi + MT * T * 64;
+        bf16x8* mid = hi + MT * T * 64;
+        bf16x8* lo = mid + MT * T * 64;
         for (int e = threadIdx.x; e < MT * T * 64; e += kBlock) {
             const int lane = e & 63, t = (e >> 6) % T, m = (e >> 6) / T;
-            bf16x8 vh, vl;
+            bf16x8 vh, vm, vl;
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
                 const double v = embed(U, D, 32 * m + (lane & 31), kcol_bf16(K, t, lane >> 5, s));
                 const __bf16 h = (__bf16)(float)v;
+                const double r1 = v - (double)(float)h;       // exact
+                const __bf16 md = (__bf16)(float)r1;
+                const double r2 = r1 - (double)(float)md;     // exact
                 vh[s] = h;
-                vl[s] = (__bf16)(float)(v - (double)(float)h);
+                vm[s] = md;
+                vl[s] = (__bf16)(float)r2;
             }
             hi[e] = vh;
+            mid[e] = vm;
             lo[e] = vl;
         }
     }
@@ -122,9 +133,9 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
     const int h = lane >> 5, j = lane & 31;
     const uint64_t hin = h ? ma.hbit_in : 0u, hout = h ? ma.hbit_out : 0u;
 
-    constexpr int TB = (2 * D) / 16;   // BF16X3 K-steps
+    constexpr int TB = (2 * D) / 16;   // BF16 K-steps
     constexpr int TF = D;              // F32 K-steps
-    bf16x8 ah[F32 ? 1 : MT][F32 ? 1 : TB], al[F32 ? 1 : MT][F32 ? 1 : TB];
+    bf16x8 ah[F32 ? 1 : MT][F32 ? 1 : TB], am[F32 ? 1 : MT][F32 ? 1 : TB], al[F32 ? 1 : MT][F32 ? 1 : TB];
     float af[F32 ? MT : 1][F32 ? TF : 1];
     if constexpr (F32) {
         const float* f = (const float*)frags;
@@ -134,12 +145,14 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
             for (int t = 0; t < TF; ++t) af[m][t] = f[(m * TF + t) * 64 + lane];
     } else {
         const bf16x8* fh = (const bf16x8*)frags;
-        const bf16x8* fl = fh + MT * TB * 64;
+        const bf16x8* fm = fh + MT * TB * 64;
+        const bf16x8* fl = fm + MT * TB * 64;
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
             for (int t = 0; t < TB; ++t) {
                 ah[m][t] = fh[(m * TB + t) * 64 + lane];
+                am[m][t] = fm[(m * TB + t) * 64 + lane];
                 al[m][t] = fl[(m * TB + t) * 64 + lane];
             }
     }
@@ -190,6 +203,9 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
                     acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[m][t], xh[t], acc[m], 0, 0, 0);
 #pragma unroll
                 for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am[m][t], xh[t], acc[m], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
                     acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m][t], xl[t], acc[m], 0, 0, 0);
 #pragma unroll
                 for (int t = 0; t < TB; ++t)
@@ -211,7 +227,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
 
 size_t frag_bytes(int k, bool f32) {
     const size_t D = size_t{1} << k, MT = 2 * D / 32;
-    return f32 ? MT * D * 64 * sizeof(float) : 2 * MT * (2 * D / 16) * 64 * 16;
+    return f32 ? MT * D * 64 * sizeof(float) : 3 * MT * (2 * D / 16) * 64 * 16;
 }
 
 }  // namespace
@@ -287,7 +303,7 @@ void prepare_dense(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const 
     const uint32_t dk = opts.dense_kernel;
     if (dk > QLX_DENSE_MFMA_F32) fail(QLX_EINVAL, "unknown dense_kernel");
     if (st->dtype != QLX_C64 || dk == QLX_DENSE_VALU || !dpool || !meta) return;
-    const int variant = dk == QLX_DENSE_MFMA_F32 ? QLX_DENSE_MFMA_F32 : QLX_DENSE_MFMA_BF16X3;
+    const int variant = dk == QLX_DENSE_MFMA_F32 ? QLX_DENSE_MFMA_F32 : QLX_DENSE_MFMA_BF16;
     std::vector<MfmaSplitJob> jobs;
     size_t total = 0;
     for (size_t i = 0; i < n; ++i) {

@@ -425,10 +425,10 @@ int qlx_server_compare(qlx_server* s, uint64_t id, qlx_state* ref, double* infid
         if (r.status != qlx_server::DONE) fail(QLX_EINVAL, "request not completed");
         if (!ref || !ref->amp || ref->nq != r.nq) fail(QLX_EINVAL, "reference state does not match");
         QLX_HIP_CHECK(hipStreamSynchronize(ref->stream));
-        double re = 0, im = 0, m = 0;
+        double re = 0, im = 0, m = 0, nb = 0;
         compare_amplitudes(ref->amp, ref->dtype, (char*)s->arena.ptr + r.amp_off * s->amp, s->dtype, 1ull << r.nq,
-                           s->aux, s->scratch, &re, &im, &m);
-        if (infidelity) *infidelity = 1.0 - (re * re + im * im);
+                           s->aux, s->scratch, &re, &im, &m, &nb);
+        if (infidelity) *infidelity = infidelity_from(re, im, reduce_norm_sq(ref), nb);
         if (max_abs) *max_abs = m;
     });
 }

@@ -342,31 +342,34 @@ static void upload_host_state(qlx_state* st, const double* amps, size_t dim) {
 }
 
 void compare_amplitudes(const void* a, int dta, const void* b, int dtb, uint64_t n, hipStream_t stream,
-                        DevBuf& scratch, double* re, double* im, double* maxd) {
+                        DevBuf& scratch, double* re, double* im, double* maxd, double* norm_b) {
     const unsigned nb = red_blocks(n);
-    scratch.reserve((size_t)nb * 3 * sizeof(double), stream);
+    scratch.reserve((size_t)nb * 4 * sizeof(double), stream);
     double* part = (double*)scratch.ptr;
     auto go = [&](auto pa, auto pb) {
         k_inner_partial<<<nb, kRedThreads, 0, stream>>>(pa, pb, n, part);
         k_maxdiff_partial<<<nb, kRedThreads, 0, stream>>>(pa, pb, n, part + 2 * nb);
+        k_norm_partial<<<nb, kRedThreads, 0, stream>>>(pb, n, part + 3 * nb);
     };
     if (dta == QLX_C128 && dtb == QLX_C128) go((const double2*)a, (const double2*)b);
     else if (dta == QLX_C128) go((const double2*)a, (const float2*)b);
     else if (dtb == QLX_C128) go((const float2*)a, (const double2*)b);
     else go((const float2*)a, (const float2*)b);
     QLX_HIP_CHECK(hipGetLastError());
-    std::vector<double> h(3 * nb);
-    QLX_HIP_CHECK(hipMemcpyAsync(h.data(), part, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, stream));
+    std::vector<double> h(4 * nb);
+    QLX_HIP_CHECK(hipMemcpyAsync(h.data(), part, 4 * nb * sizeof(double), hipMemcpyDeviceToHost, stream));
     QLX_HIP_CHECK(hipStreamSynchronize(stream));
-    double r = 0, i = 0, m = 0;
+    double r = 0, i = 0, m = 0, nrm = 0;
     for (unsigned k = 0; k < nb; ++k) {
         r += h[2 * k];
         i += h[2 * k + 1];
         m = std::fmax(m, h[2 * nb + k]);
+        nrm += h[3 * nb + k];
     }
     *re = r;
     *im = i;
     *maxd = m;
+    if (norm_b) *norm_b = nrm;
 }
 
 template <class F>

@@ -29,12 +29,46 @@
 
 namespace qlx {
 
+// Persistent sweep: work-group b runs chunks b, b + G, b + 2G, ... (G = grid size, the
+// resident work-groups of the device). With up to 16 amplitude registers per work-item,
+// the next chunk's HBM loads are issued into a second register set before the current
+// chunk's phases run, so they overlap its compute.
 template <class A, int LOGT, int RB>
 __global__ void __launch_bounds__(1 << LOGT) k_sweep(A* __restrict__ amp, const DevSweep* __restrict__ sw,
                                                     const DevPhase* __restrict__ ph, const DevOp* __restrict__ ops,
-                                                    const double* __restrict__ coef) {
+                                                    const double* __restrict__ coef, uint64_t nchunks) {
     extern __shared__ __align__(16) unsigned char smem[];
-    run_chunk<A, LOGT, RB>(amp, sw, blockIdx.x, ph, ops, coef, reinterpret_cast<A*>(smem), false);
+    A* lds = reinterpret_cast<A*>(smem);
+    uint64_t c = blockIdx.x;
+    if (c >= nchunks) return;
+    // 32 registers of amplitudes per work-item leave no room for a second set
+    constexpr bool kPrefetch = RB <= 4;
+    if constexpr (!kPrefetch) {
+        // one chunk per work-group (grid = chunks): a loop here would keep every register's
+        // offsets live across iterations and push the kernel past the register file
+        A v[1 << RB];
+        const uint64_t base = chunk_base<LOGT, RB>(sw, c);
+        load_chunk<A, LOGT, RB>(v, amp, sw, base);
+        compute_chunk<A, LOGT, RB>(v, sw, ph, ops, coef, lds);
+        store_chunk<A, LOGT, RB>(v, amp, sw, base);
+        return;
+    }
+    A v[1 << RB], w[1 << RB];
+    uint64_t base = chunk_base<LOGT, RB>(sw, c);
+    load_chunk<A, LOGT, RB>(v, amp, sw, base);
+    for (;;) {
+        const uint64_t nc = c + gridDim.x;
+        const bool more = nc < nchunks;
+        const uint64_t nbase = more ? chunk_base<LOGT, RB>(sw, nc) : 0;
+        if (more) load_chunk<A, LOGT, RB>(w, amp, sw, nbase);
+        compute_chunk<A, LOGT, RB>(v, sw, ph, ops, coef, lds);
+        store_chunk<A, LOGT, RB>(v, amp, sw, base);
+        if (!more) break;
+#pragma unroll
+        for (int i = 0; i < (1 << RB); ++i) v[i] = w[i];
+        c = nc;
+        base = nbase;
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -421,10 +455,10 @@ std::unique_ptr<SweepPlan> build_plan_geo(uint32_t nq, const Geometry& geo, cons
         DevSweep d{};
         d.n_phases = (uint32_t)sw.phases.size();
         d.phase_begin = (uint32_t)plan->dph.size();
-        for (int j = 0; j < W; ++j) d.S[j] = (uint8_t)sw.S[j];
+        for (int j = 0; j < W; ++j) d.S[j] = (uint32_t)sw.S[j];
         auto glob_of_local = [&](int l) { return sw.S[l]; };
-        auto fill_ends = [&](const PlanPhase& ph, uint8_t* tglob, uint64_t* roff) {
-            for (int b = 0; b < geo.logt; ++b) tglob[b] = (uint8_t)glob_of_local(ph.st2loc[b]);
+        auto fill_ends = [&](const PlanPhase& ph, uint32_t* tglob, uint64_t* roff) {
+            for (int b = 0; b < geo.logt; ++b) tglob[b] = (uint32_t)glob_of_local(ph.st2loc[b]);
             for (int i = 0; i < (1 << geo.rb); ++i) {
                 uint64_t o = 0;
                 for (int b = 0; b < geo.rb; ++b)
@@ -441,7 +475,7 @@ std::unique_ptr<SweepPlan> build_plan_geo(uint32_t nq, const Geometry& geo, cons
             dp.remap = p > 0 ? 1u : 0u;
             dp.op_begin = (uint32_t)plan->dop.size();
             dp.n_ops = (uint32_t)ph.ops.size();
-            for (int b = 0; b < geo.logt; ++b) dp.tlocal[b] = (uint8_t)ph.st2loc[b];
+            for (int b = 0; b < geo.logt; ++b) dp.tlocal[b] = (uint32_t)ph.st2loc[b];
             for (int i = 0; i < (1 << geo.rb); ++i) {
                 uint32_t o = 0;
                 for (int b = 0; b < geo.rb; ++b)
@@ -458,7 +492,7 @@ std::unique_ptr<SweepPlan> build_plan_geo(uint32_t nq, const Geometry& geo, cons
                 if (two && (op.l1 < 0 || op.l0 == op.l1)) fail(QLX_EHIP, "sweep planner: bad two-qubit operands");
                 if (op.l0 < 0 || op.l0 >= geo.rb || (two && op.l1 >= geo.rb))
                     fail(QLX_EHIP, "sweep planner: operand is not a register bit");
-                o.key = (uint16_t)((kind << 8) | (op.l0 << 4) | (two ? op.l1 : 0));
+                o.key = (uint32_t)((kind << 8) | (op.l0 << 4) | (two ? op.l1 : 0));
                 if (kind == OP_RX || kind == OP_H) {
                     o.c0 = plan->coef[o.coef];
                     o.c1 = kind == OP_RX ? plan->coef[o.coef + 1] : 0.0;
@@ -493,9 +527,19 @@ static void launch_sweep(qlx_state* st, const char* dev, size_t off_sw, size_t o
                          int si) {
     const uint64_t chunks = 1ull << (st->nq - (LOGT + RB));
     const size_t lds = (size_t(1) << (LOGT + RB)) * sizeof(A);
-    k_sweep<A, LOGT, RB><<<(unsigned)chunks, 1 << LOGT, lds, st->stream>>>(
+    static thread_local int resident[2][8][8] = {};  // work-groups per CU, by (dtype, LOGT-5, RB-1)
+    int& per_cu = resident[sizeof(A) == 16][LOGT - 5][RB - 1];
+    if (!per_cu) {
+        QLX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep<A, LOGT, RB>, 1 << LOGT, lds));
+        per_cu = std::max(per_cu, 1);
+    }
+    int dev_id = 0, cus = 0;
+    QLX_HIP_CHECK(hipGetDevice(&dev_id));
+    QLX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id));
+    const uint64_t grid = RB <= 4 ? std::min<uint64_t>(chunks, (uint64_t)per_cu * (uint64_t)cus) : chunks;
+    k_sweep<A, LOGT, RB><<<(unsigned)grid, 1 << LOGT, lds, st->stream>>>(
         (A*)st->amp, (const DevSweep*)(dev + off_sw) + si, (const DevPhase*)(dev + off_ph),
-        (const DevOp*)(dev + off_op), (const double*)(dev + off_cf));
+        (const DevOp*)(dev + off_op), (const double*)(dev + off_cf), chunks);
     QLX_HIP_CHECK(hipGetLastError());
 }
 

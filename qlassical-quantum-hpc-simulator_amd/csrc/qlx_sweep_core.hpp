@@ -16,28 +16,33 @@ constexpr int kMaxR = 32;
 
 enum OpKind : uint8_t { OP_U2 = 1, OP_RX = 2, OP_H = 3, OP_CX = 4, OP_SWAP = 5, OP_U4 = 6 };
 
+// 32-bit fields only: the descriptor is read with scalar loads (a 16-bit field would be
+// fetched with a vector load, and the wait for it would also wait for every HBM load
+// in flight).
 struct DevOp {
-    uint16_t key;   // (kind << 8) | (register bit 0 << 4) | register bit 1
-    uint16_t pad;
+    uint32_t key;   // (kind << 8) | (register bit 0 << 4) | register bit 1
     uint32_t coef;  // index into the coefficient array (doubles): U2/U4 matrices
     double c0, c1;  // inline coefficients: RX (cos, sin), H (1/sqrt 2)
 };
+static_assert(sizeof(DevOp) == 24, "DevOp layout");
 
+// Descriptors use 32/64-bit fields throughout so that kernels read them with scalar loads.
 struct DevPhase {
     uint32_t n_ops, op_begin, remap, pad;
     uint32_t roff[kMaxR];        // local-index contribution of register index i
-    uint8_t tlocal[kMaxLogT];    // local bit held by work-item bit b
-    uint8_t pad2[6];
+    uint32_t tlocal[kMaxLogT];   // local bit held by work-item bit b
+    uint32_t pad2[2];
 };
 
 struct DevSweep {
     uint32_t n_phases, phase_begin;
-    uint8_t S[kMaxW];            // global qubit of local bit j (ascending)
-    uint8_t ld_tglob[kMaxLogT];  // global qubit at work-item bit b in the first layout
-    uint8_t st_tglob[kMaxLogT];  // ... in the last layout
+    uint32_t S[kMaxW];           // global qubit of local bit j (ascending)
+    uint32_t ld_tglob[kMaxLogT]; // global qubit at work-item bit b in the first layout
+    uint32_t st_tglob[kMaxLogT]; // ... in the last layout
     uint64_t ld_roff[kMaxR];     // global offset of register i in the first layout
     uint64_t st_roff[kMaxR];     // ... in the last layout
 };
+static_assert(sizeof(DevPhase) % 16 == 0 && sizeof(DevSweep) % 16 == 0, "descriptor alignment");
 
 // ---------------------------------------------------------------------------------
 // device side
@@ -194,36 +199,58 @@ __device__ __forceinline__ void apply_op(A (&v)[R], const DevOp op, const double
 #undef F_U4
 }
 
-// Executes one sweep on one 2^W-amplitude chunk: load (or synthesise |0...0>), the
-// register phases with their LDS re-layouts, store. Used by k_sweep (one chunk per
-// work-group) and by the batched persistent kernel (a queue of chunks).
-template <class A, int LOGT, int RB>
-__device__ __forceinline__ void run_chunk(A* __restrict__ amp, const DevSweep* __restrict__ sw, uint64_t chunk,
-                                          const DevPhase* __restrict__ ph, const DevOp* __restrict__ ops,
-                                          const double* __restrict__ coef, A* __restrict__ lds, bool init_zero) {
-    constexpr int R = 1 << RB;
-    constexpr int W = LOGT + RB;
-    constexpr int ELEM = (int)sizeof(A);
-    using T = typename Real<A>::T;
-    const uint32_t t = threadIdx.x;
-
+// One sweep on one 2^W-amplitude chunk, in three steps: load the chunk into registers
+// (or synthesise |0...0>), run the register phases with their LDS re-layouts, store.
+template <int LOGT, int RB>
+__device__ __forceinline__ uint64_t chunk_base(const DevSweep* __restrict__ sw, uint64_t chunk) {
     uint64_t base = chunk;
 #pragma unroll
-    for (int j = 0; j < W; ++j) base = dins0(base, sw->S[j]);
+    for (int j = 0; j < LOGT + RB; ++j) base = dins0(base, sw->S[j]);
+    return base;
+}
 
+template <class A, int LOGT, int RB>
+__device__ __forceinline__ void load_chunk(A (&v)[1 << RB], const A* __restrict__ amp,
+                                           const DevSweep* __restrict__ sw, uint64_t base) {
+    const uint32_t t = threadIdx.x;
     uint64_t toff = 0;
 #pragma unroll
     for (int b = 0; b < LOGT; ++b) toff |= (uint64_t)((t >> b) & 1u) << sw->ld_tglob[b];
-
-    A v[R];
-    if (init_zero) {
 #pragma unroll
-        for (int i = 0; i < R; ++i) v[i] = A{(base | toff | sw->ld_roff[i]) == 0 ? T(1) : T(0), T(0)};
-    } else {
-#pragma unroll
-        for (int i = 0; i < R; ++i) v[i] = amp[base | toff | sw->ld_roff[i]];
-    }
+    for (int i = 0; i < (1 << RB); ++i) v[i] = amp[base | toff | sw->ld_roff[i]];
+}
 
+template <class A, int LOGT, int RB>
+__device__ __forceinline__ void zero_chunk(A (&v)[1 << RB], const DevSweep* __restrict__ sw, uint64_t base) {
+    using T = typename Real<A>::T;
+    const uint32_t t = threadIdx.x;
+    uint64_t toff = 0;
+#pragma unroll
+    for (int b = 0; b < LOGT; ++b) toff |= (uint64_t)((t >> b) & 1u) << sw->ld_tglob[b];
+#pragma unroll
+    for (int i = 0; i < (1 << RB); ++i) v[i] = A{(base | toff | sw->ld_roff[i]) == 0 ? T(1) : T(0), T(0)};
+}
+
+template <class A, int LOGT, int RB>
+__device__ __forceinline__ void store_chunk(const A (&v)[1 << RB], A* __restrict__ amp,
+                                            const DevSweep* __restrict__ sw, uint64_t base) {
+    const uint32_t t = threadIdx.x;
+    uint64_t toff = 0;
+#pragma unroll
+    for (int b = 0; b < LOGT; ++b) toff |= (uint64_t)((t >> b) & 1u) << sw->st_tglob[b];
+#pragma unroll
+    for (int i = 0; i < (1 << RB); ++i) amp[base | toff | sw->st_roff[i]] = v[i];
+}
+
+// The register phases of one sweep. No global-memory access: HBM loads issued before
+// the call (the next chunk's) stay in flight across it.
+template <class A, int LOGT, int RB>
+__device__ __forceinline__ void compute_chunk(A (&v)[1 << RB], const DevSweep* __restrict__ sw,
+                                              const DevPhase* __restrict__ ph, const DevOp* __restrict__ ops,
+                                              const double* __restrict__ coef, A* __restrict__ lds) {
+    constexpr int R = 1 << RB;
+    constexpr int ELEM = (int)sizeof(A);
+    const uint32_t t = threadIdx.x;
     const DevPhase* cur = ph + sw->phase_begin;
     uint32_t lt = 0;
 #pragma unroll
@@ -245,14 +272,27 @@ __device__ __forceinline__ void run_chunk(A* __restrict__ amp, const DevSweep* _
         }
         cur = P;
         const uint32_t nops = P->n_ops, ob = P->op_begin;
-        for (uint32_t o = 0; o < nops; ++o) apply_op<A, R, RB>(v, ops[ob + o], coef);
+        // the next descriptor's scalar loads are issued before the current op runs
+        DevOp op = nops ? ops[ob] : DevOp{};
+        for (uint32_t o = 0; o < nops; ++o) {
+            const DevOp nop = ops[ob + (o + 1 < nops ? o + 1 : o)];
+            apply_op<A, R, RB>(v, op, coef);
+            op = nop;
+        }
     }
+}
 
-    toff = 0;
-#pragma unroll
-    for (int b = 0; b < LOGT; ++b) toff |= (uint64_t)((t >> b) & 1u) << sw->st_tglob[b];
-#pragma unroll
-    for (int i = 0; i < R; ++i) amp[base | toff | sw->st_roff[i]] = v[i];
+// One chunk start to end (the batched and served kernels).
+template <class A, int LOGT, int RB>
+__device__ __forceinline__ void run_chunk(A* __restrict__ amp, const DevSweep* __restrict__ sw, uint64_t chunk,
+                                          const DevPhase* __restrict__ ph, const DevOp* __restrict__ ops,
+                                          const double* __restrict__ coef, A* __restrict__ lds, bool init_zero) {
+    A v[1 << RB];
+    const uint64_t base = chunk_base<LOGT, RB>(sw, chunk);
+    if (init_zero) zero_chunk<A, LOGT, RB>(v, sw, base);
+    else load_chunk<A, LOGT, RB>(v, amp, sw, base);
+    compute_chunk<A, LOGT, RB>(v, sw, ph, ops, coef, lds);
+    store_chunk<A, LOGT, RB>(v, amp, sw, base);
 }
 
 // ---------------------------------------------------------------------------------

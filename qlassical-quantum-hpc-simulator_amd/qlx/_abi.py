@@ -27,7 +27,7 @@ QLX_GSWAP_PERMUTE = 0
 QLX_GSWAP_NOOP = 1
 QLX_DENSE_AUTO = 0
 QLX_DENSE_VALU = 1
-QLX_DENSE_MFMA_BF16X3 = 2
+QLX_DENSE_MFMA_BF16 = 2
 QLX_DENSE_MFMA_F32 = 3
 
 

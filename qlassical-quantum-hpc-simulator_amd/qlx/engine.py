@@ -28,14 +28,14 @@ def _dt(dtype) -> int:
     return int(dtype)
 
 
-DENSE = {"auto": _abi.QLX_DENSE_AUTO, "valu": _abi.QLX_DENSE_VALU, "bf16x3": _abi.QLX_DENSE_MFMA_BF16X3,
+DENSE = {"auto": _abi.QLX_DENSE_AUTO, "valu": _abi.QLX_DENSE_VALU, "bf16": _abi.QLX_DENSE_MFMA_BF16,
          "f32": _abi.QLX_DENSE_MFMA_F32}
 
 
 def exec_opts(mode: str = "plain", window_qubits: int = 0, coalesce_qubits: int = 0,
               gswap: str = "permute", register_qubits: int = 0, dense: str = "auto") -> _abi.ExecOpts:
     """qlx_exec_opts. dense: arithmetic of 4/5-qubit dense gates on complex64 states
-    ('auto' = 'bf16x3' matrix cores, 'f32' matrix cores, 'valu' vector units)."""
+    ('auto' = 'bf16' matrix cores, 'f32' matrix cores, 'valu' vector units)."""
     o = _abi.ExecOpts()
     o.dense_kernel = DENSE[dense]
     o.mode = {"plain": _abi.QLX_EXEC_PLAIN, "windowed": _abi.QLX_EXEC_WINDOWED}[mode]
