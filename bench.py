@@ -254,6 +254,8 @@ def main():
     ap.add_argument("--mode", choices=["auto", "plain", "windowed"], default="auto")
     ap.add_argument("--window", type=int, default=0, help="window qubits (0 = default for the precision)")
     ap.add_argument("--regs", type=int, default=0, help="register qubits per work-item (0 = default)")
+    ap.add_argument("--sweep-wg", type=int, default=0,
+                    help="work-groups per window-sweep launch (0 = every resident slot)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", choices=["circuit30", "precision", "batch", "serve"], default="circuit30")
@@ -296,7 +298,7 @@ def main():
     st = qlx.State(nq, "c128")
     mode, opts = None, None
     for m in modes:
-        o = qlx.exec_opts(m, window_qubits=args.window, register_qubits=args.regs)
+        o = qlx.exec_opts(m, window_qubits=args.window, register_qubits=args.regs, sweep_workgroups=args.sweep_wg)
         try:
             st.execute(prog.slice(0, 1), o)
             mode, opts = m, o

@@ -91,7 +91,9 @@ typedef struct qlx_exec_opts {
                                  (c128: 4, c64: 5). Built (window, register) pairs:
                                  c128 (12,4) (12,5) (11,4) (11,3); c64 (13,5) (12,5) (12,4) */
     uint32_t dense_kernel;    /* QLX_DENSE_*: arithmetic of UNITARY / FUSED_BLOCK gates */
-    uint32_t reserved[2];
+    uint32_t sweep_workgroups; /* window sweeps with <= 16 registers per work-item: work-groups
+                                  per launch (each loops over chunks); 0 = every resident slot */
+    uint32_t reserved[1];
 } qlx_exec_opts;
 
 /* Dense-gate arithmetic (qlx_exec_opts::dense_kernel). The MFMA forms apply to 4- and

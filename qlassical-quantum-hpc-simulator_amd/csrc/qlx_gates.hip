@@ -43,8 +43,9 @@ __device__ inline uint64_t insert_zero(uint64_t idx, uint32_t t) {
 template <class A, int KIND>
 __global__ void __launch_bounds__(kThreads) k_1q(A* __restrict__ amp, uint64_t npairs, uint32_t t, Coef1 cf) {
     using R = typename RealOf<A>::T;
-    const uint64_t k = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (k >= npairs) return;
+    // grid-stride: a launch holds fewer than 2^32 work-items, a 32-qubit state has 2^31 pairs
+    for (uint64_t k = (uint64_t)blockIdx.x * kThreads + threadIdx.x; k < npairs;
+         k += (uint64_t)gridDim.x * kThreads) {
     const uint64_t i0 = insert_zero(k, t);
     const uint64_t i1 = i0 | (1ull << t);
     if constexpr (KIND == K_Z) {
@@ -90,6 +91,7 @@ __global__ void __launch_bounds__(kThreads) k_1q(A* __restrict__ amp, uint64_t n
         amp[i0] = o0;
         amp[i1] = o1;
     }
+    }
 }
 
 // Two zero holes at qlo < qhi; swaps amp[base|ma] <-> amp[base|mb].
@@ -98,14 +100,15 @@ __global__ void __launch_bounds__(kThreads) k_1q(A* __restrict__ amp, uint64_t n
 template <class A>
 __global__ void __launch_bounds__(kThreads) k_2hole(A* __restrict__ amp, uint64_t nquads, uint32_t qlo,
                                                    uint32_t qhi, uint64_t ma, uint64_t mb) {
-    const uint64_t k = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (k >= nquads) return;
-    const uint64_t base = insert_zero(insert_zero(k, qlo), qhi);
-    const uint64_t ia = base | ma, ib = base | mb;
-    const A x = amp[ia];
-    const A y = amp[ib];
-    amp[ia] = y;
-    amp[ib] = x;
+    for (uint64_t k = (uint64_t)blockIdx.x * kThreads + threadIdx.x; k < nquads;
+         k += (uint64_t)gridDim.x * kThreads) {
+        const uint64_t base = insert_zero(insert_zero(k, qlo), qhi);
+        const uint64_t ia = base | ma, ib = base | mb;
+        const A x = amp[ia];
+        const A y = amp[ib];
+        amp[ia] = y;
+        amp[ib] = x;
+    }
 }
 
 // Three zero holes (sorted q0 < q1 < q2); swaps amp[base|ma] <-> amp[base|mb].
@@ -113,14 +116,15 @@ __global__ void __launch_bounds__(kThreads) k_2hole(A* __restrict__ amp, uint64_
 template <class A>
 __global__ void __launch_bounds__(kThreads) k_3hole(A* __restrict__ amp, uint64_t nocts, uint32_t q0,
                                                    uint32_t q1, uint32_t q2, uint64_t ma, uint64_t mb) {
-    const uint64_t k = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (k >= nocts) return;
-    const uint64_t base = insert_zero(insert_zero(insert_zero(k, q0), q1), q2);
-    const uint64_t ia = base | ma, ib = base | mb;
-    const A x = amp[ia];
-    const A y = amp[ib];
-    amp[ia] = y;
-    amp[ib] = x;
+    for (uint64_t k = (uint64_t)blockIdx.x * kThreads + threadIdx.x; k < nocts;
+         k += (uint64_t)gridDim.x * kThreads) {
+        const uint64_t base = insert_zero(insert_zero(insert_zero(k, q0), q1), q2);
+        const uint64_t ia = base | ma, ib = base | mb;
+        const A x = amp[ia];
+        const A y = amp[ib];
+        amp[ia] = y;
+        amp[ib] = x;
+    }
 }
 
 // ---- dense k-qubit unitary (UNITARY / FUSED_BLOCK), k <= 5 ------------------------
@@ -140,44 +144,50 @@ __global__ void __launch_bounds__(kThreads) k_dense(A* __restrict__ amp, uint64_
     constexpr int D = 1 << K;
     constexpr int RPW = D >= 4 ? D / 4 : 1;   // rows per wave
     __shared__ A tile[D][64];
-    const uint64_t g0 = (uint64_t)blockIdx.x * 64;
-    for (int e = threadIdx.x; e < D * 64; e += kThreads) {
-        const int j = e >> 6, g = e & 63;
-        const uint64_t gg = g0 + g;
-        if (gg < ngroups) {
-            uint64_t base = gg;
-#pragma unroll
-            for (int q = 0; q < K; ++q) base = insert_zero(base, da.sorted[q]);
-            tile[j][g] = amp[base | da.off[j]];
-        }
-    }
-    __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (wave >= D) return;  // D == 2: two waves idle
-    R accr[RPW], acci[RPW];
+    const uint64_t ntiles = (ngroups + 63) / 64;
+    for (uint64_t tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
+        const uint64_t g0 = tix * 64;
+        for (int e = threadIdx.x; e < D * 64; e += kThreads) {
+            const int j = e >> 6, g = e & 63;
+            const uint64_t gg = g0 + g;
+            if (gg < ngroups) {
+                uint64_t base = gg;
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) accr[r] = acci[r] = R(0);
-#pragma unroll 4
-    for (int m = 0; m < D; ++m) {
-        const A x = tile[m][lane];
-#pragma unroll
-        for (int r = 0; r < RPW; ++r) {
-            const int j = wave + 4 * r;
-            const R ur = (R)U[2 * (j * D + m)], ui = (R)U[2 * (j * D + m) + 1];
-            accr[r] += ur * x.x - ui * x.y;
-            acci[r] += ur * x.y + ui * x.x;
+                for (int q = 0; q < K; ++q) base = insert_zero(base, da.sorted[q]);
+                tile[j][g] = amp[base | da.off[j]];
+            }
         }
-    }
-    const uint64_t gg = g0 + lane;
-    if (gg >= ngroups) return;
-    uint64_t base = gg;
+        __syncthreads();
+        if (wave < D) {  // D == 2: two waves idle
+            R accr[RPW], acci[RPW];
 #pragma unroll
-    for (int q = 0; q < K; ++q) base = insert_zero(base, da.sorted[q]);
+            for (int r = 0; r < RPW; ++r) accr[r] = acci[r] = R(0);
+#pragma unroll 4
+            for (int m = 0; m < D; ++m) {
+                const A x = tile[m][lane];
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-        const int j = wave + 4 * r;
-        amp[base | da.off[j]] = A{accr[r], acci[r]};
+                for (int r = 0; r < RPW; ++r) {
+                    const int j = wave + 4 * r;
+                    const R ur = (R)U[2 * (j * D + m)], ui = (R)U[2 * (j * D + m) + 1];
+                    accr[r] += ur * x.x - ui * x.y;
+                    acci[r] += ur * x.y + ui * x.x;
+                }
+            }
+            const uint64_t gg = g0 + lane;
+            if (gg < ngroups) {
+                uint64_t base = gg;
+#pragma unroll
+                for (int q = 0; q < K; ++q) base = insert_zero(base, da.sorted[q]);
+#pragma unroll
+                for (int r = 0; r < RPW; ++r) {
+                    const int j = wave + 4 * r;
+                    amp[base | da.off[j]] = A{accr[r], acci[r]};
+                }
+            }
+        }
+        __syncthreads();  // the tile is read completely before the next one is staged
     }
 }
 
@@ -209,27 +219,29 @@ template <class A>
 __global__ void __launch_bounds__(kThreads) k_collapse(A* __restrict__ amp, uint64_t half, uint32_t t,
                                                       int outcome, double inv_norm) {
     using R = typename RealOf<A>::T;
-    const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= half) return;
-    const uint64_t i0 = insert_zero(i, t), i1 = i0 | (1ull << t);
-    const uint64_t keep = outcome ? i1 : i0, drop = outcome ? i0 : i1;
-    A v = amp[keep];
-    if constexpr (sizeof(R) == 8) {
-        v = A{v.x * inv_norm, v.y * inv_norm};
-    } else {
-        v = A{(R)((double)v.x * inv_norm), (R)((double)v.y * inv_norm)};
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < half;
+         i += (uint64_t)gridDim.x * kThreads) {
+        const uint64_t i0 = insert_zero(i, t), i1 = i0 | (1ull << t);
+        const uint64_t keep = outcome ? i1 : i0, drop = outcome ? i0 : i1;
+        A v = amp[keep];
+        if constexpr (sizeof(R) == 8) {
+            v = A{v.x * inv_norm, v.y * inv_norm};
+        } else {
+            v = A{(R)((double)v.x * inv_norm), (R)((double)v.y * inv_norm)};
+        }
+        amp[keep] = v;
+        amp[drop] = A{R(0), R(0)};
     }
-    amp[keep] = v;
-    amp[drop] = A{R(0), R(0)};
 }
 
 // ---- host side -----------------------------------------------------------------------
 
+// Blocks for n work-items, capped so a launch stays below 2^32 work-items (the kernels
+// are grid-stride loops; above 2^30 items each work-item takes several).
 static unsigned grid_for(uint64_t n) {
     uint64_t b = (n + kThreads - 1) / kThreads;
     if (b == 0) b = 1;
-    if (b > 0x7FFFFFFFull) fail(QLX_ERANGE, "state too large for one launch");
-    return (unsigned)b;
+    return (unsigned)std::min<uint64_t>(b, uint64_t{1} << 22);
 }
 
 template <class A>
@@ -280,7 +292,7 @@ static void launch_3hole(qlx_state* st, uint32_t a, uint32_t b, uint32_t c, uint
 template <class A>
 static void launch_dense_t(qlx_state* st, const double* dU, const DenseArgs& da) {
     const uint64_t ngroups = 1ull << (st->nq - da.k);
-    const unsigned g = (unsigned)((ngroups + 63) / 64);
+    const unsigned g = (unsigned)std::min<uint64_t>((ngroups + 63) / 64, uint64_t{1} << 22);
     A* amp = (A*)st->amp;
     switch (da.k) {
         case 1: k_dense<A, 1><<<g, kThreads, 0, st->stream>>>(amp, ngroups, dU, da); break;

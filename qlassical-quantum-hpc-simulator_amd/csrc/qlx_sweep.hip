@@ -524,7 +524,7 @@ static uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
 
 template <class A, int LOGT, int RB>
 static void launch_sweep(qlx_state* st, const char* dev, size_t off_sw, size_t off_ph, size_t off_op, size_t off_cf,
-                         int si) {
+                         int si, uint32_t wg_override) {
     const uint64_t chunks = 1ull << (st->nq - (LOGT + RB));
     const size_t lds = (size_t(1) << (LOGT + RB)) * sizeof(A);
     static thread_local int resident[2][8][8] = {};  // work-groups per CU, by (dtype, LOGT-5, RB-1)
@@ -536,7 +536,8 @@ static void launch_sweep(qlx_state* st, const char* dev, size_t off_sw, size_t o
     int dev_id = 0, cus = 0;
     QLX_HIP_CHECK(hipGetDevice(&dev_id));
     QLX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id));
-    const uint64_t grid = RB <= 4 ? std::min<uint64_t>(chunks, (uint64_t)per_cu * (uint64_t)cus) : chunks;
+    uint64_t grid = RB <= 4 ? std::min<uint64_t>(chunks, (uint64_t)per_cu * (uint64_t)cus) : chunks;
+    if (RB <= 4 && wg_override) grid = std::min<uint64_t>(chunks, wg_override);
     k_sweep<A, LOGT, RB><<<(unsigned)grid, 1 << LOGT, lds, st->stream>>>(
         (A*)st->amp, (const DevSweep*)(dev + off_sw) + si, (const DevPhase*)(dev + off_ph),
         (const DevOp*)(dev + off_op), (const double*)(dev + off_cf), chunks);
@@ -618,13 +619,13 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
             const size_t a = cp.off_sw, b = cp.off_ph, c = cp.off_op, d = cp.off_cf;
             const int si = s.sweep_index;
             switch (key) {
-                case 1124: launch_sweep<double2, 8, 4>(st, dev, a, b, c, d, si); break;
-                case 1125: launch_sweep<double2, 7, 5>(st, dev, a, b, c, d, si); break;
-                case 1114: launch_sweep<double2, 7, 4>(st, dev, a, b, c, d, si); break;
-                case 1113: launch_sweep<double2, 8, 3>(st, dev, a, b, c, d, si); break;
-                case 135: launch_sweep<float2, 8, 5>(st, dev, a, b, c, d, si); break;
-                case 125: launch_sweep<float2, 7, 5>(st, dev, a, b, c, d, si); break;
-                case 124: launch_sweep<float2, 8, 4>(st, dev, a, b, c, d, si); break;
+                case 1124: launch_sweep<double2, 8, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                case 1125: launch_sweep<double2, 7, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                case 1114: launch_sweep<double2, 7, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                case 1113: launch_sweep<double2, 8, 3>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                case 135: launch_sweep<float2, 8, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                case 125: launch_sweep<float2, 7, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                case 124: launch_sweep<float2, 8, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
                 default: fail(QLX_EUNSUPPORTED, "sweep geometry not built");
             }
         }

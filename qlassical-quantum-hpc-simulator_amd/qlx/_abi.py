@@ -70,7 +70,8 @@ class ExecOpts(C.Structure):
         ("gswap_mode", C.c_uint32),
         ("register_qubits", C.c_uint32),
         ("dense_kernel", C.c_uint32),
-        ("reserved", C.c_uint32 * 2),
+        ("sweep_workgroups", C.c_uint32),
+        ("reserved", C.c_uint32 * 1),
     ]
 
 
