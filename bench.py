@@ -93,11 +93,16 @@ def run_precision(args):
     kinds = {int(k): int(v) for k, v in zip(*np.unique(fused.meta["num_qubits"], return_counts=True))}
     log(f"precision: {nq}q depth {depth}: {len(prog)} gates -> {len(fused)} instructions "
         f"(fused widths {kinds}), fusion {t_fuse:.2f} s")
+    # "mode:dense": the dense gates' arithmetic, the other gates one kernel each (plain) or
+    # grouped into window sweeps between the dense passes (windowed)
     variants = [v for v in args.dense_variants.split(",") if v]
+    def opts_of(v):
+        mode, dense = v.split(":")
+        return qlx.exec_opts(mode, dense=dense)
     res = {}
     st = qlx.State(nq, "c64")
     for v in variants:
-        o = qlx.exec_opts("plain", dense=v)
+        o = opts_of(v)
         st.reset()
         for _ in range(args.warmup):
             st.execute(fused, o)
@@ -121,7 +126,7 @@ def run_precision(args):
     acc = {}
     for v in variants:
         st.reset()
-        st.execute(fused, qlx.exec_opts("plain", dense=v))
+        st.execute(fused, opts_of(v))
         st.sync()
         ip = st.inner(ref)
         n64 = st.norm_sq()
@@ -259,8 +264,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", choices=["circuit30", "precision", "batch", "serve"], default="circuit30")
-    ap.add_argument("--dense-variants", default="bf16,f32,valu",
-                    help="precision config: dense-gate arithmetic forms to time (first one is reported)")
+    ap.add_argument("--dense-variants", default="windowed:bf16,plain:bf16,windowed:f32,windowed:valu",
+                    help="precision config: mode:dense-arithmetic forms to time (the first one is reported)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--batch-circuits", type=int, default=4096)
