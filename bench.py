@@ -259,6 +259,8 @@ def main():
     ap.add_argument("--mode", choices=["auto", "plain", "windowed"], default="auto")
     ap.add_argument("--window", type=int, default=0, help="window qubits (0 = default for the precision)")
     ap.add_argument("--regs", type=int, default=0, help="register qubits per work-item (0 = default)")
+    ap.add_argument("--generic", action="store_true",
+                    help="windowed: run the generic sweep kernel instead of circuit-specialised ones")
     ap.add_argument("--sweep-wg", type=int, default=0,
                     help="work-groups per window-sweep launch (0 = every resident slot)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
@@ -303,7 +305,8 @@ def main():
     st = qlx.State(nq, "c128")
     mode, opts = None, None
     for m in modes:
-        o = qlx.exec_opts(m, window_qubits=args.window, register_qubits=args.regs, sweep_workgroups=args.sweep_wg)
+        o = qlx.exec_opts(m, window_qubits=args.window, register_qubits=args.regs, sweep_workgroups=args.sweep_wg,
+                          specialize=(m == "windowed" and not args.generic))
         try:
             st.execute(prog.slice(0, 1), o)
             mode, opts = m, o
@@ -312,7 +315,12 @@ def main():
             continue
     st.reset()
     log(f"bench: {nq}q depth {depth} ({len(prog)} gates), mode {mode}, warmup {args.warmup}, steps {args.steps}")
-    for _ in range(args.warmup):
+    t0 = time.perf_counter()
+    st.execute(prog, opts)       # plans (and, when specialising, compiles) the program
+    st.sync()
+    first_s = time.perf_counter() - t0
+    log(f"bench: first execution (planning{' + kernel compilation' if opts.specialize else ''}) {first_s:.2f} s")
+    for _ in range(max(0, args.warmup - 1)):
         st.execute(prog, opts)
     st.sync()
     st.timing(True)
@@ -333,6 +341,18 @@ def main():
     bytes_step = stats["algorithmic_bytes"]
     gbps = bytes_step / (ms / 1e3) / 1e9
     dom = max(timing.items(), key=lambda kv: kv[1][1])
+    generic = None
+    if opts.specialize:
+        # the same circuit through the generic (descriptor-interpreting) sweep kernel
+        og = qlx.exec_opts(mode, window_qubits=args.window, register_qubits=args.regs)
+        st.execute(prog, og)
+        st.sync()
+        tg = time.perf_counter()
+        for _ in range(max(1, args.steps // 2)):
+            st.execute(prog, og)
+        st.sync()
+        generic = (time.perf_counter() - tg) / max(1, args.steps // 2) * 1e3
+        log(f"bench: generic sweep kernel {generic:.2f} ms/step")
     name, (launches, tot_ms, tot_bytes) = dom
     avg_ms = tot_ms / launches
     achieved = (tot_bytes / launches) / (avg_ms / 1e3)
@@ -358,6 +378,9 @@ def main():
             "sweeps_per_step": {"full": stats["sweeps_full"], "half": stats["sweeps_half"],
                                 "quarter": stats["sweeps_quarter"]},
             "circuit_ms": round(ms, 3),
+            "sweep_kernels": "circuit-specialised (compiled at first execution)" if opts.specialize else "generic",
+            "first_execution_s": round(first_s, 3),
+            "generic_kernel_ms_per_step": round(generic, 3) if generic else None,
             "effective_gbps_unfused_equiv": round(unfused_pass_eq * 2 * 16 * (1 << nq) / (ms / 1e3) / 1e9, 1),
             "hbm_frac_circuit": round(gbps * 1e9 / PEAK_HBM, 4),
         },

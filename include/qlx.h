@@ -93,7 +93,9 @@ typedef struct qlx_exec_opts {
     uint32_t dense_kernel;    /* QLX_DENSE_*: arithmetic of UNITARY / FUSED_BLOCK gates */
     uint32_t sweep_workgroups; /* window sweeps with <= 16 registers per work-item: work-groups
                                   per launch (each loops over chunks); 0 = every resident slot */
-    uint32_t reserved[1];
+    uint32_t specialize;      /* windowed: 1 = compile the program's sweeps into straight-line
+                                 kernels (hipRTC, once per program and state; later executions of
+                                 the same program reuse them); 0 = the generic sweep kernel */
 } qlx_exec_opts;
 
 /* Dense-gate arithmetic (qlx_exec_opts::dense_kernel). The MFMA forms apply to 4- and

@@ -1,0 +1,296 @@
+// qlx_jit.cpp — circuit-specialised window-sweep kernels, compiled at run time (hipRTC).
+//
+// The generic sweep kernel (qlx_sweep_core.hpp) interprets a sweep's descriptors: a
+// runtime switch per op, register moves at the switch merge, LDS addresses built per
+// register. For a program that is executed many times on a large state, the same
+// sweeps can instead be emitted as straight-line HIP source — every register index,
+// qubit position, LDS offset and gate coefficient a compile-time constant, CX/SWAP
+// reduced to renaming registers in the generator — and compiled once for gfx950.
+// The generated code follows the generic kernel operation by operation (the same
+// descriptors: DevSweep / DevPhase / DevOp and the coefficient array), so both compute
+// the same gates in the same order with the same expressions.
+
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "qlx_jit.hpp"
+
+namespace qlx {
+namespace {
+
+std::string lit(double x, bool f32) {
+    char b[64];
+    if (f32) std::snprintf(b, sizeof b, "%af", (double)(float)x);
+    else std::snprintf(b, sizeof b, "%a", x);
+    return b;
+}
+
+std::string hex64(uint64_t x) {
+    char b[32];
+    std::snprintf(b, sizeof b, "0x%llxull", (unsigned long long)x);
+    return b;
+}
+
+// swz of the generic kernel (linear over XOR on disjoint bit sets).
+uint32_t swz_host(uint32_t l, int elem) {
+    const int sh = elem == 16 ? 4 : 5;
+    const uint32_t m = (1u << sh) - 1;
+    return l ^ (((l >> sh) ^ (l >> (2 * sh)) ^ (l >> (3 * sh))) & m);
+}
+
+struct Gen {
+    const SweepPlan& p;
+    int si;
+    bool f32;
+    int logt, rb, w, R, elem;
+    std::string out;
+    std::vector<int> map;  // register index -> variable
+    int tmp = 0;
+
+    void line(const std::string& s) { out += s; out += '\n'; }
+    std::string var(int i) const { return "r" + std::to_string(map[i]); }
+
+    std::string thread_bits(const uint32_t* pos) const {
+        std::string e = "0u";
+        for (int b = 0; b < logt; ++b)
+            e += " | (((t >> " + std::to_string(b) + ") & 1u) << " + std::to_string(pos[b]) + ")";
+        return e;
+    }
+    std::string thread_bits64(const uint32_t* pos) const {
+        std::string e = "0ull";
+        for (int b = 0; b < logt; ++b)
+            e += " | ((unsigned long long)((t >> " + std::to_string(b) + ") & 1u) << " + std::to_string(pos[b]) + ")";
+        return e;
+    }
+
+    void op(const DevOp& o) {
+        const int kind = (int)(o.key >> 8), b0 = (int)((o.key >> 4) & 15), b1 = (int)(o.key & 15);
+        const std::string T = f32 ? "float" : "double", T2 = f32 ? "float2" : "double2";
+        const double* m = p.coef.data() + o.coef;
+        const std::string id = std::to_string(tmp++);
+        switch (kind) {
+            case OP_RX: {
+                line("  { const " + T + " c = " + lit(o.c0, f32) + ", s = " + lit(o.c1, f32) + ";");
+                for (int i = 0; i < R; ++i) {
+                    if (i & (1 << b0)) continue;
+                    const std::string x = var(i), y = var(i | (1 << b0));
+                    line("    { const " + T2 + " a0 = " + x + ", a1 = " + y + "; " + x + " = " + T2 +
+                         "{c * a0.x + s * a1.y, c * a0.y + (-s * a1.x)}; " + y + " = " + T2 +
+                         "{s * a0.y + c * a1.x, -s * a0.x + c * a1.y}; }");
+                }
+                line("  }");
+                break;
+            }
+            case OP_H: {
+                line("  { const " + T + " h = " + lit(o.c0, f32) + ";");
+                for (int i = 0; i < R; ++i) {
+                    if (i & (1 << b0)) continue;
+                    const std::string x = var(i), y = var(i | (1 << b0));
+                    line("    { const " + T2 + " a0 = " + x + ", a1 = " + y + "; " + x + " = " + T2 +
+                         "{h * (a0.x + a1.x), h * (a0.y + a1.y)}; " + y + " = " + T2 +
+                         "{h * (a0.x - a1.x), h * (a0.y - a1.y)}; }");
+                }
+                line("  }");
+                break;
+            }
+            case OP_U2: {
+                std::string d = "  { const " + T + " ";
+                const char* nm[8] = {"m0r", "m0i", "m1r", "m1i", "m2r", "m2i", "m3r", "m3i"};
+                for (int k = 0; k < 8; ++k) d += std::string(k ? ", " : "") + nm[k] + " = " + lit(m[k], f32);
+                line(d + ";");
+                for (int i = 0; i < R; ++i) {
+                    if (i & (1 << b0)) continue;
+                    const std::string x = var(i), y = var(i | (1 << b0));
+                    line("    { const " + T2 + " a0 = " + x + ", a1 = " + y + "; " + x + " = " + T2 +
+                         "{m0r * a0.x - m0i * a0.y + m1r * a1.x - m1i * a1.y, m0r * a0.y + m0i * a0.x + m1r * a1.y + m1i * a1.x}; " +
+                         y + " = " + T2 +
+                         "{m2r * a0.x - m2i * a0.y + m3r * a1.x - m3i * a1.y, m2r * a0.y + m2i * a0.x + m3r * a1.y + m3i * a1.x}; }");
+                }
+                line("  }");
+                break;
+            }
+            case OP_CX:  // control b0, target b1: rename
+                for (int i = 0; i < R; ++i)
+                    if ((i & (1 << b0)) && !(i & (1 << b1))) std::swap(map[i], map[i | (1 << b1)]);
+                break;
+            case OP_SWAP:
+                for (int i = 0; i < R; ++i)
+                    if ((i & (1 << b0)) && !(i & (1 << b1))) std::swap(map[i], map[i ^ (1 << b0) ^ (1 << b1)]);
+                break;
+            case OP_U4: {
+                std::string d = "  { const " + T + " ";
+                for (int k = 0; k < 32; ++k) d += std::string(k ? ", " : "") + "u" + std::to_string(k) + " = " + lit(m[k], f32);
+                line(d + ";");
+                for (int i = 0; i < R; ++i) {
+                    if ((i & (1 << b0)) || (i & (1 << b1))) continue;
+                    const int idx[4] = {i, i | (1 << b0), i | (1 << b1), i | (1 << b0) | (1 << b1)};
+                    std::string s = "    { const " + T2 + " x0 = " + var(idx[0]) + ", x1 = " + var(idx[1]) + ", x2 = " +
+                                    var(idx[2]) + ", x3 = " + var(idx[3]) + ";";
+                    for (int r = 0; r < 4; ++r) {
+                        s += " { " + T + " re = " + T + "(0), im = " + T + "(0);";
+                        for (int c = 0; c < 4; ++c) {
+                            const std::string ur = "u" + std::to_string(2 * (4 * r + c));
+                            const std::string ui = "u" + std::to_string(2 * (4 * r + c) + 1);
+                            const std::string xc = "x" + std::to_string(c);
+                            s += " re += " + ur + " * " + xc + ".x - " + ui + " * " + xc + ".y; im += " + ur + " * " + xc +
+                                 ".y + " + ui + " * " + xc + ".x;";
+                        }
+                        s += " " + var(idx[r]) + " = " + T2 + "{re, im}; }";
+                    }
+                    line(s + " }");
+                }
+                line("  }");
+                break;
+            }
+            default:
+                fail(QLX_EUNSUPPORTED, "specialised sweep: unknown op kind");
+        }
+    }
+
+    void emit() {
+        const DevSweep& sw = p.dsw[si];
+        const std::string T2 = f32 ? "float2" : "double2";
+        const int sh = elem == 16 ? 4 : 5;
+        line("extern \"C\" __global__ void __launch_bounds__(" + std::to_string(1 << logt) + ") qlx_sweep_spec_" +
+             std::to_string(si) + "(" + T2 + "* __restrict__ amp) {");
+        line("  __shared__ " + T2 + " lds[" + std::to_string(1 << w) + "];");
+        line("  const unsigned t = threadIdx.x;");
+        line("  unsigned long long b = blockIdx.x;");
+        for (int j = 0; j < w; ++j) {
+            const unsigned q = sw.S[j];
+            line("  b = ((b >> " + std::to_string(q) + ") << " + std::to_string(q + 1) + ") | (b & " +
+                 hex64((1ull << q) - 1) + ");");
+        }
+        line("  const unsigned long long base = b;");
+        line("  const unsigned long long lo = base | " + thread_bits64(sw.ld_tglob) + ";");
+        map.resize(R);
+        for (int i = 0; i < R; ++i) {
+            map[i] = i;
+            line("  " + T2 + " r" + std::to_string(i) + " = amp[lo | " + hex64(sw.ld_roff[i]) + "];");
+        }
+        const DevPhase* cur = &p.dph[sw.phase_begin];
+        for (uint32_t ph = 0; ph < sw.n_phases; ++ph) {
+            const DevPhase& P = p.dph[sw.phase_begin + ph];
+            if (P.remap) {
+                line("  {");
+                line("    const unsigned l0 = " + thread_bits(cur->tlocal) + ";");
+                line("    const unsigned s0 = l0 ^ (((l0 >> " + std::to_string(sh) + ") ^ (l0 >> " + std::to_string(2 * sh) +
+                     ") ^ (l0 >> " + std::to_string(3 * sh) + ")) & " + std::to_string((1 << sh) - 1) + "u);");
+                for (int i = 0; i < R; ++i)
+                    line("    lds[s0 ^ " + std::to_string(swz_host(cur->roff[i], elem)) + "u] = " + var(i) + ";");
+                line("    __syncthreads();");
+                line("    const unsigned l1 = " + thread_bits(P.tlocal) + ";");
+                line("    const unsigned s1 = l1 ^ (((l1 >> " + std::to_string(sh) + ") ^ (l1 >> " + std::to_string(2 * sh) +
+                     ") ^ (l1 >> " + std::to_string(3 * sh) + ")) & " + std::to_string((1 << sh) - 1) + "u);");
+                for (int i = 0; i < R; ++i) {
+                    map[i] = i;
+                    line("    r" + std::to_string(i) + " = lds[s1 ^ " + std::to_string(swz_host(P.roff[i], elem)) + "u];");
+                }
+                line("    __syncthreads();");
+                line("  }");
+            }
+            cur = &P;
+            for (uint32_t k = 0; k < P.n_ops; ++k) op(p.dop[P.op_begin + k]);
+        }
+        line("  const unsigned long long so = base | " + thread_bits64(sw.st_tglob) + ";");
+        for (int i = 0; i < R; ++i) line("  amp[so | " + hex64(sw.st_roff[i]) + "] = " + var(i) + ";");
+        line("}");
+    }
+};
+
+std::vector<char> compile(const std::string& src, const std::string& name) {
+    hiprtcProgram prog = nullptr;
+    if (hiprtcCreateProgram(&prog, src.c_str(), name.c_str(), 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+        fail(QLX_EHIP, "hiprtcCreateProgram failed");
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult r = hiprtcCompileProgram(prog, 6, opts);
+    std::vector<char> code;
+    if (r != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(prog, log.data());
+        hiprtcDestroyProgram(&prog);
+        fail(QLX_EHIP, "specialised sweep compilation failed: " + log.substr(0, 2000));
+    }
+    size_t n = 0;
+    hiprtcGetCodeSize(prog, &n);
+    code.resize(n);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return code;
+}
+
+}  // namespace
+
+std::string jit_sweep_source(const SweepPlan& p, int sweep_index, int dtype) {
+    if (sweep_index < 0 || (size_t)sweep_index >= p.dsw.size()) fail(QLX_ERANGE, "sweep index out of range");
+    Gen g{p, sweep_index, dtype == QLX_C64, p.geo.logt, p.geo.rb, p.geo.w, 1 << p.geo.rb, dtype == QLX_C64 ? 8 : 16, {}, {}};
+    g.emit();
+    return g.out;
+}
+
+JitSweeps::~JitSweeps() {
+    for (hipModule_t m : modules)
+        if (m) (void)hipModuleUnload(m);
+}
+
+std::shared_ptr<JitSweeps> build_jit_sweeps(const SweepPlan& p, int dtype) {
+    auto t0 = std::chrono::steady_clock::now();
+    const size_t n = p.dsw.size();
+    std::vector<std::string> src(n);
+    for (size_t s = 0; s < n; ++s) src[s] = jit_sweep_source(p, (int)s, dtype);
+    std::vector<std::vector<char>> code(n);
+    std::vector<std::string> err(n);
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    std::mutex mu;
+    size_t next = 0;
+    for (unsigned k = 0; k < std::min<size_t>(hw, n); ++k)
+        pool.emplace_back([&] {
+            for (;;) {
+                size_t s;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (next >= n) return;
+                    s = next++;
+                }
+                try {
+                    code[s] = compile(src[s], "qlx_sweep_spec_" + std::to_string(s) + ".hip");
+                } catch (const Error& e) {
+                    err[s] = e.msg;
+                }
+            }
+        });
+    for (auto& th : pool) th.join();
+    for (size_t s = 0; s < n; ++s)
+        if (!err[s].empty()) fail(QLX_EHIP, err[s]);
+    auto j = std::make_shared<JitSweeps>();
+    j->fns.resize(n);
+    j->modules.resize(n, nullptr);
+    for (size_t s = 0; s < n; ++s) {
+        QLX_HIP_CHECK(hipModuleLoadData(&j->modules[s], code[s].data()));
+        const std::string name = "qlx_sweep_spec_" + std::to_string(s);
+        QLX_HIP_CHECK(hipModuleGetFunction(&j->fns[s], j->modules[s], name.c_str()));
+    }
+    j->threads = 1u << p.geo.logt;
+    j->compile_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return j;
+}
+
+void launch_jit_sweep(const JitSweeps& j, int sweep_index, qlx_state* st, int w) {
+    const uint64_t chunks = 1ull << (st->nq - (uint32_t)w);
+    void* amp = st->amp;
+    void* args[] = {&amp};
+    QLX_HIP_CHECK(hipModuleLaunchKernel(j.fns[sweep_index], (unsigned)chunks, 1, 1, j.threads, 1, 1, 0, st->stream,
+                                        args, nullptr));
+}
+
+}  // namespace qlx

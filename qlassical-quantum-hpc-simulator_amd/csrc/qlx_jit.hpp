@@ -1,0 +1,32 @@
+// qlx_jit.hpp — circuit-specialised window-sweep kernels compiled at run time (hipRTC).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "qlx_sweep_core.hpp"
+
+namespace qlx {
+
+// Compiled sweep kernels of one plan (one module per sweep; unloaded with the object).
+struct JitSweeps {
+    std::vector<hipModule_t> modules;
+    std::vector<hipFunction_t> fns;
+    unsigned threads = 0;
+    double compile_s = 0.0;
+    JitSweeps() = default;
+    JitSweeps(const JitSweeps&) = delete;
+    JitSweeps& operator=(const JitSweeps&) = delete;
+    ~JitSweeps();
+};
+
+// HIP source of sweep `sweep_index` of the plan, specialised to its descriptors.
+std::string jit_sweep_source(const SweepPlan& p, int sweep_index, int dtype);
+// Generates and compiles every sweep of the plan for gfx950 (parallel host threads),
+// then loads them on the current device.
+std::shared_ptr<JitSweeps> build_jit_sweeps(const SweepPlan& p, int dtype);
+// One launch of a compiled sweep over the state (one work-group per chunk).
+void launch_jit_sweep(const JitSweeps& j, int sweep_index, qlx_state* st, int window_qubits);
+
+}  // namespace qlx

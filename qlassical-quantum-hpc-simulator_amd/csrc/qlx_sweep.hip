@@ -25,6 +25,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "qlx_jit.hpp"
 #include "qlx_sweep_core.hpp"
 
 namespace qlx {
@@ -551,6 +552,7 @@ struct CachedPlan {
     std::unique_ptr<SweepPlan> plan;
     DevBuf dev;
     size_t off_sw = 0, off_ph = 0, off_op = 0, off_cf = 0;
+    std::shared_ptr<JitSweeps> jit;  // specialised kernels of `plan` (built on first use)
 };
 
 static CachedPlan& plan_cache(qlx_state* st) {
@@ -584,6 +586,7 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
     if (!cp.plan || cp.key != key) {
         cp.plan = build_plan(st->nq, st->dtype, instrs, n, pool, meta, opts);
         cp.key = key;
+        cp.jit.reset();
         SweepPlan& p = *cp.plan;
         cp.off_sw = 0;
         cp.off_ph = align_up(cp.off_sw + p.dsw.size() * sizeof(DevSweep), 256);
@@ -613,7 +616,11 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
             launch_plain_instr(st, instrs[s.src], dpool, meta, opts, stats);
             continue;
         }
-        {
+        if (opts.specialize) {
+            if (!cp.jit) cp.jit = build_jit_sweeps(p, st->dtype);
+            TimedLaunch tl(st, "k_sweep_spec", pass);
+            launch_jit_sweep(*cp.jit, s.sweep_index, st, geo.w);
+        } else {
             TimedLaunch tl(st, "k_sweep", pass);
             const int key = (st->dtype == QLX_C128 ? 1000 : 0) + geo.w * 10 + geo.rb;
             const size_t a = cp.off_sw, b = cp.off_ph, c = cp.off_op, d = cp.off_cf;

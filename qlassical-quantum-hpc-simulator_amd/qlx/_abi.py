@@ -71,7 +71,7 @@ class ExecOpts(C.Structure):
         ("register_qubits", C.c_uint32),
         ("dense_kernel", C.c_uint32),
         ("sweep_workgroups", C.c_uint32),
-        ("reserved", C.c_uint32 * 1),
+        ("specialize", C.c_uint32),
     ]
 
 

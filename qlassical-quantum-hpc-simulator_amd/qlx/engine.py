@@ -34,12 +34,13 @@ DENSE = {"auto": _abi.QLX_DENSE_AUTO, "valu": _abi.QLX_DENSE_VALU, "bf16": _abi.
 
 def exec_opts(mode: str = "plain", window_qubits: int = 0, coalesce_qubits: int = 0,
               gswap: str = "permute", register_qubits: int = 0, dense: str = "auto",
-              sweep_workgroups: int = 0) -> _abi.ExecOpts:
+              sweep_workgroups: int = 0, specialize: bool = False) -> _abi.ExecOpts:
     """qlx_exec_opts. dense: arithmetic of 4/5-qubit dense gates on complex64 states
     ('auto' = 'bf16' matrix cores, 'f32' matrix cores, 'valu' vector units)."""
     o = _abi.ExecOpts()
     o.dense_kernel = DENSE[dense]
     o.sweep_workgroups = sweep_workgroups
+    o.specialize = int(specialize)
     o.mode = {"plain": _abi.QLX_EXEC_PLAIN, "windowed": _abi.QLX_EXEC_WINDOWED}[mode]
     o.window_qubits = window_qubits
     o.register_qubits = register_qubits
