@@ -102,7 +102,7 @@ typedef struct qlx_exec_opts {
  * 5-qubit gates on complex64 states; every other dense gate runs on the vector units
  * (FP64 for complex128, FP32 for complex64). The matrices stay FP64 in the pool and are
  * converted on the device. */
-#define QLX_DENSE_AUTO        0  /* complex64 4/5-qubit gates: MFMA BF16 */
+#define QLX_DENSE_AUTO        0  /* complex64 4/5-qubit gates on states of >= k+5 qubits: MFMA BF16 */
 #define QLX_DENSE_VALU        1  /* vector units only */
 #define QLX_DENSE_MFMA_BF16   2  /* BF16 split operands (matrix hi+mid+lo, state hi+lo), four
                                     products accumulated in FP32 */
@@ -263,6 +263,8 @@ int qlx_gpu_safe_qubit_limit(int device, int dtype, uint32_t* k_safe);
 #define QLX_BATCH_PER_CIRCUIT        1  /* every circuit as its own stream of one-kernel-per-gate launches */
 #define QLX_BATCH_PER_CIRCUIT_GRAPH  2  /* the same launches captured once into a hipGraph, then replayed */
 #define QLX_BATCH_LEVELS             3  /* one launch per sweep level over all circuits */
+#define QLX_BATCH_HYBRID             4  /* single-chunk circuits through the persistent launch, the
+                                           others level by level on a second stream, concurrently */
 typedef struct qlx_batch qlx_batch;
 typedef struct qlx_batch_info {
     uint64_t circuits;

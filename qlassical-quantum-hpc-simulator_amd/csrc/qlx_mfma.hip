@@ -157,25 +157,33 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
             }
     }
 
-    const uint64_t ntiles = (ngroups + 31) / 32;
+    const uint64_t ntiles = ngroups / 32;
     const uint64_t wave0 = (uint64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * kWaves;
-    for (uint64_t tile = wave0; tile < ntiles; tile += nwaves) {
-        const uint64_t g = tile * 32 + j;
-        const bool valid = g < ngroups;
-        uint64_t base = g;
+    if (wave0 >= ntiles) return;  // whole wave
+    // Two register sets alternate: the next tile's amplitudes are loaded into one while the
+    // tile in the other is multiplied and stored (a wave owns its tiles, so the reads never
+    // see its own writes). Loads and stores are unconditional (whole tiles; a wave with no
+    // next tile re-reads its current one and ignores it), so the wait before a tile covers
+    // that tile's loads only.
+    auto group_base = [&](uint64_t tile) {
+        uint64_t base = tile * 32 + j;
 #pragma unroll
         for (int q = 0; q < K; ++q) base = ins0(base, ma.sorted[q]);
-        float2 x[HALF];
+        return base;
+    };
+    auto load = [&](float2 (&x)[HALF], uint64_t base) {
 #pragma unroll
-        for (int r = 0; r < HALF; ++r) x[r] = valid ? amp[base + hin + ma.in_off[r]] : float2{0.f, 0.f};
-
+        for (int r = 0; r < HALF; ++r) x[r] = amp[base + hin + ma.in_off[r]];
+        // keep the loads here: the scheduler would otherwise sink them below the MFMAs
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    auto multiply_store = [&](const float2 (&x)[HALF], uint64_t base) {
         f32x16 acc[MT];
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[m][v] = 0.f;
-
         if constexpr (F32) {
 #pragma unroll
             for (int t = 0; t < TF; ++t) {
@@ -212,16 +220,31 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
                     acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m][t], xh[t], acc[m], 0, 0, 0);
             }
         }
-        if (valid) {
 #pragma unroll
-            for (int m = 0; m < MT; ++m)
+        for (int m = 0; m < MT; ++m)
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
+            for (int a = 0; a < 4; ++a)
 #pragma unroll
-                    for (int b = 0; b < 2; ++b)
-                        amp[base + hout + ma.out_off[m * 8 + a * 2 + b]] =
-                            float2{acc[m][4 * a + 2 * b], acc[m][4 * a + 2 * b + 1]};
-        }
+                for (int b = 0; b < 2; ++b)
+                    amp[base + hout + ma.out_off[m * 8 + a * 2 + b]] =
+                        float2{acc[m][4 * a + 2 * b], acc[m][4 * a + 2 * b + 1]};
+    };
+    float2 xa[HALF], xb[HALF];
+    uint64_t tile = wave0;
+    uint64_t ba = group_base(tile);
+    load(xa, ba);
+    for (;;) {
+        const uint64_t t1 = tile + nwaves;
+        const uint64_t bb = group_base(t1 < ntiles ? t1 : tile);
+        load(xb, bb);
+        multiply_store(xa, ba);
+        if (t1 >= ntiles) break;
+        const uint64_t t2 = t1 + nwaves;
+        ba = group_base(t2 < ntiles ? t2 : t1);
+        load(xa, ba);
+        multiply_store(xb, bb);
+        if (t2 >= ntiles) break;
+        tile = t2;
     }
 }
 
@@ -258,7 +281,7 @@ void mfma_split(const double* d_pool, const MfmaSplitJob* jobs, size_t n, void* 
 void launch_dense_mfma(qlx_state* st, const void* d_frag, const uint8_t* qubits, int k, int variant) {
     if (st->dtype != QLX_C64) fail(QLX_EUNSUPPORTED, "MFMA dense gates run on complex64 states");
     if (k != 4 && k != 5) fail(QLX_EUNSUPPORTED, "MFMA dense gates take 4 or 5 qubits");
-    if ((uint32_t)k > st->nq) fail(QLX_ERANGE, "gate arity exceeds the qubit count");
+    if ((uint32_t)k + 5 > st->nq) fail(QLX_EUNSUPPORTED, "MFMA dense gates need at least 32 groups (k + 5 qubits)");
     MfmaArgs ma{};
     auto off = [&](int loc) {
         uint32_t o = 0;
@@ -311,7 +334,8 @@ void prepare_dense(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const 
         if (g.type != QLX_UNITARY && g.type != QLX_FUSED_BLOCK) continue;
         const int k = instr_arity(g);
         const uint32_t mi = g.payload.matrix_idx;
-        if ((k != 4 && k != 5) || (uint32_t)k > st->nq || mi >= n_meta || meta[mi].num_qubits != k) continue;
+        // states with fewer than 32 groups of 2^k amplitudes keep the vector-unit kernel
+        if ((k != 4 && k != 5) || (uint32_t)k + 5 > st->nq || mi >= n_meta || meta[mi].num_qubits != k) continue;
         if (st->frag_index.empty()) st->frag_index.assign(n_meta, -1);
         if (st->frag_index[mi] >= 0) continue;
         st->frag_index[mi] = (int64_t)total;

@@ -12,7 +12,8 @@ from .engine import State, _dt, exec_opts
 from .ir import MT19937_64, Program, layered
 
 MODES = {"persistent": _abi.QLX_BATCH_PERSISTENT, "per_circuit": _abi.QLX_BATCH_PER_CIRCUIT,
-         "per_circuit_graph": _abi.QLX_BATCH_PER_CIRCUIT_GRAPH, "levels": _abi.QLX_BATCH_LEVELS}
+         "per_circuit_graph": _abi.QLX_BATCH_PER_CIRCUIT_GRAPH, "levels": _abi.QLX_BATCH_LEVELS,
+         "hybrid": _abi.QLX_BATCH_HYBRID}
 
 
 def batch_workload(count: int = 4096, qubits=(10, 20), depths=(5, 40), shape_seed: int = 4004,
@@ -84,3 +85,50 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---- autotuning -------------------------------------------------------------------------
+# Candidate (fusion width, window qubits) pairs for complex64 batches. Fusion width 0 keeps
+# the circuits as they are; width 2 merges runs of 1- and 2-qubit gates into 4x4 matrices
+# (fewer, heavier in-register ops: the sweep kernel's per-op dispatch cost falls, its
+# arithmetic per op doubles). The window sets the chunk of circuits wider than it.
+TUNE_SPACE = [(0, 13), (2, 13), (0, 12), (2, 12), (0, 11), (2, 11)]
+
+
+def prepare_programs(programs: Iterable[Program], fusion_width: int) -> List[Program]:
+    """The circuits as the batch runs them: fused to the given width (0: unchanged)."""
+    from .ir import transpile_gpu
+    if not fusion_width:
+        return list(programs)
+    return [transpile_gpu(p, fusion=True, max_width=fusion_width, rotations=True) for p in programs]
+
+
+def autotune(programs: List[Program], dtype="c64", sample: int = 256, reps: int = 3, seed: int = 7,
+             space=TUNE_SPACE, mode: str = "persistent") -> dict:
+    """Picks the fusion width and window for a batch by timing a random sample of its circuits
+    in every candidate configuration (median of `reps` runs of one persistent launch).
+    Returns {"fusion_width", "window_qubits", "trials": [(fw, w, ms), ...]}."""
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(len(programs), size=min(sample, len(programs)), replace=False)
+    base = [programs[i] for i in sorted(idx)]
+    trials = []
+    for fw, w in space:
+        progs = prepare_programs(base, fw)
+        try:
+            b = Batch(progs, dtype, window_qubits=w)
+        except _abi.QlxUnsupported:
+            continue
+        try:
+            b.run(mode)
+            b.sync()
+            times = []
+            for _ in range(reps):
+                b.run(mode)
+                times.append(b.sync())
+            trials.append((fw, w, float(np.median(times))))
+        finally:
+            b.close()
+    if not trials:
+        raise _abi.QlxUnsupported("no batch configuration runs these circuits")
+    fw, w, _ = min(trials, key=lambda t: t[2])
+    return {"fusion_width": fw, "window_qubits": w, "trials": trials}

@@ -23,7 +23,7 @@ def workload(qlx, count, qubits, depths, seed):
     return [qlx.layered(n, d, s) for n, d, s in qlx.batch_workload(count, qubits, depths, seed, 410000)]
 
 
-@pytest.mark.parametrize("mode", ["persistent", "levels", "per_circuit", "per_circuit_graph"])
+@pytest.mark.parametrize("mode", ["persistent", "hybrid", "levels", "per_circuit", "per_circuit_graph"])
 def test_batch_c64_matches_c128(gpu, mode):
     progs = workload(gpu, 24, (10, 16), (3, 12), 4321)
     b = gpu.Batch(progs, "c64")
@@ -89,3 +89,23 @@ def test_batch_rejects_unsupported(gpu):
     c = gpu.Circuit(12).h(0).ccx(0, 1, 2).release()
     with pytest.raises(gpu.QlxUnsupported):
         gpu.Batch([c], "c64")
+
+
+def test_batch_autotune_and_fused_batch(gpu):
+    """The autotuner returns a configuration it measured; a batch of width-2-fused circuits
+    gives the states of the unfused circuits."""
+    from qlx import batch as qb
+    progs = workload(gpu, 20, (10, 15), (4, 10), 31)
+    tune = qb.autotune(progs, "c64", sample=8, reps=1)
+    assert (tune["fusion_width"], tune["window_qubits"]) in [(fw, w) for fw, w, _ in tune["trials"]]
+    fused = qb.prepare_programs(progs, 2)
+    assert any((p.meta["num_qubits"] == 2).any() for p in fused if len(p.meta))
+    b = gpu.Batch(fused, "c64", window_qubits=12)
+    b.run("hybrid")
+    b.sync()
+    for i, p in enumerate(progs):
+        ref = ref_state(gpu, p)
+        inf, mx = b.compare(i, ref)
+        ref.close()
+        assert inf <= 1e-6 and mx <= 1e-5, (i, inf, mx)
+    b.close()

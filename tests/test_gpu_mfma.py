@@ -60,9 +60,11 @@ def run_c128(qlx, prog, psi):
 
 
 CASES = [
-    (5, (0, 1, 2, 3, 4)),        # one group: 31 of 32 lanes idle
+    (5, (0, 1, 2, 3, 4)),        # one group: vector-unit kernel
+    (10, (0, 1, 2, 3, 4)),       # exactly one 32-group tile
     (6, (5, 0, 3, 1, 4)),
-    (9, (8, 7, 6, 5, 4)),        # descending, partial tile
+    (9, (8, 7, 6, 5, 4)),        # descending, 16 groups: vector-unit kernel
+    (11, (10, 9, 8, 7, 6)),      # descending, two tiles
     (12, (2, 9, 0, 11, 5)),
     (14, (13, 0, 7, 1, 12)),
     (16, (3, 4, 5, 6, 7)),
@@ -81,7 +83,10 @@ def test_dense_gate_matches_c128(gpu, dense, n, qubits):
     prog = gpu.Circuit(n).unitary(list(qubits), u, max_arity=5).release()
     psi = rand_state(rng, n)
     got, t = run_c64(gpu, prog, psi, dense)
-    assert "k_dense_mfma" in t and "k_dense" not in t, t
+    if n >= k + 5:
+        assert "k_dense_mfma" in t and "k_dense" not in t, t
+    else:  # fewer than 32 groups: the vector-unit kernel
+        assert "k_dense" in t and "k_dense_mfma" not in t, t
     want = run_c128(gpu, prog, psi)
     # independent check of the c128 path: numpy brute force at small n
     if n <= 10:
