@@ -98,7 +98,7 @@ def run_precision(args):
     variants = [v for v in args.dense_variants.split(",") if v]
     def opts_of(v):
         mode, dense = v.split(":")
-        return qlx.exec_opts(mode, dense=dense)
+        return qlx.exec_opts(mode, dense=dense, specialize=(mode == "windowed" and not args.generic))
     res = {}
     st = qlx.State(nq, "c64")
     for v in variants:

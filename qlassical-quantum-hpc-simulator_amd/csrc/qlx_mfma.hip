@@ -33,6 +33,7 @@
 // group j, real and imaginary parts interleaved as in memory.
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "qlx_internal.hpp"
 
@@ -123,7 +124,9 @@ struct MfmaArgs {
     uint32_t hbit_out;    // 1 << qubits[1]:   lane half 1 writes locals with bit 1 set
 };
 
-template <int K, bool F32>
+// ALDS: the A fragments stay in LDS (read per tile) instead of VGPRs: fewer registers per
+// wave, more waves per SIMD and more HBM loads in flight.
+template <int K, bool F32, bool ALDS>
 __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp, uint64_t ngroups,
                                                       const unsigned char* __restrict__ frags, MfmaArgs ma) {
     constexpr int D = 1 << K;          // amplitudes per group
@@ -135,9 +138,24 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
 
     constexpr int TB = (2 * D) / 16;   // BF16 K-steps
     constexpr int TF = D;              // F32 K-steps
-    bf16x8 ah[F32 ? 1 : MT][F32 ? 1 : TB], am[F32 ? 1 : MT][F32 ? 1 : TB], al[F32 ? 1 : MT][F32 ? 1 : TB];
-    float af[F32 ? MT : 1][F32 ? TF : 1];
-    if constexpr (F32) {
+    constexpr int NB = MT * TB * 64;   // bf16x8 fragments per split term
+    constexpr int NF = MT * TF * 64;   // float fragments
+    constexpr int RA = ALDS ? 1 : MT;  // register copies
+    bf16x8 ah[F32 ? 1 : RA][F32 ? 1 : TB], am[F32 ? 1 : RA][F32 ? 1 : TB], al[F32 ? 1 : RA][F32 ? 1 : TB];
+    float af[F32 ? RA : 1][F32 ? TF : 1];
+    __shared__ __align__(16) unsigned char sA[ALDS ? (F32 ? NF * 4 : 3 * NB * 16) : 16];
+    if constexpr (ALDS) {
+        if constexpr (F32) {
+            const float* f = (const float*)frags;
+            float* d = (float*)sA;
+            for (int e = threadIdx.x; e < NF; e += kBlock) d[e] = f[e];
+        } else {
+            const bf16x8* f = (const bf16x8*)frags;
+            bf16x8* d = (bf16x8*)sA;
+            for (int e = threadIdx.x; e < 3 * NB; e += kBlock) d[e] = f[e];
+        }
+        __syncthreads();
+    } else if constexpr (F32) {
         const float* f = (const float*)frags;
 #pragma unroll
         for (int m = 0; m < MT; ++m)
@@ -145,8 +163,8 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
             for (int t = 0; t < TF; ++t) af[m][t] = f[(m * TF + t) * 64 + lane];
     } else {
         const bf16x8* fh = (const bf16x8*)frags;
-        const bf16x8* fm = fh + MT * TB * 64;
-        const bf16x8* fl = fm + MT * TB * 64;
+        const bf16x8* fm = fh + NB;
+        const bf16x8* fl = fm + NB;
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -156,6 +174,15 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
                 al[m][t] = fl[(m * TB + t) * 64 + lane];
             }
     }
+    // fragment accessors (term: 0 hi, 1 mid, 2 lo)
+    auto fragb = [&](int term, int m, int t) -> bf16x8 {
+        if constexpr (ALDS) return ((const bf16x8*)sA)[term * NB + (m * TB + t) * 64 + lane];
+        else return term == 0 ? ah[m][t] : (term == 1 ? am[m][t] : al[m][t]);
+    };
+    auto fragf = [&](int m, int t) -> float {
+        if constexpr (ALDS) return ((const float*)sA)[(m * TF + t) * 64 + lane];
+        else return af[m][t];
+    };
 
     const uint64_t ntiles = ngroups / 32;
     const uint64_t wave0 = (uint64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
@@ -190,7 +217,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
                 const float xv = (t & 1) ? x[t >> 1].y : x[t >> 1].x;
 #pragma unroll
                 for (int m = 0; m < MT; ++m)
-                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[m][t], xv, acc[m], 0, 0, 0);
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(fragf(m, t), xv, acc[m], 0, 0, 0);
             }
         } else {
             bf16x8 xh[TB], xl[TB];
@@ -208,16 +235,16 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
             for (int m = 0; m < MT; ++m) {
 #pragma unroll
                 for (int t = 0; t < TB; ++t)
-                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[m][t], xh[t], acc[m], 0, 0, 0);
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fragb(2, m, t), xh[t], acc[m], 0, 0, 0);
 #pragma unroll
                 for (int t = 0; t < TB; ++t)
-                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am[m][t], xh[t], acc[m], 0, 0, 0);
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fragb(1, m, t), xh[t], acc[m], 0, 0, 0);
 #pragma unroll
                 for (int t = 0; t < TB; ++t)
-                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m][t], xl[t], acc[m], 0, 0, 0);
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fragb(0, m, t), xl[t], acc[m], 0, 0, 0);
 #pragma unroll
                 for (int t = 0; t < TB; ++t)
-                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m][t], xh[t], acc[m], 0, 0, 0);
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fragb(0, m, t), xh[t], acc[m], 0, 0, 0);
             }
         }
 #pragma unroll
@@ -310,12 +337,24 @@ void launch_dense_mfma(qlx_state* st, const void* d_frag, const uint8_t* qubits,
     float2* amp = (float2*)st->amp;
     const auto* f = (const unsigned char*)d_frag;
     const bool f32 = variant == QLX_DENSE_MFMA_F32;
-    if (k == 5) {
-        if (f32) k_dense_mfma<5, true><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
-        else k_dense_mfma<5, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+    static const bool a_in_lds = [] {
+        const char* e = std::getenv("QLX_MFMA_A_IN_LDS");
+        return e && e[0] == '1';
+    }();
+    if (a_in_lds) {
+        if (k == 5) {
+            if (f32) k_dense_mfma<5, true, true><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+            else k_dense_mfma<5, false, true><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+        } else {
+            if (f32) k_dense_mfma<4, true, true><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+            else k_dense_mfma<4, false, true><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+        }
+    } else if (k == 5) {
+        if (f32) k_dense_mfma<5, true, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+        else k_dense_mfma<5, false, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
     } else {
-        if (f32) k_dense_mfma<4, true><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
-        else k_dense_mfma<4, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+        if (f32) k_dense_mfma<4, true, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+        else k_dense_mfma<4, false, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
     }
     QLX_HIP_CHECK(hipGetLastError());
 }
