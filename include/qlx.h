@@ -85,7 +85,8 @@ enum {
 typedef struct qlx_exec_opts {
     uint32_t mode;            /* QLX_EXEC_PLAIN | QLX_EXEC_WINDOWED */
     uint32_t window_qubits;   /* 0 = default for the precision (c128: 12, c64: 13) */
-    uint32_t coalesce_qubits; /* low qubits always kept in the window; 0 = default (3) */
+    uint32_t coalesce_qubits; /* low qubits always kept in the window; 0 = default (c128: 3, c64: 4:
+                                 one 128-byte cache line) */
     uint32_t gswap_mode;      /* QLX_GSWAP_PERMUTE | QLX_GSWAP_NOOP */
     uint32_t register_qubits; /* qubits per work-item's registers in a window; 0 = default
                                  (c128: 4, c64: 5). Built (window, register) pairs:
@@ -197,6 +198,13 @@ int qlx_plan_windowed(uint32_t num_qubits, int dtype, const qlx_gate_instr* inst
                       const double* pool_c128, size_t pool_elems,
                       const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts* opts,
                       int64_t* order, int32_t* sweep_of, size_t cap, size_t* n_out, qlx_plan_info* info);
+
+/* Host-only: the HIP source that QLX_EXEC_WINDOWED with opts->specialize compiles for sweep
+ * `sweep_index` of this stream (for inspection and tests). *len = source length; `out`
+ * (cap bytes) receives it NUL-terminated, truncated to cap - 1. */
+int qlx_plan_sweep_source(uint32_t num_qubits, int dtype, const qlx_gate_instr* instrs, size_t n,
+                          const double* pool_c128, size_t pool_elems, const qlx_matrix_entry* meta, size_t n_meta,
+                          const qlx_exec_opts* opts, uint32_t sweep_index, char* out, size_t cap, size_t* len);
 
 /* Sweep accounting of the last qlx_execute on this state. */
 int qlx_last_exec_stats(const qlx_state* st, qlx_exec_stats* out);

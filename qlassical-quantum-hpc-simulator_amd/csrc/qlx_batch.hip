@@ -180,7 +180,7 @@ Geometry batch_geometry(int dtype, uint32_t nq, const qlx_exec_opts& o) {
     g.logt = 8;
     g.w = std::min<int>((int)nq, wmax);
     g.rb = g.w - g.logt;
-    g.c = 3;
+    g.c = dtype == QLX_C64 ? 4 : 3;  // one 128-byte cache line of amplitudes
     const int rbmax = dtype == QLX_C128 ? 4 : 5;
     if (g.rb < 2) fail(QLX_EUNSUPPORTED, "batched circuits need at least 10 qubits");
     if (g.rb > rbmax) fail(QLX_EUNSUPPORTED, "window too large for the batched kernel");

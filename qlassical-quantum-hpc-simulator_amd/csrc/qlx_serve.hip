@@ -359,7 +359,7 @@ int qlx_server_submit(qlx_server* s, uint32_t nq, const qlx_gate_instr* instrs, 
         geo.logt = 8;
         geo.w = std::min<int>((int)nq, s->dtype == QLX_C64 ? 13 : 12);
         geo.rb = geo.w - 8;
-        geo.c = 3;
+        geo.c = s->dtype == QLX_C64 ? 4 : 3;  // one 128-byte cache line of amplitudes
         r.plan = build_plan_geo(nq, geo, r.instrs.data(), n, nullptr, nullptr, o);
         const SweepPlan& p = *r.plan;
         r.n_sweeps = (uint32_t)p.dsw.size();

@@ -89,7 +89,8 @@ static Geometry geometry_for(int dtype, uint32_t nq, const qlx_exec_opts& o) {
         fail(QLX_EUNSUPPORTED, "window " + std::to_string(g.w) + " with " + std::to_string(g.rb) +
                                    " register qubits is not built for this precision");
     g.logt = g.w - g.rb;
-    g.c = o.coalesce_qubits ? (int)o.coalesce_qubits : 3;
+    // low qubits kept in the window: one 128-byte cache line of amplitudes (8 x 16 B, 16 x 8 B)
+    g.c = o.coalesce_qubits ? (int)o.coalesce_qubits : (dtype == QLX_C64 ? 4 : 3);
     if (g.c < 0 || g.c > g.logt) fail(QLX_EINVAL, "coalesce_qubits out of range");
     (void)nq;
     return g;
@@ -679,6 +680,31 @@ extern "C" int qlx_plan_windowed(uint32_t num_qubits, int dtype, const qlx_gate_
                 info->plain_steps = p->steps.size() - p->sweeps.size();
                 info->gates_in_sweeps = p->gates_in_sweeps;
             }
+        }
+    });
+}
+
+extern "C" int qlx_plan_sweep_source(uint32_t num_qubits, int dtype, const qlx_gate_instr* instrs, size_t n,
+                                     const double* pool, size_t pool_elems, const qlx_matrix_entry* meta,
+                                     size_t n_meta, const qlx_exec_opts* opts, uint32_t sweep_index, char* out,
+                                     size_t cap, size_t* len) {
+    return guard([&] {
+        if (n && !instrs) fail(QLX_EINVAL, "null instruction stream");
+        qlx_exec_opts o{};
+        if (opts) o = *opts;
+        for (size_t i = 0; i < n; ++i) {
+            if (instrs[i].type == QLX_MEASURE) fail(QLX_EINVAL, "plans cover MEASURE-free streams");
+            validate_instr(instrs[i], num_qubits, meta, n_meta, pool_elems);
+        }
+        const Geometry geo = geometry_for(dtype, num_qubits, o);
+        if ((int)num_qubits < geo.w) fail(QLX_EINVAL, "the state is smaller than one window: no sweeps");
+        const auto p = build_plan(num_qubits, dtype, instrs, n, pool, meta, o);
+        const std::string src = jit_sweep_source(*p, (int)sweep_index, dtype);
+        if (len) *len = src.size();
+        if (out && cap) {
+            const size_t k = std::min(cap - 1, src.size());
+            std::memcpy(out, src.data(), k);
+            out[k] = 0;
         }
     });
 }

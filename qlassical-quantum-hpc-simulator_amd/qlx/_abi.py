@@ -177,6 +177,9 @@ SIGNATURES = [
     ("qlx_plan_windowed", [C.c_uint32, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,
                            C.c_size_t, C.POINTER(ExecOpts), C.c_void_p, C.c_void_p, C.c_size_t,
                            C.POINTER(C.c_size_t), C.POINTER(PlanInfo)]),
+    ("qlx_plan_sweep_source", [C.c_uint32, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,
+                               C.c_size_t, C.POINTER(ExecOpts), C.c_uint32, C.c_char_p, C.c_size_t,
+                               C.POINTER(C.c_size_t)]),
     ("qlx_last_exec_stats", [C.c_void_p, C.POINTER(ExecStats)]),
     ("qlx_measure_prob1", [C.c_void_p, C.c_uint32, C.POINTER(C.c_double)]),
     ("qlx_collapse", [C.c_void_p, C.c_uint32, C.c_int, C.c_double]),

@@ -69,6 +69,23 @@ def plan_windowed(prog: Program, dtype="c128", opts: Optional[_abi.ExecOpts] = N
     return order, sweep, {k: getattr(info, k) for k, _ in info._fields_}
 
 
+def sweep_source(prog: Program, sweep_index: int, dtype="c128", opts: Optional[_abi.ExecOpts] = None) -> str:
+    """HIP source of one circuit-specialised sweep (host-only; see qlx_plan_sweep_source)."""
+    L = _abi.lib()
+    o = opts if opts is not None else exec_opts("windowed", specialize=True)
+    ins = np.ascontiguousarray(prog.instrs)
+    pool = np.ascontiguousarray(prog.pool, np.complex128)
+    meta = np.ascontiguousarray(prog.meta)
+    args = (prog.num_qubits, _dt(dtype), ins.ctypes.data if len(ins) else None, len(ins),
+            pool.ctypes.data if pool.size else None, pool.size, meta.ctypes.data if len(meta) else None, len(meta),
+            C.byref(o), sweep_index)
+    n = C.c_size_t(0)
+    _abi.check(L.qlx_plan_sweep_source(*args, None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    _abi.check(L.qlx_plan_sweep_source(*args, buf, n.value + 1, C.byref(n)))
+    return buf.value.decode()
+
+
 def device_info(device: int = 0) -> dict:
     info = _abi.DeviceInfo()
     _abi.check(_abi.lib().qlx_device_query(device, C.byref(info)))

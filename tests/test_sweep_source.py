@@ -1,0 +1,77 @@
+"""Circuit-specialised sweep sources (qlx_plan_sweep_source): what the windowed executor compiles
+with hipRTC when qlx_exec_opts::specialize is set. CPU-only: the generated source is checked for
+its structure (one load and one store per register, constants in place of descriptors, CX and
+SWAP as register renaming) and compiled for gfx950 with hipRTC, which needs no GPU."""
+import ctypes as C
+import ctypes.util
+import os
+import re
+
+import numpy as np
+import pytest
+
+import qlx
+
+HIPRTC = "/opt/rocm/lib/libhiprtc.so"
+
+
+def compile_ok(src):
+    rtc = C.CDLL(HIPRTC)
+    prog = C.c_void_p()
+    assert rtc.hiprtcCreateProgram(C.byref(prog), src.encode(), b"s.hip", 0, None, None) == 0
+    opts = (C.c_char_p * 6)(b"--offload-arch=gfx950", b"-O3", b"-Xclang", b"-target-feature", b"-Xclang",
+                           b"-packed-fp32-ops")
+    rc = rtc.hiprtcCompileProgram(prog, 6, opts)
+    log = b""
+    if rc:
+        n = C.c_size_t()
+        rtc.hiprtcGetProgramLogSize(prog, C.byref(n))
+        buf = C.create_string_buffer(n.value + 1)
+        rtc.hiprtcGetProgramLog(prog, buf)
+        log = buf.value
+    rtc.hiprtcDestroyProgram(C.byref(prog))
+    return rc, log
+
+
+@pytest.mark.parametrize("dtype,regs", [("c128", 16), ("c64", 32)])
+def test_source_structure(dtype, regs):
+    prog = qlx.layered(14, 4, 7)
+    order, sweep_of, info = qlx.plan_windowed(prog, dtype, qlx.exec_opts("windowed"))
+    assert info["sweeps"] >= 1
+    for s in range(info["sweeps"]):
+        src = qlx.sweep_source(prog, s, dtype)
+        assert f"qlx_sweep_spec_{s}(" in src
+        assert len(re.findall(r"= amp\[lo \| 0x[0-9a-f]+ull\];", src)) == regs
+        assert len(re.findall(r"amp\[so \| 0x[0-9a-f]+ull\] = r\d+;", src)) == regs
+        # gate coefficients are literals; no descriptor arrays are read
+        assert "coef[" not in src and "ops[" not in src
+        # every remap writes and reads all registers through LDS
+        assert src.count("lds[s0 ^") == regs * src.count("__syncthreads();") // 2
+
+
+def test_cx_and_swap_are_register_renaming():
+    c = qlx.Circuit(13).cx(0, 1).swap(1, 2).cx(2, 0)
+    prog = c.release()
+    src = qlx.sweep_source(prog, 0, "c128")
+    assert "const double2 a0" not in src          # no arithmetic at all
+    stores = re.findall(r"amp\[so \| (0x[0-9a-f]+)ull\] = r(\d+);", src)
+    assert sorted(int(v) for _, v in stores) == list(range(16))
+    # the renamed registers show up where the chunk leaves the registers (an LDS re-layout or
+    # the final store): some register index is written from another register's variable
+    writes = [int(v) for v in re.findall(r"lds\[s0 \^ \d+u\] = r(\d+);", src)] + [int(v) for _, v in stores]
+    assert any(v != i % 16 for i, v in enumerate(writes))
+
+
+@pytest.mark.skipif(not os.path.exists(HIPRTC), reason="hipRTC not installed")
+def test_sources_compile_for_gfx950():
+    from test_oracle import random_program
+    rng = np.random.default_rng(11)
+    ins, pool, meta = random_program(13, 40, rng)
+    rnd = qlx.Program(13, ins, pool if pool is not None else np.zeros(0, complex),
+                      meta if meta is not None else np.zeros(0, qlx.META_DTYPE))
+    fused = qlx.transpile_gpu(qlx.layered(13, 3, 5), max_width=2, rotations=True)   # U4 ops
+    for prog, dtype in [(rnd, "c128"), (fused, "c64"), (qlx.hea(13, 2), "c128")]:
+        _, _, info = qlx.plan_windowed(prog, dtype, qlx.exec_opts("windowed"))
+        for s in range(min(2, info["sweeps"])):
+            rc, log = compile_ok(qlx.sweep_source(prog, s, dtype))
+            assert rc == 0, log[-2000:]
