@@ -166,7 +166,7 @@ def run_precision(args):
 def run_batch(args):
     """4096 independent layered circuits (N ~ U[10,20], D ~ U[5,40], mt19937_64(4004) shapes,
     seeds 400000+i), complex64. The fusion width and window are autotuned on a sample of the
-    circuits; then the batch runs in each mode (persistent launch, hybrid, per-level launches)
+    circuits; then the batch runs in each mode (persistent launch, per-level launches)
     and as per-circuit plain launches (the baseline). Every state is checked against the
     complex128 plain path of its unfused circuit."""
     import numpy as np
@@ -287,7 +287,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--batch-circuits", type=int, default=4096)
-    ap.add_argument("--batch-modes", default="persistent,hybrid,levels,per_circuit")
+    ap.add_argument("--batch-modes", default="persistent,levels,per_circuit")
     ap.add_argument("--batch-check-sample", action="store_true",
                     help="check every 16th state against complex128 instead of all of them")
     ap.add_argument("--calib-copy", action="store_true",

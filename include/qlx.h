@@ -271,8 +271,6 @@ int qlx_gpu_safe_qubit_limit(int device, int dtype, uint32_t* k_safe);
 #define QLX_BATCH_PER_CIRCUIT        1  /* every circuit as its own stream of one-kernel-per-gate launches */
 #define QLX_BATCH_PER_CIRCUIT_GRAPH  2  /* the same launches captured once into a hipGraph, then replayed */
 #define QLX_BATCH_LEVELS             3  /* one launch per sweep level over all circuits */
-#define QLX_BATCH_HYBRID             4  /* single-chunk circuits through the persistent launch, the
-                                           others level by level on a second stream, concurrently */
 typedef struct qlx_batch qlx_batch;
 typedef struct qlx_batch_info {
     uint64_t circuits;

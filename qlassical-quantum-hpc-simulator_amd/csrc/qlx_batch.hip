@@ -13,9 +13,6 @@
 // another; the launch ends when the queue is empty.
 // QLX_BATCH_LEVELS: the (circuit, sweep, chunk) tasks, one launch per sweep level
 // (kernel boundaries order the levels; chunks of a level run in parallel).
-// QLX_BATCH_HYBRID: circuits that fit one chunk go through the persistent queue while the
-// others run level by level on a second stream, concurrently; the persistent grid gets
-// the single-chunk circuits' share of the work's resident slots.
 // QLX_BATCH_PER_CIRCUIT(_GRAPH): the baseline — each circuit's gates as
 // one-kernel-per-gate launches (optionally captured once into a hipGraph).
 
@@ -129,14 +126,6 @@ struct qlx_batch {
     std::vector<BatchTaskDev> tasks;
     std::vector<uint32_t> queue;  // circuits in persistent-queue order
     std::vector<uint32_t> level_first, level_count;
-    // hybrid mode: single-chunk circuits (queue order) and the level tasks of the others
-    std::vector<uint32_t> queue_small;
-    std::vector<BatchTaskDev> tasks_big;
-    std::vector<uint32_t> big_first, big_count;
-    size_t off_queue_small = 0, off_task_big = 0;
-    uint32_t grid_small = 0;
-    hipStream_t stream2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint32_t grid = 0;
     size_t lds_bytes = 0;
     qlx_batch_info info{};
@@ -166,10 +155,6 @@ void destroy_batch(qlx_batch* b) {
     }
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
-    if (b->stream2) (void)hipStreamSynchronize(b->stream2);
-    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
-    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
-    if (b->stream2) (void)hipStreamDestroy(b->stream2);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -201,32 +186,6 @@ void launch_persistent(qlx_batch* b) {
     if (b->dtype == QLX_C64) go((float2*)b->arena.ptr);
     else go((double2*)b->arena.ptr);
     QLX_HIP_CHECK(hipGetLastError());
-}
-
-void launch_hybrid(qlx_batch* b) {
-    const char* d = (const char*)b->desc.ptr;
-    QLX_HIP_CHECK(hipEventRecord(b->ev_fork, b->stream));
-    QLX_HIP_CHECK(hipStreamWaitEvent(b->stream2, b->ev_fork, 0));
-    auto go = [&](auto* arena) {
-        using A = std::remove_pointer_t<decltype(arena)>;
-        if (!b->queue_small.empty()) {
-            k_zero_u32<<<1, 64, 0, b->stream>>>((uint32_t*)b->ctl.ptr, 1u);
-            k_batch<A><<<b->grid_small, 256, b->lds_bytes, b->stream>>>(
-                arena, (const BatchCircuitDev*)(d + b->off_circ), (const uint32_t*)(d + b->off_queue_small),
-                (uint32_t)b->queue_small.size(), (const DevSweep*)(d + b->off_sw), (const DevPhase*)(d + b->off_ph),
-                (const DevOp*)(d + b->off_op), (const double*)(d + b->off_cf), (uint32_t*)b->ctl.ptr);
-        }
-        for (size_t L = 0; L < b->big_first.size(); ++L)
-            k_batch_level<A><<<b->big_count[L], 256, b->lds_bytes, b->stream2>>>(
-                arena, (const BatchCircuitDev*)(d + b->off_circ), (const BatchTaskDev*)(d + b->off_task_big),
-                b->big_first[L], (const DevSweep*)(d + b->off_sw), (const DevPhase*)(d + b->off_ph),
-                (const DevOp*)(d + b->off_op), (const double*)(d + b->off_cf));
-    };
-    if (b->dtype == QLX_C64) go((float2*)b->arena.ptr);
-    else go((double2*)b->arena.ptr);
-    QLX_HIP_CHECK(hipGetLastError());
-    QLX_HIP_CHECK(hipEventRecord(b->ev_join, b->stream2));
-    QLX_HIP_CHECK(hipStreamWaitEvent(b->stream, b->ev_join, 0));
 }
 
 void launch_levels(qlx_batch* b) {
@@ -271,9 +230,6 @@ int qlx_batch_create(int dtype, qlx_batch** out) {
             QLX_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
             QLX_HIP_CHECK(hipEventCreate(&b->ev0));
             QLX_HIP_CHECK(hipEventCreate(&b->ev1));
-            QLX_HIP_CHECK(hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking));
-            QLX_HIP_CHECK(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
-            QLX_HIP_CHECK(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
         } catch (...) {
             destroy_batch(b);
             throw;
@@ -377,39 +333,11 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
         for (size_t ci = 0; ci < cdev.size(); ++ci) b->queue[ci] = (uint32_t)ci;
         auto work = [&](uint32_t ci) { return (uint64_t)cdev[ci].n_sweeps << cdev[ci].log_chunks; };
         std::stable_sort(b->queue.begin(), b->queue.end(), [&](uint32_t x, uint32_t y) { return work(x) > work(y); });
-        // hybrid split
-        b->queue_small.clear();
-        b->tasks_big.clear();
-        b->big_first.clear();
-        b->big_count.clear();
-        uint64_t work_small = 0, work_big = 0;
-        for (uint32_t ci : b->queue)
-            if (cdev[ci].log_chunks == 0) {
-                b->queue_small.push_back(ci);
-                work_small += work(ci);
-            } else {
-                work_big += work(ci);
-            }
-        for (uint32_t L = 0; L < max_sweeps; ++L) {
-            const uint32_t first = (uint32_t)b->tasks_big.size();
-            for (size_t ci = 0; ci < cdev.size(); ++ci) {
-                const BatchCircuitDev& dd = cdev[ci];
-                if (dd.log_chunks == 0 || L >= dd.n_sweeps) continue;
-                for (uint32_t k = 0; k < (1u << dd.log_chunks); ++k)
-                    b->tasks_big.push_back(BatchTaskDev{(uint32_t)ci, L, k, 0});
-            }
-            if ((uint32_t)b->tasks_big.size() > first) {
-                b->big_first.push_back(first);
-                b->big_count.push_back((uint32_t)b->tasks_big.size() - first);
-            }
-        }
         // descriptor image
         b->off_circ = 0;
         b->off_task = align_up(cdev.size() * sizeof(BatchCircuitDev), 256);
         b->off_queue = align_up(b->off_task + b->tasks.size() * sizeof(BatchTaskDev), 256);
-        b->off_queue_small = align_up(b->off_queue + b->queue.size() * sizeof(uint32_t), 256);
-        b->off_task_big = align_up(b->off_queue_small + b->queue_small.size() * sizeof(uint32_t), 256);
-        b->off_sw = align_up(b->off_task_big + b->tasks_big.size() * sizeof(BatchTaskDev), 256);
+        b->off_sw = align_up(b->off_queue + b->queue.size() * sizeof(uint32_t), 256);
         b->off_ph = align_up(b->off_sw + dsw.size() * sizeof(DevSweep), 256);
         b->off_op = align_up(b->off_ph + dph.size() * sizeof(DevPhase), 256);
         b->off_cf = align_up(b->off_op + dop.size() * sizeof(DevOp), 256);
@@ -418,10 +346,6 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
         std::memcpy(img.data() + b->off_circ, cdev.data(), cdev.size() * sizeof(BatchCircuitDev));
         std::memcpy(img.data() + b->off_task, b->tasks.data(), b->tasks.size() * sizeof(BatchTaskDev));
         std::memcpy(img.data() + b->off_queue, b->queue.data(), b->queue.size() * sizeof(uint32_t));
-        if (!b->queue_small.empty())
-            std::memcpy(img.data() + b->off_queue_small, b->queue_small.data(), b->queue_small.size() * sizeof(uint32_t));
-        if (!b->tasks_big.empty())
-            std::memcpy(img.data() + b->off_task_big, b->tasks_big.data(), b->tasks_big.size() * sizeof(BatchTaskDev));
         std::memcpy(img.data() + b->off_sw, dsw.data(), dsw.size() * sizeof(DevSweep));
         std::memcpy(img.data() + b->off_ph, dph.data(), dph.size() * sizeof(DevPhase));
         if (!dop.empty()) std::memcpy(img.data() + b->off_op, dop.data(), dop.size() * sizeof(DevOp));
@@ -460,12 +384,7 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
         QLX_HIP_CHECK(hipGetDevice(&dev));
         QLX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         b->grid = (uint32_t)std::max(1, per_cu) * (uint32_t)cus;
-        const uint32_t slots = b->grid;
-        b->grid = std::min<uint32_t>(slots, (uint32_t)b->queue.size());
-        // hybrid: the persistent grid takes the single-chunk circuits' share of the slots
-        const double share = (work_small + work_big) ? (double)work_small / (double)(work_small + work_big) : 1.0;
-        b->grid_small = (uint32_t)std::max<double>(1.0, std::ceil(share * slots));
-        b->grid_small = std::min<uint32_t>(b->grid_small, std::max<uint32_t>(1, (uint32_t)b->queue_small.size()));
+        b->grid = std::min<uint32_t>(b->grid, (uint32_t)b->queue.size());
         b->prepared = true;
         b->info = qlx_batch_info{};
         b->info.circuits = b->circs.size();
@@ -486,7 +405,6 @@ int qlx_batch_run(qlx_batch* b, int mode) {
         switch (mode) {
             case QLX_BATCH_PERSISTENT: launch_persistent(b); break;
             case QLX_BATCH_LEVELS: launch_levels(b); break;
-            case QLX_BATCH_HYBRID: launch_hybrid(b); break;
             case QLX_BATCH_PER_CIRCUIT: launch_per_circuit(b); break;
             case QLX_BATCH_PER_CIRCUIT_GRAPH: {
                 if (!b->graph_exec) {

@@ -139,7 +139,6 @@ QLX_BATCH_PERSISTENT = 0
 QLX_BATCH_PER_CIRCUIT = 1
 QLX_BATCH_PER_CIRCUIT_GRAPH = 2
 QLX_BATCH_LEVELS = 3
-QLX_BATCH_HYBRID = 4
 
 
 class DeviceInfo(C.Structure):

@@ -12,8 +12,7 @@ from .engine import State, _dt, exec_opts
 from .ir import MT19937_64, Program, layered
 
 MODES = {"persistent": _abi.QLX_BATCH_PERSISTENT, "per_circuit": _abi.QLX_BATCH_PER_CIRCUIT,
-         "per_circuit_graph": _abi.QLX_BATCH_PER_CIRCUIT_GRAPH, "levels": _abi.QLX_BATCH_LEVELS,
-         "hybrid": _abi.QLX_BATCH_HYBRID}
+         "per_circuit_graph": _abi.QLX_BATCH_PER_CIRCUIT_GRAPH, "levels": _abi.QLX_BATCH_LEVELS}
 
 
 def batch_workload(count: int = 4096, qubits=(10, 20), depths=(5, 40), shape_seed: int = 4004,

@@ -23,7 +23,7 @@ def workload(qlx, count, qubits, depths, seed):
     return [qlx.layered(n, d, s) for n, d, s in qlx.batch_workload(count, qubits, depths, seed, 410000)]
 
 
-@pytest.mark.parametrize("mode", ["persistent", "hybrid", "levels", "per_circuit", "per_circuit_graph"])
+@pytest.mark.parametrize("mode", ["persistent", "levels", "per_circuit", "per_circuit_graph"])
 def test_batch_c64_matches_c128(gpu, mode):
     progs = workload(gpu, 24, (10, 16), (3, 12), 4321)
     b = gpu.Batch(progs, "c64")
@@ -101,7 +101,7 @@ def test_batch_autotune_and_fused_batch(gpu):
     fused = qb.prepare_programs(progs, 2)
     assert any((p.meta["num_qubits"] == 2).any() for p in fused if len(p.meta))
     b = gpu.Batch(fused, "c64", window_qubits=12)
-    b.run("hybrid")
+    b.run("levels")
     b.sync()
     for i, p in enumerate(progs):
         ref = ref_state(gpu, p)
