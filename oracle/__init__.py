@@ -113,11 +113,16 @@ def execute(num_qubits: int, instrs: np.ndarray, pool=None, meta=None, amps=None
     return a, [int(x) for x in meas[:nm.value]]
 
 
-def time_execute(num_qubits: int, instrs: np.ndarray, reps: int = 1, lib: str = "fast") -> float:
+def time_execute(num_qubits: int, instrs: np.ndarray, reps: int = 1, lib: str = "fast", pool=None,
+                 meta=None) -> float:
     L = strict() if lib == "strict" else fast()
     a = np.empty(1 << num_qubits, np.complex128)
     ins = np.ascontiguousarray(instrs)
-    return L.orc_time_execute(a.ctypes.data, num_qubits, ins.ctypes.data, len(ins), None, 0, None, 0, reps)
+    pool = None if pool is None or len(pool) == 0 else np.ascontiguousarray(pool, np.complex128)
+    meta = None if meta is None or len(meta) == 0 else np.ascontiguousarray(meta)
+    return L.orc_time_execute(a.ctypes.data, num_qubits, ins.ctypes.data, len(ins),
+                              pool.ctypes.data if pool is not None else None, 0 if pool is None else pool.size,
+                              meta.ctypes.data if meta is not None else None, 0 if meta is None else len(meta), reps)
 
 
 def num_threads(lib: str = "fast") -> int:

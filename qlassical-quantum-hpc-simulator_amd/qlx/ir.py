@@ -236,6 +236,24 @@ def qft(num_qubits: int) -> Program:
     return c.release()
 
 
+def clustered(num_qubits: int, cluster_size: int = 6, depth: int = 15) -> Program:
+    """The strategy-matrix benchmark circuit (reference benchmarks/bench_hw_awareness.cpp:30-53,
+    run there with cluster_size 6, depth 15 at 24-28 qubits): the qubits split into clusters of
+    `cluster_size` consecutive qubits (the last one may be shorter); each cluster in turn gets
+    `depth` layers of H on every member followed by the CX chain (q, q+1) inside it."""
+    if cluster_size < 1:
+        raise ValueError("cluster_size must be positive")
+    c = Circuit(num_qubits)
+    for lo in range(0, num_qubits, cluster_size):
+        hi = min(lo + cluster_size, num_qubits)
+        for _ in range(depth):
+            for q in range(lo, hi):
+                c.h(q)
+            for q in range(lo, hi - 1):
+                c.cx(q, q + 1)
+    return c.release()
+
+
 def transpile_gpu(prog: Program, fusion: bool = True, max_width: int = 4, rotations: bool = True,
                   windowing: bool = False, k_safe: int = 0, numa_penalty: int = 10, swap_penalty: int = 1,
                   restore: bool = True, dtype: int = 0, on_device: bool = False) -> Program:

@@ -90,6 +90,18 @@ def run(n: int, instrs, pool=None, meta=None, psi=None, gswap: str = "noop") -> 
     return psi
 
 
+def clustered_state(n: int, cluster: int, depth: int, cluster_instrs) -> np.ndarray:
+    """The clustered benchmark circuit never entangles two clusters, so from |0...0> its
+    state is the Kronecker product of each cluster's own state (low qubits = low bits:
+    cluster 0 is the rightmost factor). cluster_instrs(size) gives the instructions of one
+    cluster of `size` qubits relabelled to qubits 0..size-1; each factor is brute-forced."""
+    out = np.ones(1, complex)
+    for lo in range(0, n, cluster):
+        size = min(cluster, n - lo)
+        out = np.kron(run(size, cluster_instrs(size)), out)
+    return out
+
+
 def random_state(n: int, rng: np.random.Generator) -> np.ndarray:
     v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
     return v / np.linalg.norm(v)

@@ -6,6 +6,7 @@
 #include <type_traits>
 #include <vector>
 
+#include <frontend/circuit_generators.hpp>
 #include <frontend/frontend.hpp>
 #include <middleware/gpu_middleware.hpp>
 
@@ -146,6 +147,32 @@ TEST_CASE("transpile_gpu windowing (test_middleware.cpp:94-140, 164-235, 296-323
     gpu::transpile_gpu(m4, cfg);
     CHECK(m4.gate_count() > 5);
     CHECK(types(m4).back() == GateType::GLOBAL_SWAP);
+}
+
+TEST_CASE("clustered benchmark circuit: gate counts of the strategy matrix (bench.txt:9, 15, 21, 27)") {
+    // 28 qubits, clusters of 6, depth 15: the reference prints 765 -> 765 / 774 / 76 / 85 gates
+    // for no pass / SWAP windowing at k_safe 18 / fusion / both.
+    auto make = [] { return frontend::make_clustered_circuit(28, 6, 15).release(); };
+    IRModule base = make();
+    CHECK(base.gate_count() == 765);
+    auto p = base.program();
+    CHECK(p[0].type == GateType::H && p[0].qubits[0] == 0);
+    CHECK(p[6].type == GateType::CX && p[6].qubits[0] == 0 && p[6].qubits[1] == 1);
+    CHECK(p[764].type == GateType::CX && p[764].qubits[0] == 26 && p[764].qubits[1] == 27);
+    const struct { bool fuse, window; uint32_t gates; } rows[] = {
+        {false, true, 774}, {true, false, 76}, {true, true, 85}};
+    for (const auto& r : rows) {
+        gpu::GpuTranspilerConfig cfg;
+        cfg.enable_gate_fusion = r.fuse;
+        cfg.fuse_rotations = false;
+        cfg.max_fusion_width = 4;
+        cfg.enable_windowing = r.window;
+        cfg.restore_layout = false;
+        cfg.k_safe = 18;
+        IRModule m = make();
+        gpu::transpile_gpu(m, cfg);
+        CHECK(m.gate_count() == r.gates);
+    }
 }
 
 int main(int argc, char** argv) { return mt::run_all(argc > 1 ? argv[1] : nullptr) ? 1 : 0; }

@@ -134,3 +134,38 @@ def test_fusion_then_windowing_preserves_the_state():
     a, _ = oracle.execute(n, p.instrs, amps=init)
     b, _ = oracle.execute(n, w.instrs, w.pool, w.meta, amps=init, gswap="permute")
     np.testing.assert_allclose(b, a, atol=1e-12)
+
+
+# ---- the clustered benchmark circuit (benchmarks/bench_hw_awareness.cpp:30-53) -----------------
+def test_clustered_circuit_shape():
+    p = qlx.clustered(28)
+    assert len(p) == 765                                   # bench.txt:9 "Gates: 765"
+    t = p.instrs["type"]
+    assert (t == 0x00).sum() == 28 * 15 and (t == 0x20).sum() == 23 * 15
+    q = p.instrs["qubits"]
+    assert list(q[6][:2]) == [0, 1] and list(q[764][:2]) == [26, 27]
+    assert len(qlx.clustered(5, 2, 1)) == 5 + 2            # clusters {0,1}, {2,3}, {4}
+
+
+def test_clustered_strategy_matrix_gate_counts():
+    """The gate counts the reference prints at 28 qubits (bench.txt:9, 15, 21, 27): SWAP
+    windowing at k_safe 18 -> 774, fusion -> 76, both -> 85 (no layout restore, as there)."""
+    p = qlx.clustered(28)
+    style = dict(max_width=4, rotations=False, restore=False, k_safe=18)
+    assert len(qlx.transpile_gpu(p, fusion=False, windowing=True, **style)) == 774
+    assert len(qlx.transpile_gpu(p, fusion=True, windowing=False, **style)) == 76
+    assert len(qlx.transpile_gpu(p, fusion=True, windowing=True, **style)) == 85
+
+
+@pytest.mark.parametrize("n,cluster,depth", [(8, 3, 4), (13, 6, 15)])
+def test_clustered_product_form_on_oracle(n, cluster, depth):
+    """Oracle (plain, fused to width 4 and 5, windowed with restore) against the product of
+    brute-forced cluster states."""
+    p = qlx.clustered(n, cluster, depth)
+    want = bf.clustered_state(n, cluster, depth, lambda s: qlx.clustered(s, s, depth).instrs)
+    got, _ = oracle.execute(n, p.instrs)
+    assert np.abs(got - want).max() <= 1e-12
+    for width in (4, 5):
+        f = qlx.transpile_gpu(p, max_width=width, rotations=True, windowing=True, k_safe=n - 3)
+        got, _ = oracle.execute(n, f.instrs, f.pool, f.meta, gswap="permute")
+        assert np.abs(got - want).max() <= 1e-12, width
