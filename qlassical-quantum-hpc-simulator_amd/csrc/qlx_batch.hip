@@ -283,7 +283,8 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
         for (auto& c : b->circs) {
             const Geometry geo = batch_geometry(b->dtype, c.nq, o);
             c.plan = build_plan_geo(c.nq, geo, c.instrs.data(), c.instrs.size(),
-                                    c.pool.empty() ? nullptr : c.pool.data(), c.meta.empty() ? nullptr : c.meta.data(), o);
+                                    c.pool.empty() ? nullptr : c.pool.data(), c.meta.empty() ? nullptr : c.meta.data(), o,
+                                    /*thorough=*/false);
             for (const PlanStep& s : c.plan->steps)
                 if (!s.sweep) fail(QLX_EUNSUPPORTED, "a batched circuit needs a kernel outside the window sweep");
             c.amp_off = amp_total;

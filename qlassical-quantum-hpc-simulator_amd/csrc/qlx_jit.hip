@@ -7,13 +7,24 @@
 // qubit position, LDS offset and gate coefficient a compile-time constant, CX/SWAP
 // reduced to renaming registers in the generator — and compiled once for gfx950.
 // The generated code follows the generic kernel operation by operation (the same
-// descriptors: DevSweep / DevPhase / DevOp and the coefficient array), so both compute
-// the same gates in the same order with the same expressions.
+// descriptors: DevSweep / DevPhase / DevOp and the coefficient array), applying the same
+// gates in the same order. Two gates are computed with half the floating-point work:
+// every gate is linear and H and RX scale every amplitude of the chunk alike, so their
+// common real factor is taken out and applied once, later:
+//   H  = h [[1, 1], [1, -1]]                        -> butterfly (adds only), factor h
+//   RX = c [[1, -i t], [-i t, 1]],  t = s / c       -> two FMAs per amplitude, factor c
+//      = s [[u, -i], [-i, u]],      u = c / s       -> (when |s| > |c|), factor s
+// (c = cos(theta/2), s = sin(theta/2)). The generator multiplies the pending factors
+// together on the host and scales the registers by their product before the stores,
+// or earlier once it falls below 2^-30 (the registers then hold at most 2^30 times the
+// amplitudes' magnitude; |factor| >= 1/sqrt 2 per gate). The results differ from the
+// generic kernel's by rounding only.
 
 #include <hip/hiprtc.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <mutex>
 #include <string>
@@ -53,6 +64,23 @@ struct Gen {
     std::string out;
     std::vector<int> map;  // register index -> variable
     int tmp = 0;
+    double scale = 1.0;    // real factor taken out of the gates so far, not yet applied
+
+    // multiplies every register by the pending factor
+    void flush() {
+        if (scale == 1.0) return;
+        const std::string T = f32 ? "float" : "double";
+        line("  { const " + T + " k = " + lit(scale, f32) + ";");
+        std::string l = "   ";
+        for (int i = 0; i < R; ++i) l += " " + var(i) + ".x *= k; " + var(i) + ".y *= k;";
+        line(l);
+        line("  }");
+        scale = 1.0;
+    }
+    void defer(double factor) {
+        scale *= factor;
+        if (std::fabs(scale) < 0x1p-30) flush();
+    }
 
     void line(const std::string& s) { out += s; out += '\n'; }
     std::string var(int i) const { return "r" + std::to_string(map[i]); }
@@ -77,27 +105,45 @@ struct Gen {
         const std::string id = std::to_string(tmp++);
         switch (kind) {
             case OP_RX: {
-                line("  { const " + T + " c = " + lit(o.c0, f32) + ", s = " + lit(o.c1, f32) + ";");
-                for (int i = 0; i < R; ++i) {
-                    if (i & (1 << b0)) continue;
-                    const std::string x = var(i), y = var(i | (1 << b0));
-                    line("    { const " + T2 + " a0 = " + x + ", a1 = " + y + "; " + x + " = " + T2 +
-                         "{c * a0.x + s * a1.y, c * a0.y + (-s * a1.x)}; " + y + " = " + T2 +
-                         "{s * a0.y + c * a1.x, -s * a0.x + c * a1.y}; }");
+                // a0' = c a0 - i s a1, a1' = -i s a0 + c a1 (c = c0, s = c1)
+                const double c = o.c0, s = o.c1;
+                const std::string F = f32 ? "__builtin_fmaf" : "__builtin_fma";
+                if (std::fabs(c) >= std::fabs(s)) {
+                    line("  { const " + T + " t = " + lit(s / c, f32) + ";");
+                    for (int i = 0; i < R; ++i) {
+                        if (i & (1 << b0)) continue;
+                        const std::string x = var(i), y = var(i | (1 << b0));
+                        line("    { const " + T2 + " a0 = " + x + ", a1 = " + y + "; " + x + " = " + T2 + "{" + F +
+                             "(t, a1.y, a0.x), " + F + "(-t, a1.x, a0.y)}; " + y + " = " + T2 + "{" + F +
+                             "(t, a0.y, a1.x), " + F + "(-t, a0.x, a1.y)}; }");
+                    }
+                    line("  }");
+                    defer(c);
+                } else {
+                    line("  { const " + T + " u = " + lit(c / s, f32) + ";");
+                    for (int i = 0; i < R; ++i) {
+                        if (i & (1 << b0)) continue;
+                        const std::string x = var(i), y = var(i | (1 << b0));
+                        line("    { const " + T2 + " a0 = " + x + ", a1 = " + y + "; " + x + " = " + T2 + "{" + F +
+                             "(u, a0.x, a1.y), " + F + "(u, a0.y, -a1.x)}; " + y + " = " + T2 + "{" + F +
+                             "(u, a1.x, a0.y), " + F + "(u, a1.y, -a0.x)}; }");
+                    }
+                    line("  }");
+                    defer(s);
                 }
-                line("  }");
                 break;
             }
             case OP_H: {
-                line("  { const " + T + " h = " + lit(o.c0, f32) + ";");
+                // h [[1, 1], [1, -1]]: the butterfly now, the factor h later
+                line("  {");
                 for (int i = 0; i < R; ++i) {
                     if (i & (1 << b0)) continue;
                     const std::string x = var(i), y = var(i | (1 << b0));
                     line("    { const " + T2 + " a0 = " + x + ", a1 = " + y + "; " + x + " = " + T2 +
-                         "{h * (a0.x + a1.x), h * (a0.y + a1.y)}; " + y + " = " + T2 +
-                         "{h * (a0.x - a1.x), h * (a0.y - a1.y)}; }");
+                         "{a0.x + a1.x, a0.y + a1.y}; " + y + " = " + T2 + "{a0.x - a1.x, a0.y - a1.y}; }");
                 }
                 line("  }");
+                defer(o.c0);
                 break;
             }
             case OP_U2: {
@@ -199,6 +245,7 @@ struct Gen {
             cur = &P;
             for (uint32_t k = 0; k < P.n_ops; ++k) op(p.dop[P.op_begin + k]);
         }
+        flush();
         line("  const unsigned long long so = base | " + thread_bits64(sw.st_tglob) + ";");
         for (int i = 0; i < R; ++i) line("  amp[so | " + hex64(sw.st_roff[i]) + "] = " + var(i) + ";");
         line("}");

@@ -360,7 +360,7 @@ int qlx_server_submit(qlx_server* s, uint32_t nq, const qlx_gate_instr* instrs, 
         geo.w = std::min<int>((int)nq, s->dtype == QLX_C64 ? 13 : 12);
         geo.rb = geo.w - 8;
         geo.c = s->dtype == QLX_C64 ? 4 : 3;  // one 128-byte cache line of amplitudes
-        r.plan = build_plan_geo(nq, geo, r.instrs.data(), n, nullptr, nullptr, o);
+        r.plan = build_plan_geo(nq, geo, r.instrs.data(), n, nullptr, nullptr, o, /*thorough=*/false);
         const SweepPlan& p = *r.plan;
         r.n_sweeps = (uint32_t)p.dsw.size();
         r.rb = (uint32_t)geo.rb;

@@ -338,8 +338,10 @@ struct SweepPlan {
 };
 
 // Plans windowed execution of a MEASURE-free stream with an explicit geometry (W <= nq).
+// thorough: also try the classic greedy window chooser and keep the better plan (twice
+// the planning time; for programs planned once and executed many times).
 std::unique_ptr<SweepPlan> build_plan_geo(uint32_t nq, const Geometry& geo, const qlx_gate_instr* instrs, size_t n,
                                           const double* pool, const qlx_matrix_entry* meta,
-                                          const qlx_exec_opts& opts);
+                                          const qlx_exec_opts& opts, bool thorough);
 
 }  // namespace qlx

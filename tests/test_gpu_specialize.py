@@ -61,3 +61,53 @@ def test_specialized_kernels_are_reused(gpu):
     assert t["k_sweep_spec"][0] == 3 * st.last_stats()["sweeps_full"]
     assert abs(st.norm_sq() - 1) < 1e-12
     st.close()
+
+
+@pytest.mark.parametrize("dtype,w,rb", [("c128", 13, 4), ("c128", 13, 5), ("c128", 11, 3), ("c64", 14, 5),
+                                        ("c64", 12, 4)])
+def test_specialized_window_geometries(gpu, dtype, w, rb):
+    """Windows only the specialised sweeps take (c128: 13 qubits, 128 KiB of LDS; c64: 14)
+    and smaller ones, against the oracle."""
+    o = gpu.exec_opts("windowed", window_qubits=w, register_qubits=rb, specialize=True)
+    for prog in (gpu.layered(18, 10, 4321), gpu.clustered(17, 6, 15), gpu.hea(16, 5)):
+        n = prog.num_qubits
+        init = bf.random_state(n, np.random.default_rng(n + w))
+        want, _ = oracle.execute(n, prog.instrs, amps=init)
+        st = gpu.State(amps=init, dtype=dtype)
+        st.timing(True)
+        st.execute(prog, o)
+        assert "k_sweep_spec" in st.timing_read()
+        got = st.to_host()
+        st.close()
+        if dtype == "c128":
+            assert np.abs(got - want).max() <= 1e-12, (w, rb, n)
+        else:
+            inf = 1 - abs(np.vdot(want, got)) ** 2 / np.vdot(got, got).real
+            assert inf <= 1e-6, (w, rb, n, inf)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_long_runs_of_factored_gates(gpu, dtype):
+    """Hundreds of H and RX gates in one sweep: their factors are applied in several steps
+    (the registers stay within 2^30 of the amplitudes' magnitude, also in FP32)."""
+    n = 14
+    c = gpu.Circuit(n)
+    rng = np.random.default_rng(17)
+    for k in range(400):
+        q = int(rng.integers(0, 6))
+        if k % 3:
+            c.h(q)
+        else:
+            c.rx(q, float(rng.choice([np.pi, np.pi / 2, 3 * np.pi / 2 + 1e-3, 1e-4, rng.uniform(0, 2 * np.pi)])))
+    prog = c.release()
+    init = bf.random_state(n, np.random.default_rng(3))
+    want, _ = oracle.execute(n, prog.instrs, amps=init)
+    st = gpu.State(amps=init, dtype=dtype)
+    st.execute(prog, gpu.exec_opts("windowed", specialize=True))
+    got = st.to_host()
+    st.close()
+    if dtype == "c128":
+        assert np.abs(got - want).max() <= 1e-12
+    else:
+        assert 1 - abs(np.vdot(want, got)) ** 2 / np.vdot(got, got).real <= 1e-6
+        assert abs(np.vdot(got, got).real - 1) <= 1e-5

@@ -75,3 +75,32 @@ def test_sources_compile_for_gfx950():
         for s in range(min(2, info["sweeps"])):
             rc, log = compile_ok(qlx.sweep_source(prog, s, dtype))
             assert rc == 0, log[-2000:]
+
+
+def test_h_and_rx_take_their_factor_out():
+    """H runs as a butterfly (no multiplications) and RX as two FMAs per amplitude; their
+    common factors are applied once, as one multiplication per register before the stores."""
+    prog = qlx.Circuit(13).h(0).h(5).rx(3, 0.7).rx(4, 3.0).release()
+    src = qlx.sweep_source(prog, 0, "c128")
+    body = [l for l in src.splitlines() if "a0 = " in l]
+    assert body and all("*" not in l for l in body)        # fma calls and adds only
+    assert sum("__builtin_fma(" in l for l in body) == 16   # 2 RX x 8 pairs x 1 line
+    assert len(re.findall(r"r\d+\.x \*= k;", src)) == 16  # the factor, once per register
+    k = float.fromhex(re.search(r"const double k = ([^;]+);", src).group(1))
+    import math
+    # h^2, then c for RX(0.7) (|c| >= |s|) and s for RX(3.0) (|s| > |c|); angles are stored as float
+    want = (1 / math.sqrt(2)) ** 2 * math.cos(np.float32(0.7) / 2) * math.sin(np.float32(3.0) / 2)
+    assert abs(k - want) <= 1e-15
+
+
+def test_wider_windows_for_specialised_sweeps():
+    prog = qlx.layered(20, 6, 3)
+    with pytest.raises(qlx.QlxUnsupported):
+        qlx.plan_windowed(prog, "c128", qlx.exec_opts("windowed", window_qubits=13, register_qubits=4))
+    o = qlx.exec_opts("windowed", window_qubits=13, register_qubits=4, specialize=True)
+    _, _, info = qlx.plan_windowed(prog, "c128", o)
+    assert info["window_qubits"] == 13 and info["threads_per_group"] == 512
+    assert "__shared__ double2 lds[8192]" in qlx.sweep_source(prog, 0, "c128", o)
+    with pytest.raises(qlx.QlxUnsupported):       # 256 KiB of LDS
+        qlx.plan_windowed(prog, "c128", qlx.exec_opts("windowed", window_qubits=14, register_qubits=5,
+                                                      specialize=True))
