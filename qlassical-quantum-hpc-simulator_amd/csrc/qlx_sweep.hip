@@ -409,9 +409,159 @@ static std::vector<int> choose_window_wide(const std::vector<PGate>& gates, cons
     return S;
 }
 
+// Plain-kernel cost of a gate in full-state passes (what launch_plain_instr touches).
+static double plain_passes(uint8_t type) {
+    switch (type) {
+        case QLX_Z: case QLX_S: case QLX_CX: case QLX_SWAP: case QLX_GLOBAL_SWAP: return 0.5;
+        case QLX_CCX: case QLX_CSWAP: return 0.25;
+        case QLX_BARRIER: return 0.0;
+        default: return 1.0;
+    }
+}
+
+// The planner's walk over the gate DAG: plain-only gates run as soon as they are ready;
+// otherwise the next sweep takes every pending gate its window can run.
+struct Walk {
+    const std::vector<PGate>* gates;
+    const std::vector<std::vector<int>>* preds;
+    std::vector<char> done;
+    std::vector<int> pending;
+    size_t n_done = 0;
+
+    Walk(const std::vector<PGate>& g, const std::vector<std::vector<int>>& p)
+        : gates(&g), preds(&p), done(g.size(), 0), pending(g.size()) {
+        for (size_t i = 0; i < g.size(); ++i) pending[i] = (int)i;
+    }
+    bool ready(int gi) const {
+        for (int p : (*preds)[gi])
+            if (!done[p]) return false;
+        return true;
+    }
+    void prune() {
+        std::vector<int> still;
+        still.reserve(pending.size());
+        for (int gi : pending)
+            if (!done[gi]) still.push_back(gi);
+        pending.swap(still);
+    }
+    // runs every plain-only gate that is (or becomes) ready; returns them in the order run
+    std::vector<int> run_plain() {
+        std::vector<int> ran;
+        bool progress = true;
+        while (progress) {
+            progress = false;
+            for (int gi : pending) {
+                if (done[gi] || (*gates)[gi].sweepable || !ready(gi)) continue;
+                done[gi] = 1;
+                ++n_done;
+                ran.push_back(gi);
+                progress = true;
+            }
+            prune();
+        }
+        return ran;
+    }
+    // the first ready sweepable gate, or -1
+    int seed_gate() const {
+        for (int gi : pending)
+            if ((*gates)[gi].sweepable && ready(gi)) return gi;
+        return -1;
+    }
+    // the gates one sweep over window S runs, in program order; marks them done
+    std::vector<int> take(const std::vector<int>& S, int nq) {
+        std::vector<char> sset(nq, 0);
+        for (int q : S) sset[q] = 1;
+        std::vector<char> run(gates->size(), 0);
+        count_runnable(*gates, pending, done, sset, run, *preds);
+        std::vector<int> out;
+        for (int gi : pending)
+            if (run[gi]) {
+                out.push_back(gi);
+                done[gi] = 1;
+                ++n_done;
+            }
+        prune();
+        return out;
+    }
+};
+
+// Candidate windows for the next sweep (see choose_window_wide), as qubit lists.
+static std::vector<std::vector<int>> window_candidates(const Walk& w, int nq, const Geometry& geo, int seed_gate) {
+    const int W = geo.w, C = std::min(geo.c, nq);
+    std::vector<std::vector<int>> out;
+    {
+        std::vector<char> s(nq, 0);
+        int have = 0;
+        for (int l = 0; l < C; ++l, ++have) s[l] = 1;
+        const PGate& g = (*w.gates)[seed_gate];
+        for (int k = 0; k < g.nq; ++k)
+            if (!s[g.q[k]]) {
+                s[g.q[k]] = 1;
+                ++have;
+            }
+        const std::vector<char> allowed(nq, 1);
+        out.push_back(choose_set(*w.gates, w.pending, w.done, *w.preds, nq, s, have, W, allowed));
+    }
+    for (int a = C; a + (W - C) <= nq; ++a) {
+        std::vector<int> S;
+        for (int l = 0; l < C; ++l) S.push_back(l);
+        for (int q = a; q < a + (W - C); ++q) S.push_back(q);
+        if (S != out.front()) out.push_back(std::move(S));
+    }
+    return out;
+}
+
+// Beam search over window sequences: each step extends every kept walk by each candidate
+// window and keeps the `width` walks that have run the most gates (distinct states only).
+// Returns the windows of the first walk to finish.
+static std::vector<std::vector<int>> beam_windows(const std::vector<PGate>& gates,
+                                                  const std::vector<std::vector<int>>& preds, int nq,
+                                                  const Geometry& geo, int width) {
+    struct Node {
+        Walk w;
+        std::vector<std::vector<int>> wins;
+    };
+    std::vector<Node> beam;
+    beam.push_back(Node{Walk(gates, preds), {}});
+    beam[0].w.run_plain();
+    for (size_t step = 0; step <= gates.size(); ++step) {
+        for (const Node& nd : beam)
+            if (nd.w.pending.empty()) return nd.wins;
+        std::vector<Node> next;
+        for (const Node& nd : beam) {
+            const int seed = nd.w.seed_gate();
+            if (seed < 0) fail(QLX_EHIP, "sweep planner: no runnable gate");
+            for (const std::vector<int>& S : window_candidates(nd.w, nq, geo, seed)) {
+                Node c = nd;
+                if (c.w.take(S, nq).empty()) continue;
+                c.w.run_plain();
+                c.wins.push_back(S);
+                next.push_back(std::move(c));
+            }
+        }
+        std::stable_sort(next.begin(), next.end(), [](const Node& x, const Node& y) { return x.w.n_done > y.w.n_done; });
+        beam.clear();
+        for (Node& c : next) {
+            bool dup = false;
+            for (const Node& k : beam) dup = dup || k.w.done == c.w.done;
+            if (dup) continue;
+            beam.push_back(std::move(c));
+            if ((int)beam.size() >= width) break;
+        }
+        if (beam.empty()) fail(QLX_EHIP, "sweep planner: beam search made no progress");
+    }
+    fail(QLX_EHIP, "sweep planner: beam search did not finish");
+}
+
+enum class Chooser { Greedy, Wide, Given };
+
+// Builds the plan along one walk. Windows come from the greedy chooser, the wide chooser,
+// or a given sequence (a beam search's). tail_plain: a sweep whose gates cost less than
+// one pass as plain kernels (e.g. a lone CX: half a pass) runs them as plain kernels.
 static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& geo, const qlx_gate_instr* instrs,
                                                   size_t n, const double* pool, const qlx_matrix_entry* meta,
-                                                  const qlx_exec_opts& opts, bool wide) {
+                                                  const qlx_exec_opts& opts, Chooser chooser,
+                                                  const std::vector<std::vector<int>>* given, bool tail_plain) {
     auto plan = std::make_unique<SweepPlan>();
     if ((int)nq < geo.w) fail(QLX_EINVAL, "window larger than the state");
     plan->geo = geo;
@@ -442,85 +592,67 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
                 last[q] = i;
             }
     }
-    std::vector<char> done(G, 0);
-    std::vector<int> pending(G);
-    for (int i = 0; i < G; ++i) pending[i] = i;
-
-    auto prune = [&]() {
-        std::vector<int> still;
-        for (int gi : pending)
-            if (!done[gi]) still.push_back(gi);
-        pending.swap(still);
+    std::vector<std::vector<int>> beam;
+    if (chooser == Chooser::Given && !given) {
+        beam = beam_windows(gates, preds, (int)nq, geo, 16);
+        given = &beam;
+    }
+    auto plain_step = [&](int gi) {
+        plan->steps.push_back(PlanStep{false, gates[gi].src, -1});
+        plan->order.push_back(gates[gi].src);
+        plan->order_sweep.push_back(-1);
+        plan->passes += plain_passes(gates[gi].type);
     };
 
-    while (!pending.empty()) {
+    Walk walk(gates, preds);
+    size_t next_window = 0;
+    while (true) {
         // 1. plain-only gates whose predecessors are done run as their own kernels
-        bool progress = true;
-        while (progress) {
-            progress = false;
-            for (int gi : pending) {
-                if (done[gi] || gates[gi].sweepable) continue;
-                bool ready = true;
-                for (int p : preds[gi]) ready = ready && done[p];
-                if (!ready) continue;
-                done[gi] = 1;
-                plan->steps.push_back(PlanStep{false, gates[gi].src, -1});
-                plan->order.push_back(gates[gi].src);
-                plan->order_sweep.push_back(-1);
-                progress = true;
-            }
-            prune();
-        }
-        if (pending.empty()) break;
+        for (int gi : walk.run_plain()) plain_step(gi);
+        if (walk.pending.empty()) break;
 
-        // 2. choose the window: low C qubits + the first runnable sweepable gate's qubits + greedy
-        std::vector<char> in_set(nq, 0);
-        int have = 0;
-        for (int l = 0; l < geo.c && l < (int)nq; ++l) {
-            in_set[l] = 1;
-            ++have;
-        }
-        int seed_gate = -1;
-        for (int gi : pending) {
-            if (!gates[gi].sweepable) continue;
-            bool ready = true;
-            for (int p : preds[gi]) ready = ready && done[p];
-            if (ready) {
-                seed_gate = gi;
-                break;
-            }
-        }
+        // 2. the window
+        const int seed_gate = walk.seed_gate();
         if (seed_gate < 0) fail(QLX_EHIP, "sweep planner: no runnable gate");
         std::vector<int> S;
-        if (wide) {
-            S = choose_window_wide(gates, pending, done, preds, (int)nq, geo, seed_gate);
+        if (chooser == Chooser::Given) {
+            if (next_window >= given->size()) fail(QLX_EHIP, "sweep planner: window sequence too short");
+            S = (*given)[next_window++];
+        } else if (chooser == Chooser::Wide) {
+            S = choose_window_wide(gates, walk.pending, walk.done, preds, (int)nq, geo, seed_gate);
         } else {
+            std::vector<char> in_set(nq, 0);
+            int have = 0;
+            for (int l = 0; l < geo.c && l < (int)nq; ++l, ++have) in_set[l] = 1;
             for (int k = 0; k < gates[seed_gate].nq; ++k)
                 if (!in_set[gates[seed_gate].q[k]]) {
                     in_set[gates[seed_gate].q[k]] = 1;
                     ++have;
                 }
             std::vector<char> allowed(nq, 1);
-            S = choose_set(gates, pending, done, preds, (int)nq, in_set, have, W, allowed);
+            S = choose_set(gates, walk.pending, walk.done, preds, (int)nq, in_set, have, W, allowed);
         }
         if ((int)S.size() != W) fail(QLX_EHIP, "sweep planner: window smaller than W");
 
         // 3. the sweep's gates, in program order
-        std::vector<char> sset(nq, 0);
-        for (int q : S) sset[q] = 1;
-        std::vector<char> run(G, 0);
-        count_runnable(gates, pending, done, sset, run, preds);
+        const std::vector<int> taken = walk.take(S, (int)nq);
+        if (taken.empty()) fail(QLX_EHIP, "sweep planner: empty sweep");
+        if (tail_plain) {
+            double cost = 0.0;
+            for (int gi : taken) cost += plain_passes(gates[gi].type);
+            if (cost < 1.0) {
+                for (int gi : taken) plain_step(gi);
+                continue;
+            }
+        }
         std::vector<int> loc(nq, -1);
         for (int j = 0; j < W; ++j) loc[S[j]] = j;
         std::vector<PGate> lg;
-        for (int gi : pending) {
-            if (!run[gi]) continue;
+        for (int gi : taken) {
             PGate l = gates[gi];
             for (int k = 0; k < l.nq; ++k) l.q[k] = loc[l.q[k]];
             lg.push_back(l);
-            done[gi] = 1;
         }
-        prune();
         PlanSweep sw;
         sw.S = S;
         plan_phases(sw, lg, geo);
@@ -532,6 +664,7 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
             }
         plan->gates_in_sweeps += lg.size();
         plan->phases += sw.phases.size();
+        plan->passes += 1.0;
         plan->sweeps.push_back(std::move(sw));
         plan->steps.push_back(PlanStep{true, -1, si});
     }
@@ -590,21 +723,25 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
     return plan;
 }
 
-// Plans with the wider window search; `thorough` also plans with the classic greedy
-// chooser alone and keeps the plan with fewer sweeps (then fewer phases). Over 3,198
+// Plans with the wider window search. `thorough` also plans with the classic greedy
+// chooser alone and with a beam search over window sequences (width 4), runs sweeps that
+// would cost less than one pass as plain kernels, and keeps the plan with the fewest
+// passes (then fewest phases). Over 3,198
 // layered, HEA, QFT and clustered circuits of 13-28 qubits (complex64, 13-qubit window)
 // the wider search alone needs fewer sweeps than the greedy chooser for 1,382 of them
 // (18,732 sweeps in all instead of 20,340) and one more sweep for 18.
 std::unique_ptr<SweepPlan> build_plan_geo(uint32_t nq, const Geometry& geo, const qlx_gate_instr* instrs, size_t n,
                                           const double* pool, const qlx_matrix_entry* meta,
                                           const qlx_exec_opts& opts, bool thorough) {
-    if (!thorough) return build_plan_with(nq, geo, instrs, n, pool, meta, opts, true);
-    auto a = build_plan_with(nq, geo, instrs, n, pool, meta, opts, false);
-    if (a->sweeps.size() <= 1) return a;
-    auto b = build_plan_with(nq, geo, instrs, n, pool, meta, opts, true);
-    const bool take_b = b->sweeps.size() < a->sweeps.size() ||
-                        (b->sweeps.size() == a->sweeps.size() && b->phases < a->phases);
-    return take_b ? std::move(b) : std::move(a);
+    if (!thorough) return build_plan_with(nq, geo, instrs, n, pool, meta, opts, Chooser::Wide, nullptr, false);
+    auto best = build_plan_with(nq, geo, instrs, n, pool, meta, opts, Chooser::Greedy, nullptr, true);
+    if (best->sweeps.size() <= 1) return best;
+    for (Chooser c : {Chooser::Wide, Chooser::Given}) {
+        auto p = build_plan_with(nq, geo, instrs, n, pool, meta, opts, c, nullptr, true);
+        if (p->passes < best->passes - 1e-9 || (p->passes < best->passes + 1e-9 && p->phases < best->phases))
+            best = std::move(p);
+    }
+    return best;
 }
 
 static std::unique_ptr<SweepPlan> build_plan(uint32_t nq, int dtype, const qlx_gate_instr* instrs, size_t n,

@@ -335,11 +335,14 @@ struct SweepPlan {
     std::vector<DevOp> dop;
     std::vector<double> coef;
     uint64_t phases = 0, gates_in_sweeps = 0;
+    double passes = 0.0;  // full-state passes: one per sweep, plus the plain steps' fractions
 };
 
 // Plans windowed execution of a MEASURE-free stream with an explicit geometry (W <= nq).
-// thorough: also try the classic greedy window chooser and keep the better plan (twice
-// the planning time; for programs planned once and executed many times).
+// thorough: also try the classic greedy window chooser and a beam search, let sweeps that
+// cost less than a pass run as plain kernels, and keep the cheapest plan (several times the
+// planning time; for programs planned once and executed many times). Batched and served
+// circuits plan without it: their executors run sweeps only.
 std::unique_ptr<SweepPlan> build_plan_geo(uint32_t nq, const Geometry& geo, const qlx_gate_instr* instrs, size_t n,
                                           const double* pool, const qlx_matrix_entry* meta,
                                           const qlx_exec_opts& opts, bool thorough);

@@ -58,8 +58,24 @@ def test_config2_plan_shape():
     order, sweep, info = qlx.plan_windowed(p)
     print(info)
     assert info["gates_in_sweeps"] == 920 and info["plain_steps"] == 0
-    # far fewer HBM sweeps than the 775 pass-equivalents of one-kernel-per-gate execution
-    assert info["sweeps"] < 120
+    # far fewer HBM sweeps than the 775 pass-equivalents of one-kernel-per-gate execution:
+    # the beam search over windows needs 14 (the first-gate greedy window alone needs 21)
+    assert info["sweeps"] <= 14
+
+
+def test_cheap_sweeps_run_as_plain_kernels():
+    """A sweep whose gates would cost less than one pass as plain kernels (here the QFT's
+    trailing SWAPs, half a pass each) runs them as their own kernels; the order stays valid."""
+    n = 17
+    p = qlx.qft(n)
+    order, sweep, info = qlx.plan_windowed(p)
+    assert info["plain_steps"] >= 1
+    plain = p.instrs[order[sweep == -1]]
+    assert all(g["type"] in (0x20, 0x21) for g in plain)
+    init = bf.random_state(n, np.random.default_rng(5))
+    a, _ = oracle.execute(n, p.instrs, amps=init)
+    b, _ = oracle.execute(n, reordered(p, order), amps=init)
+    np.testing.assert_allclose(b, a, atol=1e-12)
 
 
 def test_small_states_plan_nothing():
