@@ -13,7 +13,9 @@ namespace qlx {
 struct JitSweeps {
     std::vector<hipModule_t> modules;
     std::vector<hipFunction_t> fns;
+    std::vector<uint64_t> resident;  // work-groups of each kernel the device holds at once
     unsigned threads = 0;
+    bool persistent = false;         // kernels loop over chunks (grid = resident work-groups)
     double compile_s = 0.0;
     JitSweeps() = default;
     JitSweeps(const JitSweeps&) = delete;
@@ -21,12 +23,16 @@ struct JitSweeps {
     ~JitSweeps();
 };
 
+// Whether the kernels of this geometry are persistent (see the generator).
+bool jit_persistent(const Geometry& g, int dtype);
 // HIP source of sweep `sweep_index` of the plan, specialised to its descriptors.
 std::string jit_sweep_source(const SweepPlan& p, int sweep_index, int dtype);
 // Generates and compiles every sweep of the plan for gfx950 (parallel host threads),
 // then loads them on the current device.
 std::shared_ptr<JitSweeps> build_jit_sweeps(const SweepPlan& p, int dtype);
-// One launch of a compiled sweep over the state (one work-group per chunk).
-void launch_jit_sweep(const JitSweeps& j, int sweep_index, qlx_state* st, int window_qubits);
+// One launch of a compiled sweep over the state: one work-group per chunk, or, for the
+// persistent kernels, as many work-groups as the device holds at once (wg_override: that
+// many instead, when nonzero).
+void launch_jit_sweep(const JitSweeps& j, int sweep_index, qlx_state* st, int window_qubits, uint32_t wg_override);
 
 }  // namespace qlx

@@ -261,154 +261,6 @@ static std::vector<int> choose_set(const std::vector<PGate>& gates, const std::v
     return out;
 }
 
-// Register phases of one sweep (gates given in local bits).
-static void plan_phases(PlanSweep& sw, const std::vector<PGate>& lg, const Geometry& geo) {
-    const int W = geo.w, RB = geo.rb, LT = geo.logt;
-    const int n = (int)lg.size();
-    std::vector<std::vector<int>> preds(n);
-    {
-        std::vector<int> last(W, -1);
-        for (int i = 0; i < n; ++i) {
-            for (int k = 0; k < lg[i].nq; ++k) {
-                const int b = lg[i].q[k];
-                if (last[b] >= 0) preds[i].push_back(last[b]);
-                last[b] = i;
-            }
-        }
-    }
-    std::vector<char> done(n, 0);
-    std::vector<int> pending(n);
-    for (int i = 0; i < n; ++i) pending[i] = i;
-
-    auto make_layout = [&](const std::vector<int>& regset) {
-        std::vector<int> st2loc(W, -1);
-        std::vector<char> isreg(W, 0);
-        for (int b : regset) isreg[b] = 1;
-        int ti = 0;
-        for (int l = 0; l < W; ++l)
-            if (!isreg[l]) st2loc[ti++] = l;          // work-item bits in ascending local order
-        int ri = 0;
-        for (int l : regset) st2loc[LT + ri++] = l;    // register bits
-        return st2loc;
-    };
-    auto regset_of = [&](const std::vector<int>& st2loc) {
-        std::vector<int> r;
-        for (int s = LT; s < W; ++s) r.push_back(st2loc[s]);
-        return r;
-    };
-
-    // first layout: the low C local bits must stay work-item bits (coalesced load)
-    std::vector<char> allowed(W, 1);
-    for (int l = 0; l < geo.c; ++l) allowed[l] = 0;
-    std::vector<char> none(W, 0);
-    std::vector<int> regset = choose_set(lg, pending, done, preds, W, none, 0, RB, allowed);
-    std::vector<char> all(W, 1);
-    bool first = true;
-    int guard_iters = 0;
-    while (true) {
-        PlanPhase phase;
-        phase.st2loc = make_layout(regset);
-        std::vector<int> loc2st(W);
-        for (int s = 0; s < W; ++s) loc2st[phase.st2loc[s]] = s;
-        std::vector<char> in_set(W, 0);
-        for (int b : regset) in_set[b] = 1;
-        std::vector<char> run(n, 0);
-        count_runnable(lg, pending, done, in_set, run, preds);
-        std::vector<int> still;
-        for (int gi : pending) {
-            if (run[gi]) {
-                PlanOp op{};
-                op.src = lg[gi].src;
-                op.l0 = loc2st[lg[gi].q[0]] - LT;
-                op.l1 = lg[gi].nq > 1 ? loc2st[lg[gi].q[1]] - LT : -1;
-                op.kind = lg[gi].type;  // instruction type; resolved to an OpKind later
-                phase.ops.push_back(op);
-                done[gi] = 1;
-            } else {
-                still.push_back(gi);
-            }
-        }
-        pending.swap(still);
-        if (!phase.ops.empty() || first) sw.phases.push_back(std::move(phase));
-        first = false;
-        if (pending.empty()) break;
-        if (++guard_iters > 4 * n + 8) fail(QLX_EHIP, "sweep phase planner made no progress");
-        // next register set: seed with the first pending gate's bits
-        std::vector<char> seed(W, 0);
-        int have = 0;
-        const PGate& g0 = lg[pending.front()];
-        for (int k = 0; k < g0.nq; ++k)
-            if (!seed[g0.q[k]]) {
-                seed[g0.q[k]] = 1;
-                ++have;
-            }
-        regset = choose_set(lg, pending, done, preds, W, seed, have, RB, all);
-    }
-    // last layout must keep the low C local bits as the low work-item bits (coalesced store)
-    std::vector<int> last = regset_of(sw.phases.back().st2loc);
-    bool low_in_regs = false;
-    for (int l : last)
-        if (l < geo.c) low_in_regs = true;
-    if (low_in_regs) {
-        std::vector<int> rs;
-        for (int l = W - 1; l >= geo.c && (int)rs.size() < RB; --l) rs.push_back(l);
-        std::sort(rs.begin(), rs.end());
-        PlanPhase phase;
-        phase.st2loc = make_layout(rs);
-        sw.phases.push_back(std::move(phase));
-    }
-}
-
-// Candidate windows for the next sweep, each the low C qubits plus W - C more: (a) the
-// greedy extension of the first ready gate's operands (the classic choice), (b) every run
-// of W - C consecutive qubits. The one that runs the most pending gates wins (ties: the
-// earlier candidate). On circuits of nearest-neighbour layers, (b) lets a sweep advance a
-// band of the qubit line by several layers at once: the 30-qubit layered h/rx/cx circuit
-// of depth 20 takes 15 sweeps of a 12-qubit window instead of 21.
-static std::vector<int> choose_window_wide(const std::vector<PGate>& gates, const std::vector<int>& pending,
-                                          const std::vector<char>& done, const std::vector<std::vector<int>>& preds,
-                                          int nq, const Geometry& geo, int seed_gate) {
-    const int W = geo.w, C = std::min(geo.c, nq);
-    std::vector<std::vector<char>> cands;
-    auto base = [&] {
-        std::vector<char> s(nq, 0);
-        for (int l = 0; l < C; ++l) s[l] = 1;
-        return s;
-    };
-    {
-        std::vector<char> s = base();
-        int have = C;
-        for (int k = 0; k < gates[seed_gate].nq; ++k)
-            if (!s[gates[seed_gate].q[k]]) {
-                s[gates[seed_gate].q[k]] = 1;
-                ++have;
-            }
-        const std::vector<char> allowed(nq, 1);
-        const std::vector<int> S = choose_set(gates, pending, done, preds, nq, s, have, W, allowed);
-        std::vector<char> c(nq, 0);
-        for (int q : S) c[q] = 1;
-        cands.push_back(std::move(c));
-    }
-    for (int a = C; a + (W - C) <= nq; ++a) {
-        std::vector<char> c = base();
-        for (int q = a; q < a + (W - C); ++q) c[q] = 1;
-        cands.push_back(std::move(c));
-    }
-    int best = 0, best_n = -1;
-    std::vector<char> run(gates.size(), 0);
-    for (size_t k = 0; k < cands.size(); ++k) {
-        const int cnt = count_runnable(gates, pending, done, cands[k], run, preds);
-        if (cnt > best_n) {
-            best = (int)k;
-            best_n = cnt;
-        }
-    }
-    std::vector<int> S;
-    for (int q = 0; q < nq; ++q)
-        if (cands[best][q]) S.push_back(q);
-    return S;
-}
-
 // Plain-kernel cost of a gate in full-state passes (what launch_plain_instr touches).
 static double plain_passes(uint8_t type) {
     switch (type) {
@@ -485,6 +337,213 @@ struct Walk {
     }
 };
 
+// Register sets of one sweep's phases, by beam search: each step extends every kept walk
+// by each candidate set and keeps the `width` walks that have run the most gates. The
+// candidates are the greedy extension of the first ready gate's qubits (the first phase:
+// the greedy set) and every run of RB consecutive local qubits (every RB-subset instead
+// gave no fewer phases on the layered circuits); the first phase may not
+// hold the low C local qubits in registers (its layout is the coalesced load's). Among
+// the walks that finish first, one whose last set leaves the low C qubits out of the
+// registers is preferred (otherwise the store needs one more re-layout).
+static std::vector<std::vector<int>> beam_regsets(const std::vector<PGate>& lg, const std::vector<std::vector<int>>& preds,
+                                                  const Geometry& geo, int width, bool runs) {
+    const int W = geo.w, RB = geo.rb, C = geo.c;
+    struct Node {
+        Walk w;
+        std::vector<std::vector<int>> sets;
+    };
+    auto candidates = [&](const Walk& w, bool first) {
+        std::vector<std::vector<int>> out;
+        std::vector<char> allowed(W, 1), seed(W, 0);
+        int have = 0;
+        if (first) {
+            for (int l = 0; l < C; ++l) allowed[l] = 0;
+        } else {
+            const int g0 = w.seed_gate();
+            if (g0 >= 0)
+                for (int k = 0; k < lg[g0].nq; ++k)
+                    if (!seed[lg[g0].q[k]]) {
+                        seed[lg[g0].q[k]] = 1;
+                        ++have;
+                    }
+        }
+        out.push_back(choose_set(lg, w.pending, w.done, preds, W, seed, have, RB, allowed));
+        for (int a = first ? C : 0; runs && a + RB <= W; ++a) {
+            std::vector<int> S;
+            for (int l = a; l < a + RB; ++l) S.push_back(l);
+            if (S != out.front()) out.push_back(std::move(S));
+        }
+        return out;
+    };
+    auto low_in = [&](const std::vector<int>& S) {
+        for (int l : S)
+            if (l < C) return true;
+        return false;
+    };
+    std::vector<Node> beam;
+    beam.push_back(Node{Walk(lg, preds), {}});
+    for (size_t step = 0; step <= lg.size() + 1; ++step) {
+        const Node* fin = nullptr;
+        for (const Node& nd : beam)
+            if (nd.w.pending.empty() && (!fin || (low_in(fin->sets.back()) && !low_in(nd.sets.back())))) fin = &nd;
+        if (fin) return fin->sets;
+        std::vector<Node> next;
+        for (const Node& nd : beam) {
+            const bool first = nd.sets.empty();
+            for (const std::vector<int>& S : candidates(nd.w, first)) {
+                Node c = nd;
+                if (c.w.take(S, W).empty() && !first) continue;
+                c.sets.push_back(S);
+                next.push_back(std::move(c));
+            }
+        }
+        std::stable_sort(next.begin(), next.end(), [](const Node& x, const Node& y) { return x.w.n_done > y.w.n_done; });
+        beam.clear();
+        for (Node& c : next) {
+            bool dup = false;
+            for (const Node& k : beam) dup = dup || k.w.done == c.w.done;
+            if (dup) continue;
+            beam.push_back(std::move(c));
+            if ((int)beam.size() >= width) break;
+        }
+        if (beam.empty()) fail(QLX_EHIP, "sweep phase planner made no progress");
+    }
+    fail(QLX_EHIP, "sweep phase planner did not finish");
+}
+
+// Register phases of one sweep (gates given in local bits). thorough: the beam search over
+// register sets (width 8, with the runs of consecutive qubits); otherwise the greedy set
+// alone at each step (width 1), which plans several times faster.
+static void plan_phases(PlanSweep& sw, const std::vector<PGate>& lg, const Geometry& geo, bool thorough) {
+    const int W = geo.w, RB = geo.rb, LT = geo.logt;
+    const int n = (int)lg.size();
+    std::vector<std::vector<int>> preds(n);
+    {
+        std::vector<int> last(W, -1);
+        for (int i = 0; i < n; ++i) {
+            for (int k = 0; k < lg[i].nq; ++k) {
+                const int b = lg[i].q[k];
+                if (last[b] >= 0) preds[i].push_back(last[b]);
+                last[b] = i;
+            }
+        }
+    }
+    std::vector<char> done(n, 0);
+    std::vector<int> pending(n);
+    for (int i = 0; i < n; ++i) pending[i] = i;
+
+    auto make_layout = [&](const std::vector<int>& regset) {
+        std::vector<int> st2loc(W, -1);
+        std::vector<char> isreg(W, 0);
+        for (int b : regset) isreg[b] = 1;
+        int ti = 0;
+        for (int l = 0; l < W; ++l)
+            if (!isreg[l]) st2loc[ti++] = l;          // work-item bits in ascending local order
+        int ri = 0;
+        for (int l : regset) st2loc[LT + ri++] = l;    // register bits
+        return st2loc;
+    };
+    auto regset_of = [&](const std::vector<int>& st2loc) {
+        std::vector<int> r;
+        for (int s = LT; s < W; ++s) r.push_back(st2loc[s]);
+        return r;
+    };
+
+    const std::vector<std::vector<int>> sets = beam_regsets(lg, preds, geo, thorough ? 8 : 1, thorough);
+    bool first = true;
+    for (const std::vector<int>& regset : sets) {
+        PlanPhase phase;
+        phase.st2loc = make_layout(regset);
+        std::vector<int> loc2st(W);
+        for (int s = 0; s < W; ++s) loc2st[phase.st2loc[s]] = s;
+        std::vector<char> in_set(W, 0);
+        for (int b : regset) in_set[b] = 1;
+        std::vector<char> run(n, 0);
+        count_runnable(lg, pending, done, in_set, run, preds);
+        std::vector<int> still;
+        for (int gi : pending) {
+            if (run[gi]) {
+                PlanOp op{};
+                op.src = lg[gi].src;
+                op.l0 = loc2st[lg[gi].q[0]] - LT;
+                op.l1 = lg[gi].nq > 1 ? loc2st[lg[gi].q[1]] - LT : -1;
+                op.kind = lg[gi].type;  // instruction type; resolved to an OpKind later
+                phase.ops.push_back(op);
+                done[gi] = 1;
+            } else {
+                still.push_back(gi);
+            }
+        }
+        pending.swap(still);
+        if (!phase.ops.empty() || first) sw.phases.push_back(std::move(phase));
+        first = false;
+    }
+    if (!pending.empty()) fail(QLX_EHIP, "sweep phase planner left gates behind");
+    // last layout must keep the low C local bits as the low work-item bits (coalesced store)
+    std::vector<int> last = regset_of(sw.phases.back().st2loc);
+    bool low_in_regs = false;
+    for (int l : last)
+        if (l < geo.c) low_in_regs = true;
+    if (low_in_regs) {
+        std::vector<int> rs;
+        for (int l = W - 1; l >= geo.c && (int)rs.size() < RB; --l) rs.push_back(l);
+        std::sort(rs.begin(), rs.end());
+        PlanPhase phase;
+        phase.st2loc = make_layout(rs);
+        sw.phases.push_back(std::move(phase));
+    }
+}
+
+// Candidate windows for the next sweep, each the low C qubits plus W - C more: (a) the
+// greedy extension of the first ready gate's operands (the classic choice), (b) every run
+// of W - C consecutive qubits. The one that runs the most pending gates wins (ties: the
+// earlier candidate). On circuits of nearest-neighbour layers, (b) lets a sweep advance a
+// band of the qubit line by several layers at once: the 30-qubit layered h/rx/cx circuit
+// of depth 20 takes 15 sweeps of a 12-qubit window instead of 21.
+static std::vector<int> choose_window_wide(const std::vector<PGate>& gates, const std::vector<int>& pending,
+                                          const std::vector<char>& done, const std::vector<std::vector<int>>& preds,
+                                          int nq, const Geometry& geo, int seed_gate) {
+    const int W = geo.w, C = std::min(geo.c, nq);
+    std::vector<std::vector<char>> cands;
+    auto base = [&] {
+        std::vector<char> s(nq, 0);
+        for (int l = 0; l < C; ++l) s[l] = 1;
+        return s;
+    };
+    {
+        std::vector<char> s = base();
+        int have = C;
+        for (int k = 0; k < gates[seed_gate].nq; ++k)
+            if (!s[gates[seed_gate].q[k]]) {
+                s[gates[seed_gate].q[k]] = 1;
+                ++have;
+            }
+        const std::vector<char> allowed(nq, 1);
+        const std::vector<int> S = choose_set(gates, pending, done, preds, nq, s, have, W, allowed);
+        std::vector<char> c(nq, 0);
+        for (int q : S) c[q] = 1;
+        cands.push_back(std::move(c));
+    }
+    for (int a = C; a + (W - C) <= nq; ++a) {
+        std::vector<char> c = base();
+        for (int q = a; q < a + (W - C); ++q) c[q] = 1;
+        cands.push_back(std::move(c));
+    }
+    int best = 0, best_n = -1;
+    std::vector<char> run(gates.size(), 0);
+    for (size_t k = 0; k < cands.size(); ++k) {
+        const int cnt = count_runnable(gates, pending, done, cands[k], run, preds);
+        if (cnt > best_n) {
+            best = (int)k;
+            best_n = cnt;
+        }
+    }
+    std::vector<int> S;
+    for (int q = 0; q < nq; ++q)
+        if (cands[best][q]) S.push_back(q);
+    return S;
+}
+
 // Candidate windows for the next sweep (see choose_window_wide), as qubit lists.
 static std::vector<std::vector<int>> window_candidates(const Walk& w, int nq, const Geometry& geo, int seed_gate) {
     const int W = geo.w, C = std::min(geo.c, nq);
@@ -556,12 +615,13 @@ static std::vector<std::vector<int>> beam_windows(const std::vector<PGate>& gate
 enum class Chooser { Greedy, Wide, Given };
 
 // Builds the plan along one walk. Windows come from the greedy chooser, the wide chooser,
-// or a given sequence (a beam search's). tail_plain: a sweep whose gates cost less than
-// one pass as plain kernels (e.g. a lone CX: half a pass) runs them as plain kernels.
+// or a given sequence (a beam search's). thorough: a sweep whose gates cost less than one
+// pass as plain kernels (e.g. a lone CX: half a pass) runs them as plain kernels, and the
+// register phases come from a beam search.
 static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& geo, const qlx_gate_instr* instrs,
                                                   size_t n, const double* pool, const qlx_matrix_entry* meta,
                                                   const qlx_exec_opts& opts, Chooser chooser,
-                                                  const std::vector<std::vector<int>>* given, bool tail_plain) {
+                                                  const std::vector<std::vector<int>>* given, bool thorough) {
     auto plan = std::make_unique<SweepPlan>();
     if ((int)nq < geo.w) fail(QLX_EINVAL, "window larger than the state");
     plan->geo = geo;
@@ -637,7 +697,7 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
         // 3. the sweep's gates, in program order
         const std::vector<int> taken = walk.take(S, (int)nq);
         if (taken.empty()) fail(QLX_EHIP, "sweep planner: empty sweep");
-        if (tail_plain) {
+        if (thorough) {
             double cost = 0.0;
             for (int gi : taken) cost += plain_passes(gates[gi].type);
             if (cost < 1.0) {
@@ -655,7 +715,7 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
         }
         PlanSweep sw;
         sw.S = S;
-        plan_phases(sw, lg, geo);
+        plan_phases(sw, lg, geo, thorough);
         const int si = (int)plan->sweeps.size();
         for (const PlanPhase& ph : sw.phases)
             for (const PlanOp& op : ph.ops) {
@@ -858,7 +918,7 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
         if (opts.specialize) {
             if (!cp.jit) cp.jit = build_jit_sweeps(p, st->dtype);
             TimedLaunch tl(st, "k_sweep_spec", pass);
-            launch_jit_sweep(*cp.jit, s.sweep_index, st, geo.w);
+            launch_jit_sweep(*cp.jit, s.sweep_index, st, geo.w, opts.sweep_workgroups);
         } else {
             TimedLaunch tl(st, "k_sweep", pass);
             const int key = (st->dtype == QLX_C128 ? 1000 : 0) + geo.w * 10 + geo.rb;
