@@ -75,6 +75,25 @@ def cpu_baseline(prog, nq, budget_s):
     }
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload`, from the rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+    summarised under profiles/ (counters scaled by the copy-kernel calibration); None when no
+    summary covers this kernel and workload."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        fields = {}
+        for line in open(path):
+            if ":" in line and not line.startswith(("|", "#", " ")):
+                k, _, v = line.partition(":")
+                fields[k.strip()] = v.strip()
+        if fields.get("kernel") == kernel and fields.get("workload") == workload:
+            try:
+                return float(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+            except (KeyError, ValueError):
+                return None
+    return None
+
+
 def run_precision(args):
     """The 32-qubit depth-30 layered circuit (seed 1003) as a complex64 state (32 GiB),
     its width-5 fused blocks (matrices formed in FP64 on the GPU) applied as MFMA
@@ -508,6 +527,9 @@ def main():
     avg_ms = tot_ms / launches
     achieved = (tot_bytes / launches) / (avg_ms / 1e3)
     unfused_pass_eq = algorithmic_bytes(prog.instrs, nq) / (2.0 * 16 * (1 << nq))
+    workload = (f"layered-{nq}q-depth{depth}-seed{args.seed}-c128-{mode}"
+                f"{'-specialised' if opts.specialize else ''}")
+    traffic = measured_traffic(name, workload) if not (args.window or args.regs) else None
     out = {
         "metric": METRIC,
         "value": round(gbps, 2),
@@ -541,7 +563,10 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "kernel": name, "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9,
-            "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4),
+            "traffic": traffic[0] if traffic else None,
+            "traffic_source": (f"{traffic[1]}: rocprofv3 FETCH_SIZE and WRITE_SIZE per launch of this kernel on "
+                               f"this workload, scaled by a copy kernel's counters" if traffic else None),
             "launches": launches, "avg_ms": round(avg_ms, 4),
             "bytes_per_launch": tot_bytes / launches,
         },

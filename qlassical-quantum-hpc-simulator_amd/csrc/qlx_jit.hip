@@ -227,7 +227,7 @@ struct Gen {
         const bool persistent = jit_persistent(p.geo, f32 ? QLX_C64 : QLX_C128);
         // persistent kernels hold two register sets: ask for at least two waves per SIMD
         line("extern \"C\" __global__ void __launch_bounds__(" + std::to_string(1 << logt) + ")" +
-             (persistent ? " __attribute__((amdgpu_waves_per_eu(2)))" : "") + " qlx_sweep_spec_" + std::to_string(si) +
+             (persistent ? " __attribute__((amdgpu_waves_per_eu(2)))" : "") + " " + jit_kernel_name(p, si) +
              "(" + T2 + "* __restrict__ amp, unsigned long long nchunks) {");
         line("  __shared__ " + T2 + " lds[" + std::to_string(1 << w) + "];");
         line("  const unsigned t = threadIdx.x;");
@@ -322,6 +322,12 @@ std::vector<char> compile(const std::string& src, const std::string& name) {
 
 }  // namespace
 
+std::string jit_kernel_name(const SweepPlan& p, int sweep_index) {
+    char b[64];
+    std::snprintf(b, sizeof b, "qlx_sweep_spec_%06llx_%d", (unsigned long long)(p.tag & 0xffffff), sweep_index);
+    return b;
+}
+
 bool jit_persistent(const Geometry& g, int dtype) {
     const size_t chunk_bytes = (size_t(dtype == QLX_C64 ? 8 : 16)) << g.w;
     return g.rb <= 4 && chunk_bytes > (size_t(64) << 10);
@@ -360,7 +366,7 @@ std::shared_ptr<JitSweeps> build_jit_sweeps(const SweepPlan& p, int dtype) {
                     s = next++;
                 }
                 try {
-                    code[s] = compile(src[s], "qlx_sweep_spec_" + std::to_string(s) + ".hip");
+                    code[s] = compile(src[s], jit_kernel_name(p, (int)s) + ".hip");
                 } catch (const Error& e) {
                     err[s] = e.msg;
                 }
@@ -374,7 +380,7 @@ std::shared_ptr<JitSweeps> build_jit_sweeps(const SweepPlan& p, int dtype) {
     j->modules.resize(n, nullptr);
     for (size_t s = 0; s < n; ++s) {
         QLX_HIP_CHECK(hipModuleLoadData(&j->modules[s], code[s].data()));
-        const std::string name = "qlx_sweep_spec_" + std::to_string(s);
+        const std::string name = jit_kernel_name(p, (int)s);
         QLX_HIP_CHECK(hipModuleGetFunction(&j->fns[s], j->modules[s], name.c_str()));
     }
     j->threads = 1u << p.geo.logt;

@@ -23,6 +23,9 @@ struct JitSweeps {
     ~JitSweeps();
 };
 
+// Kernel name of a generated sweep: qlx_sweep_spec_<program tag, 6 hex digits>_<sweep>, so
+// that traces tell the sweeps of different programs apart.
+std::string jit_kernel_name(const SweepPlan& p, int sweep_index);
 // Whether the kernels of this geometry are persistent (see the generator).
 bool jit_persistent(const Geometry& g, int dtype);
 // HIP source of sweep `sweep_index` of the plan, specialised to its descriptors.

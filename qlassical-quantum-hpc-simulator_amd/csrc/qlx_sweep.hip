@@ -822,6 +822,19 @@ static uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
     return h;
 }
 
+// Identifies a program on a state (the plan cache key; also names its generated kernels).
+static uint64_t program_key(uint32_t nq, int dtype, const qlx_gate_instr* instrs, size_t n, const double* pool,
+                            size_t pool_elems, const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts& opts) {
+    uint64_t key = 1469598103934665603ull;
+    key = fnv1a(key, instrs, n * sizeof(qlx_gate_instr));
+    key = fnv1a(key, pool, pool ? pool_elems * 16 : 0);
+    key = fnv1a(key, meta, meta ? n_meta * sizeof(qlx_matrix_entry) : 0);
+    key = fnv1a(key, &opts, sizeof(opts));
+    key = fnv1a(key, &nq, sizeof(nq));
+    key = fnv1a(key, &dtype, sizeof(dtype));
+    return key;
+}
+
 template <class A, int LOGT, int RB>
 static void launch_sweep(qlx_state* st, const char* dev, size_t off_sw, size_t off_ph, size_t off_op, size_t off_cf,
                          int si, uint32_t wg_override) {
@@ -874,16 +887,11 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
         execute_plain(st, instrs, n, pool, pool_elems, meta, n_meta, opts, stats);
         return;
     }
-    uint64_t key = 1469598103934665603ull;
-    key = fnv1a(key, instrs, n * sizeof(qlx_gate_instr));
-    key = fnv1a(key, pool, pool ? pool_elems * 16 : 0);
-    key = fnv1a(key, meta, meta ? n_meta * sizeof(qlx_matrix_entry) : 0);
-    key = fnv1a(key, &opts, sizeof(opts));
-    key = fnv1a(key, &st->nq, sizeof(st->nq));
-    key = fnv1a(key, &st->dtype, sizeof(st->dtype));
+    const uint64_t key = program_key(st->nq, st->dtype, instrs, n, pool, pool_elems, meta, n_meta, opts);
     CachedPlan& cp = plan_cache(st);
     if (!cp.plan || cp.key != key) {
         cp.plan = build_plan(st->nq, st->dtype, instrs, n, pool, meta, opts);
+        cp.plan->tag = key;
         cp.key = key;
         cp.jit.reset();
         SweepPlan& p = *cp.plan;
@@ -997,6 +1005,7 @@ extern "C" int qlx_plan_sweep_source(uint32_t num_qubits, int dtype, const qlx_g
         const Geometry geo = geometry_for(dtype, num_qubits, o);
         if ((int)num_qubits < geo.w) fail(QLX_EINVAL, "the state is smaller than one window: no sweeps");
         const auto p = build_plan(num_qubits, dtype, instrs, n, pool, meta, o);
+        p->tag = program_key(num_qubits, dtype, instrs, n, pool, pool_elems, meta, n_meta, o);
         const std::string src = jit_sweep_source(*p, (int)sweep_index, dtype);
         if (len) *len = src.size();
         if (out && cap) {

@@ -336,6 +336,7 @@ struct SweepPlan {
     std::vector<double> coef;
     uint64_t phases = 0, gates_in_sweeps = 0;
     double passes = 0.0;  // full-state passes: one per sweep, plus the plain steps' fractions
+    uint64_t tag = 0;     // hash of the planned program (names its generated kernels)
 };
 
 // Plans windowed execution of a MEASURE-free stream with an explicit geometry (W <= nq).

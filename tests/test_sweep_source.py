@@ -40,7 +40,7 @@ def test_source_structure(dtype, regs):
     assert info["sweeps"] >= 1
     for s in range(info["sweeps"]):
         src = qlx.sweep_source(prog, s, dtype)
-        assert f"qlx_sweep_spec_{s}(" in src
+        assert re.search(rf"qlx_sweep_spec_[0-9a-f]{{6}}_{s}\(", src)
         assert len(re.findall(r"= amp\[lo \| 0x[0-9a-f]+ull\];", src)) == regs
         assert len(re.findall(r"amp\[so \| 0x[0-9a-f]+ull\] = r\d+;", src)) == regs
         # gate coefficients are literals; no descriptor arrays are read
