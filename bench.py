@@ -260,24 +260,24 @@ def run_clustered(args):
                              "speedup_vs_reference": round(pub / ms, 1) if pub else None}
                 log(f"clustered: {key}: {rows[key]}")
                 assert err <= 1e-12, (key, err)
-                if n == hi and sname == "fusion":
-                    if dominant is None or ms < dominant[1]:
-                        dominant = (key, ms, timing)
+                if n == hi and (dominant is None or ms < dominant[1]):
+                    dominant = (key, ms, timing)
         st.close()
     key, ms, timing = dominant
     name, (launches, tot_ms, tot_bytes) = max(timing.items(), key=lambda kv: kv[1][1])
     avg_ms = tot_ms / launches
     achieved = (tot_bytes / launches) / (avg_ms / 1e3)
-    pub = PUBLISHED_CLUSTERED_MS.get(hi, (None, None))[1]
+    pubs = [t for t in PUBLISHED_CLUSTERED_MS.get(hi, ()) if t]
+    pub = min(pubs) if pubs else None
     out = {
-        "metric": f"clustered circuit time, {hi} qubits, fusion strategy (bench_hw_awareness)",
+        "metric": f"clustered circuit time, {hi} qubits (bench_hw_awareness)",
         "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 0,
         "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "strong",
-        "vs_baseline": round(ms / pub, 6) if pub else None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": round(ms / pub, 7) if pub else None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"clustered H/CX circuit (clusters of {cluster}, depth {depth}), {lo}-{hi} qubits, "
-                               f"complex128 from |0...0>; reported: {key}",
-                   "vs_baseline_note": "value / the reference's published Karolina (128-core CPU) time for the "
-                                       "same circuit and strategy (benchmarks/bench.txt)",
+                               f"complex128 from |0...0>; reported: the fastest row at {hi} qubits ({key})",
+                   "vs_baseline_note": "value / the reference's fastest published time for this circuit at this size "
+                                       "(benchmarks/bench.txt: the fusion-only strategy on Karolina, 128 CPU cores)",
                    "rows": rows},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4), "traffic": None,
