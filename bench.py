@@ -119,10 +119,11 @@ def run_precision(args):
     def opts_of(v):
         mode, dense = v.split(":")
         win = dict(window_qubits=args.window, register_qubits=args.regs)
+        spec = dict(specialize=not args.generic, relabel=args.relabel and not args.generic)
         if mode == "unfused":
-            return qlx.exec_opts("windowed", specialize=not args.generic, **win)
+            return qlx.exec_opts("windowed", **spec, **win)
         if mode == "windowed":
-            return qlx.exec_opts(mode, dense=dense, specialize=not args.generic, **win)
+            return qlx.exec_opts(mode, dense=dense, **spec, **win)
         return qlx.exec_opts(mode, dense=dense)
     def prog_of(v):
         return prog if v.startswith("unfused") else fused
@@ -229,7 +230,8 @@ def run_clustered(args):
         st = qlx.State(n, "c128")
         for sname, p in strategies.items():
             for mode in ("plain", "windowed"):
-                o = qlx.exec_opts(mode, specialize=(mode == "windowed" and not args.generic),
+                spec = mode == "windowed" and not args.generic
+                o = qlx.exec_opts(mode, specialize=spec, relabel=spec and args.relabel,
                                   **(dict(window_qubits=args.window, register_qubits=args.regs)
                                      if mode == "windowed" else {}))
                 st.reset()
@@ -412,6 +414,9 @@ def main():
     ap.add_argument("--regs", type=int, default=0, help="register qubits per work-item (0 = default)")
     ap.add_argument("--generic", action="store_true",
                     help="windowed: run the generic sweep kernel instead of circuit-specialised ones")
+    ap.add_argument("--in-place", action="store_true",
+                    help="specialised sweeps run in place (default: out of place through a second buffer, "
+                         "re-laying out the qubits so that every window sits on the low address bits)")
     ap.add_argument("--sweep-wg", type=int, default=0,
                     help="work-groups per window-sweep launch (0 = every resident slot)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
@@ -429,6 +434,7 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one k_copy of the whole state (HBM counter calibration)")
     args = ap.parse_args()
+    args.relabel = not args.in_place
 
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this backend has no multi-GPU path "
@@ -460,8 +466,9 @@ def main():
     st = qlx.State(nq, "c128")
     mode, opts = None, None
     for m in modes:
+        spec = m == "windowed" and not args.generic
         o = qlx.exec_opts(m, window_qubits=args.window, register_qubits=args.regs, sweep_workgroups=args.sweep_wg,
-                          specialize=(m == "windowed" and not args.generic))
+                          specialize=spec, relabel=spec and args.relabel)
         try:
             st.execute(prog.slice(0, 1), o)
             mode, opts = m, o
@@ -528,7 +535,7 @@ def main():
     achieved = (tot_bytes / launches) / (avg_ms / 1e3)
     unfused_pass_eq = algorithmic_bytes(prog.instrs, nq) / (2.0 * 16 * (1 << nq))
     workload = (f"layered-{nq}q-depth{depth}-seed{args.seed}-c128-{mode}"
-                f"{'-specialised' if opts.specialize else ''}")
+                f"{'-specialised' if opts.specialize else ''}{'-relabelled' if opts.specialize & 2 else ''}")
     traffic = measured_traffic(name, workload) if not (args.window or args.regs) else None
     out = {
         "metric": METRIC,
@@ -551,7 +558,8 @@ def main():
             "sweeps_per_step": {"full": stats["sweeps_full"], "half": stats["sweeps_half"],
                                 "quarter": stats["sweeps_quarter"]},
             "circuit_ms": round(ms, 3),
-            "sweep_kernels": "circuit-specialised (compiled at first execution)" if opts.specialize else "generic",
+            "sweep_kernels": ("circuit-specialised (compiled at first execution)" if opts.specialize else "generic") +
+                             ("; out of place, qubits re-laid out between two buffers" if opts.specialize & 2 else ""),
             "first_execution_s": round(first_s, 3),
             "generic_kernel_ms_per_step": round(generic, 3) if generic else None,
             "fused_width5_ms_per_step": round(fused_ms, 3),

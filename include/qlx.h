@@ -89,15 +89,27 @@ typedef struct qlx_exec_opts {
                                  one 128-byte cache line) */
     uint32_t gswap_mode;      /* QLX_GSWAP_PERMUTE | QLX_GSWAP_NOOP */
     uint32_t register_qubits; /* qubits per work-item's registers in a window; 0 = default
-                                 (c128: 4, c64: 5). Built (window, register) pairs:
-                                 c128 (12,4) (12,5) (11,4) (11,3); c64 (13,5) (12,5) (12,4) */
+                                 (c128: 4, c64: 5). Generic-kernel (window, register) pairs:
+                                 c128 (12,4) (12,5) (11,4) (11,3); c64 (13,5) (12,5) (12,4);
+                                 specialised sweeps take any window of up to 128 KiB with 3-5
+                                 register qubits and 64-1024 work-items */
     uint32_t dense_kernel;    /* QLX_DENSE_*: arithmetic of UNITARY / FUSED_BLOCK gates */
     uint32_t sweep_workgroups; /* window sweeps with <= 16 registers per work-item: work-groups
                                   per launch (each loops over chunks); 0 = every resident slot */
-    uint32_t specialize;      /* windowed: 1 = compile the program's sweeps into straight-line
-                                 kernels (hipRTC, once per program and state; later executions of
-                                 the same program reuse them); 0 = the generic sweep kernel */
+    uint32_t specialize;      /* windowed: QLX_SPECIALIZE_* flags; 0 = the generic sweep kernel */
 } qlx_exec_opts;
+
+/* qlx_exec_opts::specialize flags.
+ * QLX_SPECIALIZE: compile the program's sweeps into straight-line kernels (hipRTC, once per
+ *   program and state; later executions of the same program reuse them).
+ * QLX_SPECIALIZE_RELABEL (with QLX_SPECIALIZE): every sweep reads the state from one buffer
+ *   and writes it to a second one with its qubits re-laid out so that the next sweep's window
+ *   sits on the low address bits (short strides, fewer DRAM pages per chunk); the last sweep
+ *   writes the logical order back. Needs a second device buffer of the state's size (allocated
+ *   on first use, kept with the state); qlx_state_device_ptr may return the other buffer after
+ *   such an execution. */
+#define QLX_SPECIALIZE 1u
+#define QLX_SPECIALIZE_RELABEL 2u
 
 /* Dense-gate arithmetic (qlx_exec_opts::dense_kernel). The MFMA forms apply to 4- and
  * 5-qubit gates on complex64 states; every other dense gate runs on the vector units
@@ -157,7 +169,9 @@ int qlx_state_load(qlx_state* st, const double* amps_c128, size_t dim);
 int qlx_state_to_host(qlx_state* st, double* out_c128, size_t dim);
 int qlx_state_num_qubits(const qlx_state* st, uint32_t* out);
 int qlx_state_dtype(const qlx_state* st, int* out);
-/* Device pointer of the amplitude array (for zero-copy hand-off between engines). */
+/* Device pointer of the amplitude array (for zero-copy hand-off between engines). Valid until
+ * the next qlx_execute with QLX_SPECIALIZE_RELABEL, which may move the state to its second
+ * buffer. */
 int qlx_state_device_ptr(const qlx_state* st, void** out);
 /* FP64 sum |c|^2 on the GPU (reference reduction: cpu_state_vector.hpp:97-100). */
 int qlx_state_norm_sq(qlx_state* st, double* out);

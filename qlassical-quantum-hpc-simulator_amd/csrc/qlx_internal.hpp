@@ -87,6 +87,7 @@ struct qlx_state {
     int dtype = QLX_C128;
     int device = 0;
     void* amp = nullptr;          // 2^nq amplitudes (double2 or float2), HBM resident
+    void* alt = nullptr;          // second buffer of `bytes` for out-of-place sweeps (or null)
     size_t bytes = 0;
     hipStream_t stream = nullptr;
     qlx::DevBuf scratch;          // reduction partials

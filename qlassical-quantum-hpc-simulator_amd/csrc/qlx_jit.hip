@@ -202,9 +202,19 @@ struct Gen {
         }
     }
 
-    // zero bits inserted at the window's qubit positions (ascending) of chunk index `c`
-    void emit_base(const std::string& c, const std::string& out, const std::string& indent) {
+    // address bits of chunk `c` outside the window: in place, zero bits inserted at the
+    // window's qubit positions (ascending); out of place (relabel), chunk-index bit j moved to
+    // address bit cpos[j] of the load (store = false) or store layout
+    void emit_base(const std::string& c, const std::string& out, const std::string& indent, bool store = false) {
         const DevSweep& sw = p.dsw[si];
+        if (p.relabel) {
+            const std::vector<int>& cpos = store ? p.sweeps[si].st_cpos : p.sweeps[si].ld_cpos;
+            std::string e = "0ull";
+            for (size_t j = 0; j < cpos.size(); ++j)
+                e += " | (((" + c + ") >> " + std::to_string(j) + " & 1ull) << " + std::to_string(cpos[j]) + ")";
+            line(indent + "const unsigned long long " + out + " = " + e + ";");
+            return;
+        }
         line(indent + "unsigned long long " + out + " = " + c + ";");
         for (int j = 0; j < w; ++j) {
             const unsigned q = sw.S[j];
@@ -224,11 +234,16 @@ struct Gen {
         const DevSweep& sw = p.dsw[si];
         const std::string T2 = f32 ? "float2" : "double2";
         const int sh = elem == 16 ? 4 : 5;
-        const bool persistent = jit_persistent(p.geo, f32 ? QLX_C64 : QLX_C128);
+        // the persistent form is not generated for out-of-place sweeps
+        const bool persistent = jit_persistent(p.geo, f32 ? QLX_C64 : QLX_C128) && !p.relabel;
         // persistent kernels hold two register sets: ask for at least two waves per SIMD
+        // in place: one pointer; out of place (relabel): read src, write dst
+        const std::string ld = p.relabel ? "src" : "amp", st = p.relabel ? "dst" : "amp";
+        const std::string params = p.relabel ? "const " + T2 + "* __restrict__ src, " + T2 + "* __restrict__ dst"
+                                             : T2 + "* __restrict__ amp";
         line("extern \"C\" __global__ void __launch_bounds__(" + std::to_string(1 << logt) + ")" +
-             (persistent ? " __attribute__((amdgpu_waves_per_eu(2)))" : "") + " " + jit_kernel_name(p, si) +
-             "(" + T2 + "* __restrict__ amp, unsigned long long nchunks) {");
+             (persistent ? " __attribute__((amdgpu_waves_per_eu(2)))" : "") + " " + jit_kernel_name(p, si) + "(" +
+             params + ", unsigned long long nchunks) {");
         line("  __shared__ " + T2 + " lds[" + std::to_string(1 << w) + "];");
         line("  const unsigned t = threadIdx.x;");
         line("  unsigned long long c = blockIdx.x;");
@@ -240,7 +255,7 @@ struct Gen {
         map.resize(R);
         for (int i = 0; i < R; ++i) {
             map[i] = i;
-            line("  " + T2 + " r" + std::to_string(i) + " = amp[lo | " + hex64(sw.ld_roff[i]) + "];");
+            line("  " + T2 + " r" + std::to_string(i) + " = " + ld + "[lo | " + hex64(sw.ld_roff[i]) + "];");
         }
         if (persistent) {
             line("  for (;;) {");
@@ -255,7 +270,7 @@ struct Gen {
             emit_base("(more ? nc : c)", "nbase", "  ");
             line("  const unsigned long long nlo = nbase | tl;");
             for (int i = 0; i < R; ++i)
-                line("  const " + T2 + " p" + std::to_string(i) + " = amp[nlo | " + hex64(sw.ld_roff[i]) + "];");
+                line("  const " + T2 + " p" + std::to_string(i) + " = " + ld + "[nlo | " + hex64(sw.ld_roff[i]) + "];");
         }
         const DevPhase* cur = &p.dph[sw.phase_begin];
         for (uint32_t ph = 0; ph < sw.n_phases; ++ph) {
@@ -282,8 +297,13 @@ struct Gen {
             for (uint32_t k = 0; k < P.n_ops; ++k) op(p.dop[P.op_begin + k]);
         }
         flush();
-        line("  const unsigned long long so = base | ts;");
-        for (int i = 0; i < R; ++i) line("  amp[so | " + hex64(sw.st_roff[i]) + "] = " + var(i) + ";");
+        if (p.relabel) {
+            emit_base("c", "sbase", "  ", true);
+            line("  const unsigned long long so = sbase | ts;");
+        } else {
+            line("  const unsigned long long so = base | ts;");
+        }
+        for (int i = 0; i < R; ++i) line("  " + st + "[so | " + hex64(sw.st_roff[i]) + "] = " + var(i) + ";");
         if (persistent) {
             line("  if (!more) break;");
             std::string l = " ";
@@ -384,7 +404,8 @@ std::shared_ptr<JitSweeps> build_jit_sweeps(const SweepPlan& p, int dtype) {
         QLX_HIP_CHECK(hipModuleGetFunction(&j->fns[s], j->modules[s], name.c_str()));
     }
     j->threads = 1u << p.geo.logt;
-    j->persistent = jit_persistent(p.geo, dtype);
+    j->relabel = p.relabel;
+    j->persistent = jit_persistent(p.geo, dtype) && !p.relabel;
     j->resident.resize(n, 0);
     if (j->persistent) {
         int dev = 0, cus = 0;
@@ -405,9 +426,13 @@ void launch_jit_sweep(const JitSweeps& j, int sweep_index, qlx_state* st, int w,
     uint64_t grid = chunks;
     if (j.persistent) grid = std::min<uint64_t>(chunks, wg_override ? wg_override : j.resident[sweep_index]);
     void* amp = st->amp;
-    void* args[] = {&amp, &chunks};
+    void* alt = st->alt;
+    void* in_place[] = {&amp, &chunks};
+    void* out_of_place[] = {&amp, &alt, &chunks};
+    if (j.relabel && (!alt || alt == amp)) fail(QLX_EHIP, "out-of-place sweep without a second buffer");
     QLX_HIP_CHECK(hipModuleLaunchKernel(j.fns[sweep_index], (unsigned)grid, 1, 1, j.threads, 1, 1, 0, st->stream,
-                                        args, nullptr));
+                                        j.relabel ? out_of_place : in_place, nullptr));
+    if (j.relabel) std::swap(st->amp, st->alt);
 }
 
 }  // namespace qlx

@@ -16,6 +16,7 @@ struct JitSweeps {
     std::vector<uint64_t> resident;  // work-groups of each kernel the device holds at once
     unsigned threads = 0;
     bool persistent = false;         // kernels loop over chunks (grid = resident work-groups)
+    bool relabel = false;            // out-of-place sweeps: read st->amp, write st->alt, then swap
     double compile_s = 0.0;
     JitSweeps() = default;
     JitSweeps(const JitSweeps&) = delete;
@@ -35,7 +36,8 @@ std::string jit_sweep_source(const SweepPlan& p, int sweep_index, int dtype);
 std::shared_ptr<JitSweeps> build_jit_sweeps(const SweepPlan& p, int dtype);
 // One launch of a compiled sweep over the state: one work-group per chunk, or, for the
 // persistent kernels, as many work-groups as the device holds at once (wg_override: that
-// many instead, when nonzero).
+// many instead, when nonzero). Out-of-place sweeps read st->amp, write st->alt and swap the
+// two pointers.
 void launch_jit_sweep(const JitSweeps& j, int sweep_index, qlx_state* st, int window_qubits, uint32_t wg_override);
 
 }  // namespace qlx

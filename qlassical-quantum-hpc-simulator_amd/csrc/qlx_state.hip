@@ -275,6 +275,8 @@ static void alloc_storage(qlx_state* st, uint32_t nq, int dtype) {
                              std::to_string(nq) + "-qubit state failed");
     }
     if (st->amp) QLX_HIP_CHECK(hipFree(st->amp));
+    if (st->alt) QLX_HIP_CHECK(hipFree(st->alt));
+    st->alt = nullptr;
     st->amp = p;
     st->bytes = bytes;
     st->nq = nq;
@@ -292,6 +294,7 @@ static void destroy_state(qlx_state* st) {
     if (!st) return;
     if (st->stream) (void)hipStreamSynchronize(st->stream);
     if (st->amp) (void)hipFree(st->amp);
+    if (st->alt) (void)hipFree(st->alt);
     st->scratch.release();
     st->prog.release();
     st->frags.release();

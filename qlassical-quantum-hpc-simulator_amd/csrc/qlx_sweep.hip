@@ -135,8 +135,6 @@ static int operand_count(const qlx_gate_instr& g) {
     }
 }
 
-// Forward pass: which pending gates can run if exactly the qubits in `in_set` are
-// available (sweepable gates only; a gate needs all predecessors done or runnable).
 // Marks (in `run`) the pending gates one sweep over the qubit set `in_set` can apply: a
 // gate runs when it is sweepable, its qubits are in the set and no earlier pending gate on
 // any of its qubits stays behind. Scanning in program order, a gate that stays behind
@@ -804,6 +802,80 @@ std::unique_ptr<SweepPlan> build_plan_geo(uint32_t nq, const Geometry& geo, cons
     return best;
 }
 
+void assign_layouts(SweepPlan& p, uint32_t nq) {
+    const Geometry& g = p.geo;
+    const int n = (int)nq;
+    std::vector<int> pos(n);
+    for (int q = 0; q < n; ++q) pos[q] = q;
+    p.relabel = true;
+    p.step_pos.clear();
+    for (size_t s = 0; s < p.steps.size(); ++s) {
+        p.step_pos.push_back(pos);
+        if (!p.steps[s].sweep) continue;
+        const int si = p.steps[s].sweep_index;
+        PlanSweep& sw = p.sweeps[si];
+        std::vector<int> next(n);
+        if (si + 1 == (int)p.sweeps.size()) {
+            for (int q = 0; q < n; ++q) next[q] = q;
+        } else {
+            // low C qubits stay; then the next window, this window, the rest in current order
+            std::vector<char> placed(n, 0);
+            int at = 0;
+            auto place = [&](int q) {
+                if (placed[q]) return;
+                placed[q] = 1;
+                next[q] = at++;
+            };
+            for (int q = 0; q < g.c && q < n; ++q) place(q);
+            for (int q : p.sweeps[si + 1].S) place(q);
+            for (int q : sw.S) place(q);
+            std::vector<int> rest;
+            for (int q = 0; q < n; ++q)
+                if (!placed[q]) rest.push_back(q);
+            std::sort(rest.begin(), rest.end(), [&](int a, int b) { return pos[a] < pos[b]; });
+            for (int q : rest) place(q);
+        }
+        for (int q = 0; q < g.c && q < n; ++q)
+            if (pos[q] != q || next[q] != q) fail(QLX_EHIP, "qubit layout moved a coalesced qubit");
+        // chunk-index bits: the qubits outside the window, in the order of their load position
+        std::vector<char> inw(n, 0);
+        for (int q : sw.S) inw[q] = 1;
+        std::vector<int> outside;
+        for (int q = 0; q < n; ++q)
+            if (!inw[q]) outside.push_back(q);
+        std::sort(outside.begin(), outside.end(), [&](int a, int b) { return pos[a] < pos[b]; });
+        sw.ld_cpos.clear();
+        sw.st_cpos.clear();
+        for (int q : outside) {
+            sw.ld_cpos.push_back(pos[q]);
+            sw.st_cpos.push_back(next[q]);
+        }
+        sw.ld_lpos.clear();
+        sw.st_lpos.clear();
+        for (int q : sw.S) {
+            sw.ld_lpos.push_back(pos[q]);
+            sw.st_lpos.push_back(next[q]);
+        }
+        // device descriptors: address bits of the first layout's work-item and register bits
+        // through the load layout, of the last layout's through the store layout
+        DevSweep& d = p.dsw[si];
+        auto fill = [&](const PlanPhase& ph, const std::vector<int>& lpos, uint32_t* tglob, uint64_t* roff) {
+            for (int b = 0; b < g.logt; ++b) tglob[b] = (uint32_t)lpos[ph.st2loc[b]];
+            for (int i = 0; i < (1 << g.rb); ++i) {
+                uint64_t o = 0;
+                for (int b = 0; b < g.rb; ++b)
+                    if ((i >> b) & 1) o |= 1ull << lpos[ph.st2loc[g.logt + b]];
+                roff[i] = o;
+            }
+        };
+        fill(sw.phases.front(), sw.ld_lpos, d.ld_tglob, d.ld_roff);
+        fill(sw.phases.back(), sw.st_lpos, d.st_tglob, d.st_roff);
+        pos = next;
+    }
+    for (int q = 0; q < n; ++q)
+        if (pos[q] != q) fail(QLX_EHIP, "qubit layout does not end in logical order");
+}
+
 static std::unique_ptr<SweepPlan> build_plan(uint32_t nq, int dtype, const qlx_gate_instr* instrs, size_t n,
                                              const double* pool, const qlx_matrix_entry* meta,
                                              const qlx_exec_opts& opts) {
@@ -889,9 +961,11 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
     }
     const uint64_t key = program_key(st->nq, st->dtype, instrs, n, pool, pool_elems, meta, n_meta, opts);
     CachedPlan& cp = plan_cache(st);
+    const bool relabel = (opts.specialize & QLX_SPECIALIZE) && (opts.specialize & QLX_SPECIALIZE_RELABEL);
     if (!cp.plan || cp.key != key) {
         cp.plan = build_plan(st->nq, st->dtype, instrs, n, pool, meta, opts);
         cp.plan->tag = key;
+        if (relabel) assign_layouts(*cp.plan, st->nq);
         cp.key = key;
         cp.jit.reset();
         SweepPlan& p = *cp.plan;
@@ -918,9 +992,27 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
     prepare_dense(st, instrs, n, dpool, meta, n_meta, opts);
     const char* dev = (const char*)cp.dev.ptr;
     const double pass = pass_bytes(st);
-    for (const PlanStep& s : p.steps) {
+    if (p.relabel && !p.sweeps.empty() && !st->alt) {
+        if (hipMalloc(&st->alt, st->bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            st->alt = nullptr;
+            fail(QLX_ENOMEM, "out-of-place sweeps need a second buffer of " + std::to_string(st->bytes) + " bytes");
+        }
+    }
+    for (size_t k = 0; k < p.steps.size(); ++k) {
+        const PlanStep& s = p.steps[k];
         if (!s.sweep) {
-            launch_plain_instr(st, instrs[s.src], dpool, meta, opts, stats);
+            if (p.relabel) {
+                // the gate in the current qubit layout
+                qlx_gate_instr g = instrs[s.src];
+                const std::vector<int>& pos = p.step_pos[k];
+                const int na = operand_count(g);
+                for (int i = 0; i < na && i < 4; ++i) g.qubits[i] = (uint8_t)pos[g.qubits[i]];
+                if (na == 5) g.pad = (uint16_t)((g.pad & 0xFF00) | (uint16_t)pos[g.pad & 0xFF]);
+                launch_plain_instr(st, g, dpool, meta, opts, stats);
+            } else {
+                launch_plain_instr(st, instrs[s.src], dpool, meta, opts, stats);
+            }
             continue;
         }
         if (opts.specialize) {
@@ -1006,6 +1098,7 @@ extern "C" int qlx_plan_sweep_source(uint32_t num_qubits, int dtype, const qlx_g
         if ((int)num_qubits < geo.w) fail(QLX_EINVAL, "the state is smaller than one window: no sweeps");
         const auto p = build_plan(num_qubits, dtype, instrs, n, pool, meta, o);
         p->tag = program_key(num_qubits, dtype, instrs, n, pool, pool_elems, meta, n_meta, o);
+        if ((o.specialize & QLX_SPECIALIZE) && (o.specialize & QLX_SPECIALIZE_RELABEL)) assign_layouts(*p, num_qubits);
         const std::string src = jit_sweep_source(*p, (int)sweep_index, dtype);
         if (len) *len = src.size();
         if (out && cap) {

@@ -316,6 +316,9 @@ struct PlanPhase {
 struct PlanSweep {
     std::vector<int> S;  // global qubits, ascending, size W
     std::vector<PlanPhase> phases;
+    // out-of-place sweeps (SweepPlan::relabel): address bit of chunk-index bit j at the load
+    // and at the store, and of local bit l at the load and at the store
+    std::vector<int> ld_cpos, st_cpos, ld_lpos, st_lpos;
 };
 
 struct PlanStep {
@@ -337,7 +340,17 @@ struct SweepPlan {
     uint64_t phases = 0, gates_in_sweeps = 0;
     double passes = 0.0;  // full-state passes: one per sweep, plus the plain steps' fractions
     uint64_t tag = 0;     // hash of the planned program (names its generated kernels)
+    // out-of-place sweeps with per-sweep qubit layouts (assign_layouts): step_pos[s][q] is the
+    // address bit of logical qubit q before step s; the layout after the last step is the identity
+    bool relabel = false;
+    std::vector<std::vector<int>> step_pos;
 };
+
+// Gives every sweep of the plan a load layout and a store layout (the relabel option): each
+// sweep stores through a layout that puts the next sweep's window on the low address bits;
+// the low C qubits never move and the last sweep stores through the identity. Also rewrites
+// the sweeps' device descriptors (work-item and register address bits) to the new layouts.
+void assign_layouts(SweepPlan& p, uint32_t nq);
 
 // Plans windowed execution of a MEASURE-free stream with an explicit geometry (W <= nq).
 // thorough: also try the classic greedy window chooser and a beam search, let sweeps that

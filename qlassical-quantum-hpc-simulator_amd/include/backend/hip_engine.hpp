@@ -43,6 +43,12 @@ struct HIPEngineOptions {
     bool windowed = true;            // LDS/register window sweeps (else one kernel per gate)
     bool global_swap_noop = false;   // reproduce the reference's no-op GLOBAL_SWAP
     Precision precision = Precision::C128;
+    // windowed: compile each program's sweeps into straight-line kernels (seconds of hipRTC
+    // time on a program's first execution, reused afterwards: for programs run many times)
+    bool specialize = false;
+    // with specialize: out-of-place sweeps through a second buffer of the state's size that
+    // keep every sweep's window on the low address bits
+    bool relabel = false;
 };
 
 class HIPEngine final : public ExecutionEngine {
@@ -87,6 +93,7 @@ public:
         qlx_exec_opts o{};
         o.mode = opts_.windowed ? QLX_EXEC_WINDOWED : QLX_EXEC_PLAIN;
         o.gswap_mode = opts_.global_swap_noop ? QLX_GSWAP_NOOP : QLX_GSWAP_PERMUTE;
+        o.specialize = opts_.specialize ? (QLX_SPECIALIZE | (opts_.relabel ? QLX_SPECIALIZE_RELABEL : 0u)) : 0u;
         const auto* instrs = reinterpret_cast<const qlx_gate_instr*>(program.data());
         const auto* pdata = reinterpret_cast<const double*>(pool.data.data());
         const auto* meta = reinterpret_cast<const qlx_matrix_entry*>(pool.metadata.data());

@@ -25,6 +25,8 @@ QLX_EXEC_PLAIN = 0
 QLX_EXEC_WINDOWED = 1
 QLX_GSWAP_PERMUTE = 0
 QLX_GSWAP_NOOP = 1
+QLX_SPECIALIZE = 1
+QLX_SPECIALIZE_RELABEL = 2
 QLX_DENSE_AUTO = 0
 QLX_DENSE_VALU = 1
 QLX_DENSE_MFMA_BF16 = 2

@@ -34,13 +34,16 @@ DENSE = {"auto": _abi.QLX_DENSE_AUTO, "valu": _abi.QLX_DENSE_VALU, "bf16": _abi.
 
 def exec_opts(mode: str = "plain", window_qubits: int = 0, coalesce_qubits: int = 0,
               gswap: str = "permute", register_qubits: int = 0, dense: str = "auto",
-              sweep_workgroups: int = 0, specialize: bool = False) -> _abi.ExecOpts:
+              sweep_workgroups: int = 0, specialize: bool = False, relabel: bool = False) -> _abi.ExecOpts:
     """qlx_exec_opts. dense: arithmetic of 4/5-qubit dense gates on complex64 states
-    ('auto' = 'bf16' matrix cores, 'f32' matrix cores, 'valu' vector units)."""
+    ('auto' = 'bf16' matrix cores, 'f32' matrix cores, 'valu' vector units). relabel (with
+    specialize): out-of-place sweeps that re-lay out the qubits between two buffers."""
+    if relabel and not specialize:
+        raise ValueError("relabel needs specialize")
     o = _abi.ExecOpts()
     o.dense_kernel = DENSE[dense]
     o.sweep_workgroups = sweep_workgroups
-    o.specialize = int(specialize)
+    o.specialize = (_abi.QLX_SPECIALIZE if specialize else 0) | (_abi.QLX_SPECIALIZE_RELABEL if relabel else 0)
     o.mode = {"plain": _abi.QLX_EXEC_PLAIN, "windowed": _abi.QLX_EXEC_WINDOWED}[mode]
     o.window_qubits = window_qubits
     o.register_qubits = register_qubits

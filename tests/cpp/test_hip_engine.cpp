@@ -19,10 +19,13 @@ using namespace qlassical;
 using namespace qlassical::backend;
 
 static bool g_windowed = false;
+static bool g_relabel = false;
 
 static HIPEngineOptions opts() {
     HIPEngineOptions o;
     o.windowed = g_windowed;
+    o.specialize = g_relabel;
+    o.relabel = g_relabel;
     return o;
 }
 
@@ -353,6 +356,9 @@ int main(int argc, char** argv) {
     int f = mt::run_all(filter);
     std::printf("== windowed sweeps ==\n");
     g_windowed = true;
+    f += mt::run_all(filter);
+    std::printf("== specialised out-of-place sweeps ==\n");
+    g_relabel = true;
     f += mt::run_all(filter);
     return f ? 1 : 0;
 }

@@ -111,3 +111,65 @@ def test_long_runs_of_factored_gates(gpu, dtype):
     else:
         assert 1 - abs(np.vdot(want, got)) ** 2 / np.vdot(got, got).real <= 1e-6
         assert abs(np.vdot(got, got).real - 1) <= 1e-5
+
+
+def relabel_programs(qlx):
+    from test_oracle import random_program
+    rng = np.random.default_rng(77)
+    ins, pool, meta = random_program(15, 160, rng)      # CCX / 3-qubit dense gates run between sweeps
+    rnd = qlx.Program(15, ins, pool if pool is not None else np.zeros(0, complex),
+                      meta if meta is not None else np.zeros(0, qlx.META_DTYPE))
+    return [("layered18", qlx.layered(18, 10, 4321)), ("hea16", qlx.hea(16, 5)), ("qft15", qlx.qft(15)),
+            ("clustered17", qlx.clustered(17)), ("random15", rnd),
+            ("fused5", qlx.transpile_gpu(qlx.layered(16, 6, 9), max_width=5, rotations=True))]
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_relabelled_sweeps_vs_oracle(gpu, dtype):
+    """Out-of-place sweeps that re-lay out the qubits between two buffers (relabel), with
+    plain kernels between sweeps running in the current layout: the state ends in logical
+    order and matches the oracle, also on a second execution of the cached plan."""
+    o = gpu.exec_opts("windowed", specialize=True, relabel=True)
+    for name, prog in relabel_programs(gpu):
+        n = prog.num_qubits
+        init = bf.random_state(n, np.random.default_rng(n + 5))
+        pool = prog.pool if prog.pool.size else None
+        meta = prog.meta if len(prog.meta) else None
+        want, _ = oracle.execute(n, prog.instrs, pool, meta, amps=init)
+        want2, _ = oracle.execute(n, prog.instrs, pool, meta, amps=want)
+        st = gpu.State(amps=init, dtype=dtype)
+        st.timing(True)
+        st.execute(prog, o)
+        assert "k_sweep_spec" in st.timing_read(), name
+        got = st.to_host()
+        st.execute(prog, o)
+        got2 = st.to_host()
+        st.close()
+        for w, g in ((want, got), (want2, got2)):
+            if dtype == "c128":
+                assert np.abs(g - w).max() <= 1e-12, name
+            else:
+                inf = 1 - abs(np.vdot(w, g)) ** 2 / np.vdot(g, g).real
+                assert inf <= 1e-6, (name, inf)
+
+
+def test_relabel_then_in_place_on_one_state(gpu):
+    """A state that ran out-of-place sweeps (and now owns two buffers) runs in-place programs
+    and reads back correctly; its device pointer names the buffer holding the state."""
+    import ctypes as C
+    n = 16
+    p = gpu.layered(n, 8, 31)
+    init = bf.random_state(n, np.random.default_rng(2))
+    want, _ = oracle.execute(n, p.instrs, amps=init)
+    want, _ = oracle.execute(n, p.instrs, amps=want)
+    want, _ = oracle.execute(n, p.instrs, amps=want)
+    st = gpu.State(amps=init)
+    st.execute(p, gpu.exec_opts("windowed", specialize=True, relabel=True))
+    st.execute(p, gpu.exec_opts("windowed", specialize=True))
+    st.execute(p, gpu.exec_opts("plain"))
+    ptr = C.c_void_p()
+    gpu._abi.check(gpu._abi.lib().qlx_state_device_ptr(st.handle, C.byref(ptr)))
+    assert ptr.value
+    got = st.to_host()
+    st.close()
+    assert np.abs(got - want).max() <= 1e-12

@@ -104,3 +104,55 @@ def test_wider_windows_for_specialised_sweeps():
     with pytest.raises(qlx.QlxUnsupported):       # 256 KiB of LDS
         qlx.plan_windowed(prog, "c128", qlx.exec_opts("windowed", window_qubits=14, register_qubits=5,
                                                       specialize=True))
+
+
+def _bits(expr):
+    """Address bit positions an expression of the generated source sets: `<< k` terms of the
+    work-item / chunk parts, hexadecimal offsets of the registers."""
+    return [int(k) for k in re.findall(r"<< (\d+)\)", expr)]
+
+
+@pytest.mark.parametrize("prog_name", ["layered", "hea", "random"])
+def test_relabelled_sweeps_address_every_amplitude_once(prog_name):
+    """Out-of-place sweeps (relabel): in every generated kernel the load address bits (chunk
+    index bits, work-item bits, register bits) and the store address bits each form a
+    permutation of the state's qubits, so loads and stores are in bounds and bijective; the
+    kernel reads `src` and writes `dst`."""
+    from test_oracle import random_program
+    n = 17
+    if prog_name == "layered":
+        prog = qlx.layered(n, 8, 5)
+    elif prog_name == "hea":
+        prog = qlx.hea(n, 4)
+    else:
+        ins, pool, meta = random_program(n, 150, np.random.default_rng(3))
+        prog = qlx.Program(n, ins, pool if pool is not None else np.zeros(0, complex),
+                           meta if meta is not None else np.zeros(0, qlx.META_DTYPE))
+    o = qlx.exec_opts("windowed", specialize=True, relabel=True)
+    _, _, info = qlx.plan_windowed(prog, "c128", o)
+    assert info["sweeps"] >= 2
+    for s in range(info["sweeps"]):
+        src = qlx.sweep_source(prog, s, "c128", o)
+        assert "const double2* __restrict__ src, double2* __restrict__ dst" in src
+        assert "amp[" not in src
+        for base, thr, ptr, idx in (("base", "tl", "src", "lo"), ("sbase", "ts", "dst", "so")):
+            chunk = _bits(re.search(rf"const unsigned long long {base} = ([^;]+);", src).group(1))
+            work = _bits(re.search(rf"const unsigned long long {thr} = ([^;]+);", src).group(1))
+            offs = [int(h, 16) for h in re.findall(rf"{ptr}\[{idx} \| (0x[0-9a-f]+)ull\]", src)]
+            assert len(offs) == 16 and len(set(offs)) == 16
+            regs = [b for b in range(n) if any((o_ >> b) & 1 for o_ in offs)]
+            # register offsets are all combinations of 4 address bits
+            assert len(regs) == 4 and sorted(offs) == sorted(
+                sum(1 << regs[k] for k in range(4) if (i >> k) & 1) for i in range(16))
+            assert sorted(chunk + work + regs) == list(range(n)), (s, base)
+        # the low 3 qubits are the low work-item bits at both ends (128-byte runs)
+        assert re.search(r"tl = 0ull \| \(\(unsigned long long\)\(\(t >> 0\) & 1u\) << 0\) \| \(\(unsigned long long\)\(\(t >> 1\) & 1u\) << 1\) \| \(\(unsigned long long\)\(\(t >> 2\) & 1u\) << 2\)", src)
+
+
+@pytest.mark.skipif(not os.path.exists(HIPRTC), reason="hipRTC not installed")
+def test_relabelled_sources_compile_for_gfx950():
+    prog = qlx.layered(16, 6, 3)
+    o = qlx.exec_opts("windowed", specialize=True, relabel=True)
+    for dtype in ("c128", "c64"):
+        rc, log = compile_ok(qlx.sweep_source(prog, 0, dtype, o))
+        assert rc == 0, log[-2000:]
