@@ -269,6 +269,7 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
         if (b->circs.empty()) fail(QLX_EINVAL, "empty batch");
         qlx_exec_opts o{};
         if (opts) o = *opts;
+        o.specialize = 0;  // batched sweeps run the generic kernel's descriptors
         o.gswap_mode = QLX_GSWAP_PERMUTE;
         const size_t amp = amp_size(b->dtype);
         // plan every circuit, lay out the arena and the device pool

@@ -7,18 +7,9 @@
 // qubit position, LDS offset and gate coefficient a compile-time constant, CX/SWAP
 // reduced to renaming registers in the generator — and compiled once for gfx950.
 // The generated code follows the generic kernel operation by operation (the same
-// descriptors: DevSweep / DevPhase / DevOp and the coefficient array), applying the same
-// gates in the same order. Two gates are computed with half the floating-point work:
-// every gate is linear and H and RX scale every amplitude of the chunk alike, so their
-// common real factor is taken out and applied once, later:
-//   H  = h [[1, 1], [1, -1]]                        -> butterfly (adds only), factor h
-//   RX = c [[1, -i t], [-i t, 1]],  t = s / c       -> two FMAs per amplitude, factor c
-//      = s [[u, -i], [-i, u]],      u = c / s       -> (when |s| > |c|), factor s
-// (c = cos(theta/2), s = sin(theta/2)). The generator multiplies the pending factors
-// together on the host and scales the registers by their product before the stores,
-// or earlier once it falls below 2^-30 (the registers then hold at most 2^30 times the
-// amplitudes' magnitude; |factor| >= 1/sqrt 2 per gate). The results differ from the
-// generic kernel's by rounding only.
+// descriptors: DevSweep / DevPhase / DevOp and the coefficient array, including the
+// factored H / RX forms and the OP_SCALE steps the planner writes), so both compute the
+// same gates in the same order with the same expressions.
 
 #include <hip/hiprtc.h>
 
@@ -64,24 +55,6 @@ struct Gen {
     std::string out;
     std::vector<int> map;  // register index -> variable
     int tmp = 0;
-    double scale = 1.0;    // real factor taken out of the gates so far, not yet applied
-
-    // multiplies every register by the pending factor
-    void flush() {
-        if (scale == 1.0) return;
-        const std::string T = f32 ? "float" : "double";
-        line("  { const " + T + " k = " + lit(scale, f32) + ";");
-        std::string l = "   ";
-        for (int i = 0; i < R; ++i) l += " " + var(i) + ".x *= k; " + var(i) + ".y *= k;";
-        line(l);
-        line("  }");
-        scale = 1.0;
-    }
-    void defer(double factor) {
-        scale *= factor;
-        if (std::fabs(scale) < 0x1p-30) flush();
-    }
-
     void line(const std::string& s) { out += s; out += '\n'; }
     std::string var(int i) const { return "r" + std::to_string(map[i]); }
 
@@ -106,37 +79,36 @@ struct Gen {
         const double* m = p.coef.data() + o.coef;
         const std::string id = std::to_string(tmp++);
         switch (kind) {
-            case OP_RX: {
-                // a0' = c a0 - i s a1, a1' = -i s a0 + c a1 (c = c0, s = c1)
-                const double c = o.c0, s = o.c1;
+            case OP_RXT: {
+                // RX / c: a0' = a0 - i t a1, a1' = a1 - i t a0
                 const std::string F = f32 ? "__builtin_fmaf" : "__builtin_fma";
-                if (std::fabs(c) >= std::fabs(s)) {
-                    line("  { const " + T + " t = " + lit(s / c, f32) + ";");
-                    for (int i = 0; i < R; ++i) {
-                        if (i & (1 << b0)) continue;
-                        const std::string x = var(i), y = var(i | (1 << b0));
-                        line("    { const " + T2 + " a0 = " + x + ", a1 = " + y + "; " + x + " = " + T2 + "{" + F +
-                             "(t, a1.y, a0.x), " + F + "(-t, a1.x, a0.y)}; " + y + " = " + T2 + "{" + F +
-                             "(t, a0.y, a1.x), " + F + "(-t, a0.x, a1.y)}; }");
-                    }
-                    line("  }");
-                    defer(c);
-                } else {
-                    line("  { const " + T + " u = " + lit(c / s, f32) + ";");
-                    for (int i = 0; i < R; ++i) {
-                        if (i & (1 << b0)) continue;
-                        const std::string x = var(i), y = var(i | (1 << b0));
-                        line("    { const " + T2 + " a0 = " + x + ", a1 = " + y + "; " + x + " = " + T2 + "{" + F +
-                             "(u, a0.x, a1.y), " + F + "(u, a0.y, -a1.x)}; " + y + " = " + T2 + "{" + F +
-                             "(u, a1.x, a0.y), " + F + "(u, a1.y, -a0.x)}; }");
-                    }
-                    line("  }");
-                    defer(s);
+                line("  { const " + T + " t = " + lit(o.c0, f32) + ";");
+                for (int i = 0; i < R; ++i) {
+                    if (i & (1 << b0)) continue;
+                    const std::string x = var(i), y = var(i | (1 << b0));
+                    line("    { const " + T2 + " a0 = " + x + ", a1 = " + y + "; " + x + " = " + T2 + "{" + F +
+                         "(t, a1.y, a0.x), " + F + "(-t, a1.x, a0.y)}; " + y + " = " + T2 + "{" + F +
+                         "(t, a0.y, a1.x), " + F + "(-t, a0.x, a1.y)}; }");
                 }
+                line("  }");
                 break;
             }
-            case OP_H: {
-                // h [[1, 1], [1, -1]]: the butterfly now, the factor h later
+            case OP_RXU: {
+                // RX / s: a0' = u a0 - i a1, a1' = u a1 - i a0
+                const std::string F = f32 ? "__builtin_fmaf" : "__builtin_fma";
+                line("  { const " + T + " u = " + lit(o.c0, f32) + ";");
+                for (int i = 0; i < R; ++i) {
+                    if (i & (1 << b0)) continue;
+                    const std::string x = var(i), y = var(i | (1 << b0));
+                    line("    { const " + T2 + " a0 = " + x + ", a1 = " + y + "; " + x + " = " + T2 + "{" + F +
+                         "(u, a0.x, a1.y), " + F + "(u, a0.y, -a1.x)}; " + y + " = " + T2 + "{" + F +
+                         "(u, a1.x, a0.y), " + F + "(u, a1.y, -a0.x)}; }");
+                }
+                line("  }");
+                break;
+            }
+            case OP_HB: {
+                // H without its factor: the butterfly
                 line("  {");
                 for (int i = 0; i < R; ++i) {
                     if (i & (1 << b0)) continue;
@@ -145,7 +117,14 @@ struct Gen {
                          "{a0.x + a1.x, a0.y + a1.y}; " + y + " = " + T2 + "{a0.x - a1.x, a0.y - a1.y}; }");
                 }
                 line("  }");
-                defer(o.c0);
+                break;
+            }
+            case OP_SCALE: {
+                line("  { const " + T + " k = " + lit(o.c0, f32) + ";");
+                std::string l = "   ";
+                for (int i = 0; i < R; ++i) l += " " + var(i) + ".x *= k; " + var(i) + ".y *= k;";
+                line(l);
+                line("  }");
                 break;
             }
             case OP_U2: {
@@ -296,7 +275,6 @@ struct Gen {
             cur = &P;
             for (uint32_t k = 0; k < P.n_ops; ++k) op(p.dop[P.op_begin + k]);
         }
-        flush();
         if (p.relabel) {
             emit_base("c", "sbase", "  ", true);
             line("  const unsigned long long so = sbase | ts;");

@@ -93,10 +93,10 @@ static Geometry geometry_for(int dtype, uint32_t nq, const qlx_exec_opts& o) {
     Geometry g{};
     g.w = o.window_qubits ? (int)o.window_qubits : (dtype == QLX_C128 ? 12 : 13);
     g.rb = o.register_qubits ? (int)o.register_qubits : (dtype == QLX_C128 ? 4 : 5);
-    if (!geometry_built(dtype, g.w, g.rb, o.specialize != 0))
+    if (!geometry_built(dtype, g.w, g.rb, (o.specialize & QLX_SPECIALIZE) != 0))
         fail(QLX_EUNSUPPORTED, "window " + std::to_string(g.w) + " with " + std::to_string(g.rb) +
                                    " register qubits is not built for this precision" +
-                                   (o.specialize ? "" : " (specialised sweeps take more windows)"));
+                                   ((o.specialize & QLX_SPECIALIZE) ? "" : " (specialised sweeps take more windows)"));
     g.logt = g.w - g.rb;
     // low qubits kept in the window: one 128-byte cache line of amplitudes (8 x 16 B, 16 x 8 B)
     g.c = o.coalesce_qubits ? (int)o.coalesce_qubits : (dtype == QLX_C64 ? 4 : 3);
@@ -727,7 +727,8 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
         plan->steps.push_back(PlanStep{true, -1, si});
     }
 
-    // device image
+    // device image (factored H / RX ops for the circuit-specialised sweeps only; see OpKind)
+    const bool factored = (opts.specialize & QLX_SPECIALIZE) != 0;
     for (const PlanSweep& sw : plan->sweeps) {
         DevSweep d{};
         d.n_phases = (uint32_t)sw.phases.size();
@@ -746,12 +747,20 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
         fill_ends(sw.phases.front(), d.ld_tglob, d.ld_roff);
         fill_ends(sw.phases.back(), d.st_tglob, d.st_roff);
         plan->dsw.push_back(d);
+        // pending real factor of the factored H / RX ops (see OpKind); applied by OP_SCALE
+        double scale = 1.0;
+        auto scale_op = [&] {
+            DevOp o{};
+            o.key = (uint32_t)OP_SCALE << 8;
+            o.c0 = scale;
+            plan->dop.push_back(o);
+            scale = 1.0;
+        };
         for (size_t p = 0; p < sw.phases.size(); ++p) {
             const PlanPhase& ph = sw.phases[p];
             DevPhase dp{};
             dp.remap = p > 0 ? 1u : 0u;
             dp.op_begin = (uint32_t)plan->dop.size();
-            dp.n_ops = (uint32_t)ph.ops.size();
             for (int b = 0; b < geo.logt; ++b) dp.tlocal[b] = (uint32_t)ph.st2loc[b];
             for (int i = 0; i < (1 << geo.rb); ++i) {
                 uint32_t o = 0;
@@ -769,20 +778,42 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
                 if (two && (op.l1 < 0 || op.l0 == op.l1)) fail(QLX_EHIP, "sweep planner: bad two-qubit operands");
                 if (op.l0 < 0 || op.l0 >= geo.rb || (two && op.l1 >= geo.rb))
                     fail(QLX_EHIP, "sweep planner: operand is not a register bit");
-                o.key = (uint32_t)((kind << 8) | (op.l0 << 4) | (two ? op.l1 : 0));
-                if (kind == OP_RX || kind == OP_H) {
+                double factor = 1.0;
+                if (kind == OP_RX && !factored) {
                     o.c0 = plan->coef[o.coef];
-                    o.c1 = kind == OP_RX ? plan->coef[o.coef + 1] : 0.0;
+                    o.c1 = plan->coef[o.coef + 1];
+                } else if (kind == OP_H && !factored) {
+                    o.c0 = plan->coef[o.coef];
+                } else if (kind == OP_RX) {
+                    const double c = plan->coef[o.coef], sn = plan->coef[o.coef + 1];
+                    if (std::fabs(c) >= std::fabs(sn)) {
+                        kind = OP_RXT;
+                        o.c0 = sn / c;
+                        factor = c;
+                    } else {
+                        kind = OP_RXU;
+                        o.c0 = c / sn;
+                        factor = sn;
+                    }
+                } else if (kind == OP_H) {
+                    kind = OP_HB;
+                    factor = plan->coef[o.coef];
                 }
+                o.key = (uint32_t)((kind << 8) | (op.l0 << 4) | (two ? op.l1 : 0));
                 plan->dop.push_back(o);
+                scale *= factor;
+                if (std::fabs(scale) < 0x1p-30) scale_op();
             }
+            // the last phase applies what is left of the factor before the store
+            if (p + 1 == sw.phases.size() && scale != 1.0) scale_op();
+            plan->dph.back().n_ops = (uint32_t)(plan->dop.size() - dp.op_begin);
         }
     }
     return plan;
 }
 
 // Plans with the wider window search. `thorough` also plans with the classic greedy
-// chooser alone and with a beam search over window sequences (width 4), runs sweeps that
+// chooser alone and with a beam search over window sequences (width 16), runs sweeps that
 // would cost less than one pass as plain kernels, and keeps the plan with the fewest
 // passes (then fewest phases). Over 3,198
 // layered, HEA, QFT and clustered circuits of 13-28 qubits (complex64, 13-qubit window)
@@ -1015,7 +1046,7 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
             }
             continue;
         }
-        if (opts.specialize) {
+        if (opts.specialize & QLX_SPECIALIZE) {
             if (!cp.jit) cp.jit = build_jit_sweeps(p, st->dtype);
             TimedLaunch tl(st, "k_sweep_spec", pass);
             launch_jit_sweep(*cp.jit, s.sweep_index, st, geo.w, opts.sweep_workgroups);
@@ -1090,6 +1121,7 @@ extern "C" int qlx_plan_sweep_source(uint32_t num_qubits, int dtype, const qlx_g
         if (n && !instrs) fail(QLX_EINVAL, "null instruction stream");
         qlx_exec_opts o{};
         if (opts) o = *opts;
+        o.specialize |= QLX_SPECIALIZE;  // the source is a specialised sweep's
         for (size_t i = 0; i < n; ++i) {
             if (instrs[i].type == QLX_MEASURE) fail(QLX_EINVAL, "plans cover MEASURE-free streams");
             validate_instr(instrs[i], num_qubits, meta, n_meta, pool_elems);
@@ -1098,7 +1130,7 @@ extern "C" int qlx_plan_sweep_source(uint32_t num_qubits, int dtype, const qlx_g
         if ((int)num_qubits < geo.w) fail(QLX_EINVAL, "the state is smaller than one window: no sweeps");
         const auto p = build_plan(num_qubits, dtype, instrs, n, pool, meta, o);
         p->tag = program_key(num_qubits, dtype, instrs, n, pool, pool_elems, meta, n_meta, o);
-        if ((o.specialize & QLX_SPECIALIZE) && (o.specialize & QLX_SPECIALIZE_RELABEL)) assign_layouts(*p, num_qubits);
+        if (o.specialize & QLX_SPECIALIZE_RELABEL) assign_layouts(*p, num_qubits);
         const std::string src = jit_sweep_source(*p, (int)sweep_index, dtype);
         if (len) *len = src.size();
         if (out && cap) {

@@ -14,7 +14,21 @@ constexpr int kMaxW = 14;
 constexpr int kMaxLogT = 10;
 constexpr int kMaxR = 32;
 
-enum OpKind : uint8_t { OP_U2 = 1, OP_RX = 2, OP_H = 3, OP_CX = 4, OP_SWAP = 5, OP_U4 = 6 };
+// For the circuit-specialised sweeps the planner rewrites OP_RX and OP_H into factored forms
+// (the generic kernel keeps OP_RX / OP_H: its cost is the runtime op dispatch, and the extra
+// cases measured slower there). Every gate is linear and H and RX scale every amplitude of the
+// chunk by the same real factor, so the factor is taken out and applied once:
+//   H  = h [[1, 1], [1, -1]]                  -> OP_HB  butterfly (adds only), factor h
+//   RX = c [[1, -i t], [-i t, 1]], t = s / c  -> OP_RXT two FMAs per amplitude, factor c
+//      = s [[u, -i], [-i, u]],     u = c / s  -> OP_RXU (when |s| > |c|), factor s
+// (c = cos(theta/2), s = sin(theta/2)); OP_SCALE multiplies every register by the product of
+// the pending factors, at the end of each sweep or as soon as the product falls below 2^-30
+// (each factor is at least 1/sqrt 2 in magnitude, so registers stay within 2^30 of the
+// amplitudes' magnitude, also in FP32).
+enum OpKind : uint8_t {
+    OP_U2 = 1, OP_RX = 2, OP_H = 3, OP_CX = 4, OP_SWAP = 5, OP_U4 = 6,
+    OP_RXT = 7, OP_RXU = 8, OP_HB = 9, OP_SCALE = 10
+};
 
 // 32-bit fields only: the descriptor is read with scalar loads (a 16-bit field would be
 // fetched with a vector load, and the wait for it would also wait for every HBM load
@@ -22,7 +36,8 @@ enum OpKind : uint8_t { OP_U2 = 1, OP_RX = 2, OP_H = 3, OP_CX = 4, OP_SWAP = 5, 
 struct DevOp {
     uint32_t key;   // (kind << 8) | (register bit 0 << 4) | register bit 1
     uint32_t coef;  // index into the coefficient array (doubles): U2/U4 matrices
-    double c0, c1;  // inline coefficients: RX (cos, sin), H (1/sqrt 2)
+    double c0, c1;  // inline coefficients: RX (cos, sin), H (1/sqrt 2); OP_RXT t, OP_RXU u,
+                    // OP_SCALE the factor
 };
 static_assert(sizeof(DevOp) == 24, "DevOp layout");
 
