@@ -97,7 +97,8 @@ def measured_traffic(kernel, workload):
 def run_precision(args):
     """The 32-qubit depth-30 layered circuit (seed 1003) as a complex64 state (32 GiB),
     its width-5 fused blocks (matrices formed in FP64 on the GPU) applied as MFMA
-    contractions, checked against the complex128 state (64 GiB) of the unfused circuit
+    contractions inside 13-qubit LDS windows (the reported variant; the others are timed
+    beside it), checked against the complex128 state (64 GiB) of the unfused circuit
     run on the GPU in FP64 (windowed path, pinned to the CPU oracle at smaller sizes):
     max |amplitude error| and infidelity 1 - |<psi64|psi128>|^2, FP64 reductions on the GPU."""
     import numpy as np
@@ -112,8 +113,9 @@ def run_precision(args):
     kinds = {int(k): int(v) for k, v in zip(*np.unique(fused.meta["num_qubits"], return_counts=True))}
     log(f"precision: {nq}q depth {depth}: {len(prog)} gates -> {len(fused)} instructions "
         f"(fused widths {kinds}), fusion {t_fuse:.2f} s")
-    # "mode:dense": the dense gates' arithmetic, the other gates one kernel each (plain) or
-    # grouped into window sweeps between the dense passes (windowed)
+    # "mode:dense": the dense gates' arithmetic; "dense:*" runs every gate inside 13-qubit LDS
+    # windows (4/5-qubit gates on the matrix cores), "windowed:*" runs the dense gates as their
+    # own passes between window sweeps of the other gates, "plain:*" one kernel per gate
     # "unfused:windowed" runs the unfused circuit in window sweeps (no dense gates), for comparison
     variants = [v for v in args.dense_variants.split(",") if v]
     def opts_of(v):
@@ -124,7 +126,7 @@ def run_precision(args):
             return qlx.exec_opts("windowed", **spec, **win)
         if mode == "windowed":
             return qlx.exec_opts(mode, dense=dense, **spec, **win)
-        return qlx.exec_opts(mode, dense=dense)
+        return qlx.exec_opts(mode, dense=dense)   # plain, or dense sweeps
     def prog_of(v):
         return prog if v.startswith("unfused") else fused
     res = {}
@@ -166,7 +168,7 @@ def run_precision(args):
     main_v = variants[0]
     r = res[main_v]
     gbps = r["bytes"] / (r["ms"] / 1e3) / 1e9
-    name = "k_dense_mfma" if "k_dense_mfma" in r["timing"] else max(r["timing"].items(), key=lambda kv: kv[1][1])[0]
+    name = max(r["timing"].items(), key=lambda kv: kv[1][1])[0]
     launches, tot_ms, tot_bytes = r["timing"][name]
     avg_ms = tot_ms / launches
     achieved = (tot_bytes / launches) / (avg_ms / 1e3)
@@ -423,7 +425,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", choices=["circuit30", "precision", "batch", "serve", "clustered"],
                     default="circuit30")
-    ap.add_argument("--dense-variants", default="windowed:bf16,plain:bf16,windowed:f32,windowed:valu,unfused:windowed",
+    ap.add_argument("--dense-variants", default="dense:bf16,windowed:bf16,dense:f32,windowed:valu,unfused:windowed",
                     help="precision config: mode:dense-arithmetic forms to time (the first one is reported)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)

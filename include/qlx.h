@@ -75,6 +75,9 @@ enum {
 /* ---- execution options --------------------------------------------------- */
 #define QLX_EXEC_PLAIN    0  /* one HBM sweep per instruction (pair/quad/dense kernels) */
 #define QLX_EXEC_WINDOWED 1  /* gates grouped into LDS/register window sweeps */
+#define QLX_EXEC_DENSE_SWEEPS 2  /* complex64: 13-qubit windows held in LDS; every gate of a window
+                                    applied there as a dense matrix, 4/5-qubit UNITARY / FUSED_BLOCK
+                                    gates on the matrix cores (dense_kernel) — for fused programs */
 
 /* GLOBAL_SWAP semantics. The reference engine treats GLOBAL_SWAP as a no-op
  * (include/backend/cpu_openmp_engine.hpp:217-219); the GPU backend applies it
@@ -83,7 +86,7 @@ enum {
 #define QLX_GSWAP_NOOP    1
 
 typedef struct qlx_exec_opts {
-    uint32_t mode;            /* QLX_EXEC_PLAIN | QLX_EXEC_WINDOWED */
+    uint32_t mode;            /* QLX_EXEC_PLAIN | QLX_EXEC_WINDOWED | QLX_EXEC_DENSE_SWEEPS */
     uint32_t window_qubits;   /* 0 = default for the precision (c128: 12, c64: 13) */
     uint32_t coalesce_qubits; /* low qubits always kept in the window; 0 = default (c128: 3, c64: 4:
                                  one 128-byte cache line) */
@@ -194,10 +197,11 @@ int qlx_execute(qlx_state* st,
                 const double* pool_c128, size_t pool_elems,
                 const qlx_matrix_entry* meta, size_t n_meta,
                 const qlx_exec_opts* opts, size_t* stop_at);
-/* Host-only planning of QLX_EXEC_WINDOWED execution (no GPU work): the order in which
- * the instructions of a MEASURE-free stream would run. order[k] is the source index of
- * the k-th executed instruction and sweep_of[k] the window sweep that runs it (-1: its
- * own kernel). Gates are only reordered past gates on disjoint qubits. */
+/* Host-only planning of QLX_EXEC_WINDOWED (or, with opts->mode = QLX_EXEC_DENSE_SWEEPS, of
+ * dense-sweep) execution (no GPU work): the order in which the instructions of a
+ * MEASURE-free stream would run. order[k] is the source index of the k-th executed
+ * instruction and sweep_of[k] the window sweep that runs it (-1: its own kernel). Gates
+ * are only reordered past gates on disjoint qubits. */
 typedef struct qlx_plan_info {
     uint64_t sweeps;           /* window-sweep launches (one full HBM pass each) */
     uint64_t phases;           /* register phases over all sweeps (LDS re-layouts = phases - sweeps) */

@@ -1,0 +1,450 @@
+// qlx_dsweep.hip — dense window sweeps (QLX_EXEC_DENSE_SWEEPS) for complex64 states.
+//
+// A fused program (pass_gate_fusion with width 5, reference include/middleware/
+// middleware.hpp:526-753) is mostly dense 4- and 5-qubit unitaries; applied one HBM pass
+// per gate (UNITARY / FUSED_BLOCK of the reference's apply_unitary,
+// include/backend/cpu_openmp_engine.hpp:539-606) the pass count equals the gate count.
+// Here each work-group loads one chunk of 2^13 amplitudes (a 13-qubit window that holds
+// the low 4 qubits, so every HBM access is a 128-byte run) into LDS, applies every gate
+// of its sweep to the chunk in LDS, and stores it: one HBM pass per window instead of one
+// per gate.
+//   * 4/5-qubit gates run on the matrix cores with the same arithmetic as k_dense_mfma
+//     (real-embedded GEMM per wave over 32 groups; BF16 split operands accumulated in
+//     FP32, or FP32 operands), the amplitudes read from and written back to LDS;
+//   * every other gate (1-3 qubits) is a dense complex matrix in FP32 applied by the
+//     work-items, one group of 2^k amplitudes at a time.
+// With dense_kernel = QLX_DENSE_VALU the program runs as QLX_EXEC_WINDOWED instead.
+// Gates of one sweep run in program order with a barrier between them; within a gate the
+// groups are disjoint, and a wave (MFMA) or work-item (vector) reads its groups
+// completely before writing them.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "qlx_internal.hpp"
+#include "qlx_sweep_core.hpp"
+
+namespace qlx {
+namespace {
+
+constexpr int kDW = 13;              // window qubits
+constexpr int kDC = 4;               // low qubits in every window (128-byte runs of float2)
+constexpr int kDThreads = 256;
+constexpr int kDChunk = 1 << kDW;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+struct DenseSweepDesc {
+    uint32_t S[kDW];       // window qubits ascending (local bit l <-> global qubit S[l])
+    uint32_t op_begin, n_ops;
+    uint32_t pad[1];
+};
+static_assert(sizeof(DenseSweepDesc) % 4 == 0, "descriptor layout");
+
+enum : uint32_t { DOP_VALU = 0, DOP_MFMA = 1 };
+
+struct DenseOpDesc {
+    uint32_t kind;          // DOP_VALU | DOP_MFMA
+    uint32_t k;             // gate qubits (1..5)
+    uint32_t mat;           // DOP_VALU: first complex element of the FP32 matrix
+    uint32_t pad0;
+    uint64_t frag_off;      // DOP_MFMA: byte offset of the A fragments
+    uint32_t bits[5];       // local bit of gate qubit i (matrix index bit i)
+    uint32_t sorted[5];     // the same, ascending (group base by zero-bit insertion)
+    uint32_t in_off[16];    // DOP_MFMA: local offset of group amplitude r < 2^(k-1)
+    uint32_t out_off[16];   // DOP_MFMA: local offset of output slot 16m + 4a + b
+    uint32_t hbit_in, hbit_out;
+};
+
+__device__ __forceinline__ uint32_t ins0_32(uint32_t idx, uint32_t t) {
+    const uint32_t lo = idx & ((1u << t) - 1);
+    return lo | ((idx ^ lo) << 1);
+}
+
+// One 4/5-qubit gate on the chunk in LDS, on the matrix cores. Wave w takes tiles w, w+4, ...
+// of 32 groups; lane (h, j) holds group j's amplitudes [h 2^(K-1), (h+1) 2^(K-1)).
+template <int K, bool F32>
+__device__ void mfma_gate(float2* __restrict__ lds, const DenseOpDesc& op, const unsigned char* __restrict__ frags) {
+    constexpr int D = 1 << K, HALF = D / 2, MT = (2 * D) / 32;
+    constexpr int TB = (2 * D) / 16, TF = D;
+    constexpr int NB = MT * TB * 64;
+    constexpr int NTILES = (kDChunk >> K) / 32;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int h = lane >> 5, j = lane & 31;
+    const uint32_t hin = h ? op.hbit_in : 0u, hout = h ? op.hbit_out : 0u;
+    const unsigned char* fr = frags + op.frag_off;
+    // the gate's A fragments, once per wave (the wave then takes 2 (K = 5) or 4 (K = 4) tiles)
+    bf16x8 ah[F32 ? 1 : MT][F32 ? 1 : TB], am[F32 ? 1 : MT][F32 ? 1 : TB], al[F32 ? 1 : MT][F32 ? 1 : TB];
+    float af[F32 ? MT : 1][F32 ? TF : 1];
+    if constexpr (F32) {
+        const float* f = (const float*)fr;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int t = 0; t < TF; ++t) af[m][t] = f[(m * TF + t) * 64 + lane];
+    } else {
+        const bf16x8* fh = (const bf16x8*)fr;
+        const bf16x8* fm = fh + NB;
+        const bf16x8* fl = fm + NB;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int t = 0; t < TB; ++t) {
+                ah[m][t] = fh[(m * TB + t) * 64 + lane];
+                am[m][t] = fm[(m * TB + t) * 64 + lane];
+                al[m][t] = fl[(m * TB + t) * 64 + lane];
+            }
+    }
+    for (int tile = wave; tile < NTILES; tile += kDThreads / 64) {
+        uint32_t base = (uint32_t)tile * 32 + j;
+#pragma unroll
+        for (int q = 0; q < K; ++q) base = ins0_32(base, op.sorted[q]);
+        float2 x[HALF];
+#pragma unroll
+        for (int r = 0; r < HALF; ++r) x[r] = lds[base + hin + op.in_off[r]];
+        f32x16 acc[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[m][v] = 0.f;
+        if constexpr (F32) {
+#pragma unroll
+            for (int t = 0; t < TF; ++t) {
+                const float xv = (t & 1) ? x[t >> 1].y : x[t >> 1].x;
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[m][t], xv, acc[m], 0, 0, 0);
+            }
+        } else {
+            bf16x8 xh[TB], xl[TB];
+#pragma unroll
+            for (int t = 0; t < TB; ++t)
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    const float v = (s & 1) ? x[4 * t + (s >> 1)].y : x[4 * t + (s >> 1)].x;
+                    const __bf16 hv = (__bf16)v;
+                    xh[t][s] = hv;
+                    xl[t][s] = (__bf16)(v - (float)hv);
+                }
+            // small terms first (as k_dense_mfma)
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[m][t], xh[t], acc[m], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am[m][t], xh[t], acc[m], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m][t], xl[t], acc[m], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m][t], xh[t], acc[m], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    lds[base + hout + op.out_off[m * 8 + a * 2 + b]] =
+                        float2{acc[m][4 * a + 2 * b], acc[m][4 * a + 2 * b + 1]};
+    }
+}
+
+// A k-qubit gate on the chunk in LDS on the vector units: work-item t takes groups t, t+256, ...
+template <int K>
+__device__ void valu_gate(float2* __restrict__ lds, const DenseOpDesc& op, const float2* __restrict__ mats) {
+    constexpr int D = 1 << K;
+    constexpr int NG = kDChunk >> K;
+    const float2* M = mats + op.mat;
+    uint32_t off[D];
+#pragma unroll
+    for (int l = 0; l < D; ++l) {
+        uint32_t o = 0;
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            if ((l >> i) & 1) o |= 1u << op.bits[i];
+        off[l] = o;
+    }
+    for (int g = threadIdx.x; g < NG; g += kDThreads) {
+        uint32_t base = (uint32_t)g;
+#pragma unroll
+        for (int q = 0; q < K; ++q) base = ins0_32(base, op.sorted[q]);
+        float2 v[D];
+#pragma unroll
+        for (int l = 0; l < D; ++l) v[l] = lds[base + off[l]];
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                const float2 m = M[r * D + c];
+                re = fmaf(m.x, v[c].x, fmaf(-m.y, v[c].y, re));
+                im = fmaf(m.x, v[c].y, fmaf(m.y, v[c].x, im));
+            }
+            lds[base + off[r]] = float2{re, im};
+        }
+    }
+}
+
+// One chunk per work-group: load into LDS, every gate of the sweep, store.
+template <bool F32>
+__global__ void __launch_bounds__(kDThreads) k_dsweep(float2* __restrict__ amp, const DenseSweepDesc* __restrict__ sw,
+                                                     const DenseOpDesc* __restrict__ ops,
+                                                     const float2* __restrict__ mats,
+                                                     const unsigned char* __restrict__ frags) {
+    __shared__ float2 lds[kDChunk];
+    const unsigned t = threadIdx.x;
+    uint64_t base = blockIdx.x;
+#pragma unroll
+    for (int l = 0; l < kDW; ++l) {
+        const uint32_t q = sw->S[l];
+        const uint64_t lo = base & ((1ull << q) - 1);
+        base = lo | ((base ^ lo) << 1);
+    }
+    // local index i = t + 256 r: bits 0-7 from the work-item, bits 8-12 from r
+    uint64_t tl = 0;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) tl |= (uint64_t)((t >> l) & 1u) << sw->S[l];
+    constexpr int R = kDChunk / kDThreads;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        uint64_t ro = 0;
+#pragma unroll
+        for (int l = 0; l < 5; ++l)
+            if ((r >> l) & 1) ro |= 1ull << sw->S[8 + l];
+        lds[t + kDThreads * r] = amp[base | tl | ro];
+    }
+    __syncthreads();
+    const uint32_t n_ops = sw->n_ops;
+    for (uint32_t o = 0; o < n_ops; ++o) {
+        const DenseOpDesc& op = ops[sw->op_begin + o];
+        if (op.kind == DOP_MFMA) {
+            if (op.k == 5) mfma_gate<5, F32>(lds, op, frags);
+            else mfma_gate<4, F32>(lds, op, frags);
+        } else {
+            switch (op.k) {
+                case 1: valu_gate<1>(lds, op, mats); break;
+                case 2: valu_gate<2>(lds, op, mats); break;
+                default: valu_gate<3>(lds, op, mats); break;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        uint64_t ro = 0;
+#pragma unroll
+        for (int l = 0; l < 5; ++l)
+            if ((r >> l) & 1) ro |= 1ull << sw->S[8 + l];
+        amp[base | tl | ro] = lds[t + kDThreads * r];
+    }
+}
+
+// Row-major complex128 matrix of a gate, index bit i <-> qubits[i] (the convention of the
+// reference's get_matrix and apply_unitary, middleware.hpp:399-439, cpu_openmp_engine.hpp:
+// 539-606). Standard gates from their closed forms with the same FP64 expressions as the
+// other kernels; UNITARY / FUSED_BLOCK from the pool.
+std::vector<double> gate_matrix(const qlx_gate_instr& g, const double* pool, const qlx_matrix_entry* meta, int& k) {
+    auto perm = [&](int kk, const std::vector<int>& to) {
+        k = kk;
+        const int D = 1 << kk;
+        std::vector<double> m(2 * D * D, 0.0);
+        for (int c = 0; c < D; ++c) m[2 * (to[c] * D + c)] = 1.0;
+        return m;
+    };
+    const double half = static_cast<double>(g.payload.param) * 0.5;
+    switch (g.type) {
+        case QLX_H: {
+            k = 1;
+            const double r = 1.0 / std::sqrt(2.0);
+            return {r, 0, r, 0, r, 0, -r, 0};
+        }
+        case QLX_X: k = 1; return {0, 0, 1, 0, 1, 0, 0, 0};
+        case QLX_Y: k = 1; return {0, 0, 0, -1, 0, 1, 0, 0};
+        case QLX_Z: k = 1; return {1, 0, 0, 0, 0, 0, -1, 0};
+        case QLX_S: k = 1; return {1, 0, 0, 0, 0, 0, 0, 1};
+        case QLX_RX: {
+            k = 1;
+            const double c = std::cos(half), s = std::sin(half);
+            return {c, 0, 0, -s, 0, -s, c, 0};
+        }
+        case QLX_RY: {
+            k = 1;
+            const double c = std::cos(half), s = std::sin(half);
+            return {c, 0, -s, 0, s, 0, c, 0};
+        }
+        case QLX_RZ:
+            k = 1;
+            return {std::cos(-half), std::sin(-half), 0, 0, 0, 0, std::cos(half), std::sin(half)};
+        case QLX_CX: return perm(2, {0, 3, 2, 1});                    // control bit 0, target bit 1
+        case QLX_SWAP: case QLX_GLOBAL_SWAP: return perm(2, {0, 2, 1, 3});
+        case QLX_CCX: return perm(3, {0, 1, 2, 7, 4, 5, 6, 3});       // controls bits 0, 1; target bit 2
+        case QLX_CSWAP: return perm(3, {0, 1, 2, 5, 4, 3, 6, 7});     // control bit 0; swap bits 1, 2
+        case QLX_UNITARY:
+        case QLX_FUSED_BLOCK: {
+            k = instr_arity(g);
+            const qlx_matrix_entry& e = meta[g.payload.matrix_idx];
+            const double* m = pool + 2 * (size_t)e.offset;
+            return std::vector<double>(m, m + 2 * (size_t)e.num_elements);
+        }
+        default:
+            fail(QLX_EUNSUPPORTED, "gate cannot run in a dense sweep");
+    }
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+struct DensePlan {
+    uint64_t key = 0;
+    std::vector<DenseSweepDesc> sweeps;
+    std::vector<DenseOpDesc> ops;
+    std::vector<float> mats;     // interleaved complex FP32
+    size_t gates = 0;
+    DevBuf dev;                  // sweeps | ops | mats
+    size_t off_ops = 0, off_mats = 0;
+};
+
+}  // namespace
+
+void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const double* pool,
+                          size_t pool_elems, const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts& opts,
+                          qlx_exec_stats& stats) {
+    if (st->dtype != QLX_C64) fail(QLX_EUNSUPPORTED, "dense sweeps run on complex64 states");
+    if (st->nq < (uint32_t)kDW) {
+        execute_plain(st, instrs, n, pool, pool_elems, meta, n_meta, opts, stats);
+        return;
+    }
+    if (opts.dense_kernel == QLX_DENSE_VALU) {
+        qlx_exec_opts o = opts;
+        o.mode = QLX_EXEC_WINDOWED;
+        execute_windowed(st, instrs, n, pool, pool_elems, meta, n_meta, o, stats);
+        return;
+    }
+    bool need_pool = false;
+    for (size_t i = 0; i < n; ++i)
+        if (instrs[i].type == QLX_UNITARY || instrs[i].type == QLX_FUSED_BLOCK) need_pool = true;
+    const double* dpool = need_pool ? upload_pool(st, pool, pool_elems) : nullptr;
+    prepare_dense(st, instrs, n, dpool, meta, n_meta, opts);
+    const bool f32 = st->frag_variant == QLX_DENSE_MFMA_F32;
+
+    uint64_t key = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t b) {
+        const unsigned char* c = (const unsigned char*)p;
+        for (size_t i = 0; i < b; ++i) {
+            key ^= c[i];
+            key *= 1099511628211ull;
+        }
+    };
+    mix(instrs, n * sizeof(qlx_gate_instr));
+    if (pool) mix(pool, pool_elems * 16);
+    if (meta) mix(meta, n_meta * sizeof(qlx_matrix_entry));
+    mix(&opts, sizeof(opts));
+    mix(&st->nq, sizeof(st->nq));
+    if (!st->dense_cache) st->dense_cache = std::shared_ptr<void>(new DensePlan(), [](void* p) {
+        auto* d = static_cast<DensePlan*>(p);
+        d->dev.release();
+        delete d;
+    });
+    DensePlan& P = *static_cast<DensePlan*>(st->dense_cache.get());
+    if (P.key != key || P.sweeps.empty()) {
+        P.sweeps.clear();
+        P.ops.clear();
+        P.mats.clear();
+        P.gates = 0;
+        for (const DenseWindow& w : plan_dense_windows(st->nq, instrs, n, opts, kDW, kDC)) {
+            DenseSweepDesc d{};
+            for (int l = 0; l < kDW; ++l) d.S[l] = (uint32_t)w.S[l];
+            if (w.S.size() != (size_t)kDW) fail(QLX_EHIP, "dense sweep window of the wrong size");
+            for (int l = 0; l < kDC; ++l)
+                if (w.S[l] != l) fail(QLX_EHIP, "dense sweep window without the low qubits");
+            d.op_begin = (uint32_t)P.ops.size();
+            std::vector<int> loc(st->nq, -1);
+            for (int l = 0; l < kDW; ++l) loc[w.S[l]] = l;
+            for (int src : w.instrs) {
+                const qlx_gate_instr& g = instrs[src];
+                int k = 0;
+                const std::vector<double> m = gate_matrix(g, pool, meta, k);
+                DenseOpDesc o{};
+                o.k = (uint32_t)k;
+                for (int i = 0; i < k; ++i) {
+                    const int lb = loc[instr_qubit(g, i)];
+                    if (lb < 0) fail(QLX_EHIP, "dense sweep gate outside its window");
+                    o.bits[i] = (uint32_t)lb;
+                    o.sorted[i] = (uint32_t)lb;
+                }
+                std::sort(o.sorted, o.sorted + k);
+                const uint32_t mi = g.payload.matrix_idx;
+                const bool mfma = (g.type == QLX_UNITARY || g.type == QLX_FUSED_BLOCK) && (k == 4 || k == 5) &&
+                                  mi < st->frag_index.size() && st->frag_index[mi] >= 0;
+                if (mfma) {
+                    o.kind = DOP_MFMA;
+                    o.frag_off = (uint64_t)st->frag_index[mi];
+                    auto off = [&](int l) {
+                        uint32_t v = 0;
+                        for (int i = 0; i < k; ++i)
+                            if ((l >> i) & 1) v |= 1u << o.bits[i];
+                        return v;
+                    };
+                    for (int r = 0; r < (1 << (k - 1)); ++r) o.in_off[r] = off(r);
+                    const int mt = (1 << (k + 1)) / 32;
+                    for (int mm = 0; mm < mt; ++mm)
+                        for (int a = 0; a < 4; ++a)
+                            for (int b = 0; b < 2; ++b) o.out_off[mm * 8 + a * 2 + b] = off(16 * mm + 4 * a + b);
+                    o.hbit_in = 1u << o.bits[k - 1];
+                    o.hbit_out = 1u << o.bits[1];
+                } else {
+                    if (k > 3) fail(QLX_EUNSUPPORTED, "dense sweeps run 4/5-qubit gates on the matrix cores only");
+                    o.kind = DOP_VALU;
+                    o.mat = (uint32_t)(P.mats.size() / 2);
+                    for (double v : m) P.mats.push_back((float)v);
+                }
+                P.ops.push_back(o);
+                ++P.gates;
+            }
+            d.n_ops = (uint32_t)(P.ops.size() - d.op_begin);
+            P.sweeps.push_back(d);
+        }
+        const size_t b_sw = P.sweeps.size() * sizeof(DenseSweepDesc);
+        P.off_ops = align_up(b_sw, 256);
+        P.off_mats = align_up(P.off_ops + P.ops.size() * sizeof(DenseOpDesc), 256);
+        const size_t total = align_up(P.off_mats + P.mats.size() * sizeof(float), 256) + 256;
+        std::vector<char> img(total, 0);
+        std::memcpy(img.data(), P.sweeps.data(), b_sw);
+        if (!P.ops.empty()) std::memcpy(img.data() + P.off_ops, P.ops.data(), P.ops.size() * sizeof(DenseOpDesc));
+        if (!P.mats.empty()) std::memcpy(img.data() + P.off_mats, P.mats.data(), P.mats.size() * sizeof(float));
+        QLX_HIP_CHECK(hipStreamSynchronize(st->stream));  // the old image may still be in use
+        P.dev.reserve(total, st->stream);
+        QLX_HIP_CHECK(hipMemcpyAsync(P.dev.ptr, img.data(), total, hipMemcpyHostToDevice, st->stream));
+        QLX_HIP_CHECK(hipStreamSynchronize(st->stream));
+        P.key = key;
+    }
+    const char* dev = (const char*)P.dev.ptr;
+    const uint64_t chunks = 1ull << (st->nq - kDW);
+    if (chunks > 0xffffffffull) fail(QLX_EUNSUPPORTED, "state too large for one dense-sweep launch");
+    const double pass = pass_bytes(st);
+    const unsigned char* frags = (const unsigned char*)st->frags.ptr;
+    for (size_t s = 0; s < P.sweeps.size(); ++s) {
+        TimedLaunch tl(st, "k_dsweep", pass);
+        const DenseSweepDesc* sw = (const DenseSweepDesc*)dev + s;
+        const DenseOpDesc* ops = (const DenseOpDesc*)(dev + P.off_ops);
+        const float2* mats = (const float2*)(dev + P.off_mats);
+        if (f32)
+            k_dsweep<true><<<(unsigned)chunks, kDThreads, 0, st->stream>>>((float2*)st->amp, sw, ops, mats, frags);
+        else
+            k_dsweep<false><<<(unsigned)chunks, kDThreads, 0, st->stream>>>((float2*)st->amp, sw, ops, mats, frags);
+        QLX_HIP_CHECK(hipGetLastError());
+        stats.sweeps_full++;
+        stats.algorithmic_bytes += pass;
+    }
+    stats.gates_applied += P.gates;
+}
+
+}  // namespace qlx

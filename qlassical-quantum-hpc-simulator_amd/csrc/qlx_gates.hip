@@ -531,7 +531,8 @@ int qlx_execute(qlx_state* st, const qlx_gate_instr* instrs, size_t n_instr, con
         if (n_instr && !instrs) fail(QLX_EINVAL, "null instruction stream");
         qlx_exec_opts o{};
         if (opts) o = *opts;
-        if (o.mode != QLX_EXEC_PLAIN && o.mode != QLX_EXEC_WINDOWED) fail(QLX_EINVAL, "unknown execution mode");
+        if (o.mode != QLX_EXEC_PLAIN && o.mode != QLX_EXEC_WINDOWED && o.mode != QLX_EXEC_DENSE_SWEEPS)
+            fail(QLX_EINVAL, "unknown execution mode");
         // The run ends at the first MEASURE (handled by the host) or the first
         // unknown opcode (executed prefix, then the reference's runtime_error).
         size_t stop = n_instr;
@@ -548,6 +549,8 @@ int qlx_execute(qlx_state* st, const qlx_gate_instr* instrs, size_t n_instr, con
         if (run > 0) {
             if (o.mode == QLX_EXEC_PLAIN)
                 execute_plain(st, instrs, run, pool, pool_elems, meta, n_meta, o, stats);
+            else if (o.mode == QLX_EXEC_DENSE_SWEEPS)
+                execute_dense_sweeps(st, instrs, run, pool, pool_elems, meta, n_meta, o, stats);
             else
                 execute_windowed(st, instrs, run, pool, pool_elems, meta, n_meta, o, stats);
         }

@@ -93,6 +93,7 @@ struct qlx_state {
     qlx::DevBuf scratch;          // reduction partials
     qlx::DevBuf prog;             // uploaded gate descriptors / matrices
     std::shared_ptr<void> sweep_cache;  // last windowed plan (host plan + device image)
+    std::shared_ptr<void> dense_cache;  // last dense-sweep plan (host plan + device image)
     qlx::DevBuf frags;            // MFMA operand fragments of the current program's dense gates
     qlx::DevBuf frag_desc;        // their build descriptors
     std::vector<int64_t> frag_index;  // per matrix_idx: byte offset into frags, or -1
@@ -144,6 +145,11 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
                       const double* pool, size_t pool_elems,
                       const qlx_matrix_entry* meta, size_t n_meta,
                       const qlx_exec_opts& opts, qlx_exec_stats& stats);
+
+// Dense-sweep executor (QLX_EXEC_DENSE_SWEEPS), qlx_dsweep.hip.
+void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const double* pool,
+                          size_t pool_elems, const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts& opts,
+                          qlx_exec_stats& stats);
 
 // Uploads the unitary pool (complex128) to the state's program buffer; returns it.
 const double* upload_pool(qlx_state* st, const double* pool, size_t pool_elems);

@@ -812,6 +812,51 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
     return plan;
 }
 
+
+std::vector<DenseWindow> plan_dense_windows(uint32_t nq, const qlx_gate_instr* instrs, size_t n,
+                                            const qlx_exec_opts& opts, int w, int c) {
+    std::vector<PGate> gates;
+    for (size_t i = 0; i < n; ++i) {
+        const qlx_gate_instr& g = instrs[i];
+        if (g.type == QLX_BARRIER) continue;
+        if (g.type == QLX_GLOBAL_SWAP && opts.gswap_mode == QLX_GSWAP_NOOP) continue;
+        PGate p{};
+        p.src = (int)i;
+        p.type = g.type;
+        p.nq = operand_count(g);
+        for (int k = 0; k < p.nq; ++k) p.q[k] = instr_qubit(g, k);
+        p.sweepable = true;
+        gates.push_back(p);
+    }
+    const int G = (int)gates.size();
+    std::vector<std::vector<int>> preds(G);
+    {
+        std::vector<int> last(nq, -1);
+        for (int i = 0; i < G; ++i)
+            for (int k = 0; k < gates[i].nq; ++k) {
+                const int q = gates[i].q[k];
+                if (last[q] >= 0) preds[i].push_back(last[q]);
+                last[q] = i;
+            }
+    }
+    Geometry geo{};
+    geo.w = w;
+    geo.c = c;
+    std::vector<DenseWindow> out;
+    if (gates.empty()) return out;
+    const std::vector<std::vector<int>> wins = beam_windows(gates, preds, (int)nq, geo, 8);
+    Walk walk(gates, preds);
+    for (const std::vector<int>& S : wins) {
+        DenseWindow d;
+        d.S = S;
+        for (int gi : walk.take(S, (int)nq)) d.instrs.push_back(gates[gi].src);
+        if (d.instrs.empty()) fail(QLX_EHIP, "dense sweep planner: empty sweep");
+        out.push_back(std::move(d));
+    }
+    if (!walk.pending.empty()) fail(QLX_EHIP, "dense sweep planner left gates behind");
+    return out;
+}
+
 // Plans with the wider window search. `thorough` also plans with the classic greedy
 // chooser alone and with a beam search over window sequences (width 16), runs sweeps that
 // would cost less than one pass as plain kernels, and keeps the plan with the fewest
@@ -1087,6 +1132,32 @@ extern "C" int qlx_plan_windowed(uint32_t num_qubits, int dtype, const qlx_gate_
         for (size_t i = 0; i < n; ++i) {
             if (instrs[i].type == QLX_MEASURE) fail(QLX_EINVAL, "plans cover MEASURE-free streams");
             validate_instr(instrs[i], num_qubits, meta, n_meta, pool_elems);
+        }
+        if (o.mode == QLX_EXEC_DENSE_SWEEPS) {
+            // dense sweeps: 13-qubit windows holding the low 4 qubits, every gate inside one
+            std::vector<DenseWindow> wins;
+            if (num_qubits >= 13) wins = plan_dense_windows(num_qubits, instrs, n, o, 13, 4);
+            size_t cnt = 0;
+            for (const DenseWindow& w : wins) cnt += w.instrs.size();
+            if (n_out) *n_out = cnt;
+            size_t k = 0;
+            for (size_t si = 0; si < wins.size(); ++si)
+                for (int src : wins[si].instrs) {
+                    if (k < cap) {
+                        if (order) order[k] = src;
+                        if (sweep_of) sweep_of[k] = (int32_t)si;
+                    }
+                    ++k;
+                }
+            if (info) {
+                std::memset(info, 0, sizeof(*info));
+                info->window_qubits = 13;
+                info->coalesce_qubits = 4;
+                info->threads_per_group = 256;
+                info->sweeps = wins.size();
+                info->gates_in_sweeps = cnt;
+            }
+            return;
         }
         const Geometry geo = geometry_for(dtype, num_qubits, o);
         std::unique_ptr<SweepPlan> p;

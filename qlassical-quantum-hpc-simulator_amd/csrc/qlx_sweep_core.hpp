@@ -361,6 +361,16 @@ struct SweepPlan {
     std::vector<std::vector<int>> step_pos;
 };
 
+// Windows of dense sweeps (QLX_EXEC_DENSE_SWEEPS): every instruction of arity <= 5 runs
+// inside a window of `w` qubits that holds the low `c` qubits; the windows come from the
+// beam search over window sequences. Each window lists its instructions in program order.
+struct DenseWindow {
+    std::vector<int> S;       // window qubits, ascending
+    std::vector<int> instrs;  // indices into the instruction stream
+};
+std::vector<DenseWindow> plan_dense_windows(uint32_t nq, const qlx_gate_instr* instrs, size_t n,
+                                            const qlx_exec_opts& opts, int w, int c);
+
 // Gives every sweep of the plan a load layout and a store layout (the relabel option): each
 // sweep stores through a layout that puts the next sweep's window on the low address bits;
 // the low C qubits never move and the last sweep stores through the identity. Also rewrites

@@ -25,6 +25,7 @@ QLX_EXEC_PLAIN = 0
 QLX_EXEC_WINDOWED = 1
 QLX_GSWAP_PERMUTE = 0
 QLX_GSWAP_NOOP = 1
+QLX_EXEC_DENSE_SWEEPS = 2
 QLX_SPECIALIZE = 1
 QLX_SPECIALIZE_RELABEL = 2
 QLX_DENSE_AUTO = 0

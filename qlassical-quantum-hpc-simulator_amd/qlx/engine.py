@@ -44,7 +44,8 @@ def exec_opts(mode: str = "plain", window_qubits: int = 0, coalesce_qubits: int 
     o.dense_kernel = DENSE[dense]
     o.sweep_workgroups = sweep_workgroups
     o.specialize = (_abi.QLX_SPECIALIZE if specialize else 0) | (_abi.QLX_SPECIALIZE_RELABEL if relabel else 0)
-    o.mode = {"plain": _abi.QLX_EXEC_PLAIN, "windowed": _abi.QLX_EXEC_WINDOWED}[mode]
+    o.mode = {"plain": _abi.QLX_EXEC_PLAIN, "windowed": _abi.QLX_EXEC_WINDOWED,
+              "dense": _abi.QLX_EXEC_DENSE_SWEEPS}[mode]
     o.window_qubits = window_qubits
     o.register_qubits = register_qubits
     o.coalesce_qubits = coalesce_qubits

@@ -158,3 +158,64 @@ def test_dense_kernel_choice_rejected_when_unknown(gpu):
     with pytest.raises(gpu.QlxError):
         st.execute(prog, o)
     st.close()
+
+
+# ---- dense window sweeps (QLX_EXEC_DENSE_SWEEPS) ---------------------------------------------
+@pytest.mark.parametrize("dense", ["bf16", "f32"])
+def test_dense_sweeps_fused_circuits_vs_oracle(gpu, dense):
+    """Fused programs (widths 5 and 4) in 13-qubit LDS windows, 4/5-qubit gates on the matrix
+    cores, the rest as FP32 matrices: against the oracle's unfused circuit."""
+    rng = np.random.default_rng(91)
+    for n, d, seed, w in [(16, 12, 1003, 5), (18, 8, 7, 4), (14, 10, 3, 5)]:
+        prog = gpu.layered(n, d, seed)
+        fused = gpu.transpile_gpu(prog, max_width=w, rotations=True)
+        psi = rand_state(rng, n)
+        want, _ = oracle.execute(n, prog.instrs, amps=psi)
+        got, t = run_c64(gpu, fused, psi, dense, "dense")
+        assert "k_dsweep" in t and "k_dense_mfma" not in t, t
+        bound = 1e-6 if dense == "bf16" else 1e-9
+        assert infidelity(got, want) <= bound, (n, w)
+        assert np.abs(got - want).max() <= 50 * TOL[dense][0] * np.abs(want).max()
+        assert abs(np.vdot(got, got).real - 1.0) <= 1e-5
+
+
+def test_dense_sweeps_every_gate_kind_vs_oracle(gpu):
+    """Random programs with every gate kind (1-3 qubit standard gates, CCX, CSWAP, dense
+    1-5 qubit unitaries) in dense sweeps, from random states; states below one window run
+    gate by gate."""
+    from test_oracle import random_program
+    for seed in range(4):
+        rng = np.random.default_rng(300 + seed)
+        n = int(rng.integers(11, 17))
+        ins, pool, meta = random_program(n, 150, rng)
+        p = gpu.Program(n, ins, pool if pool is not None else np.zeros(0, complex),
+                        meta if meta is not None else np.zeros(0, gpu.META_DTYPE))
+        c = gpu.Circuit(n)
+        for _ in range(6):
+            qs = [int(x) for x in rng.choice(n, 5, replace=False)]
+            c.unitary(qs, rand_unitary(rng, 5), max_arity=5)
+        u5 = c.release()
+        for prog in (p, u5):
+            psi = rand_state(rng, n)
+            want, _ = oracle.execute(n, prog.instrs, prog.pool if prog.pool.size else None,
+                                     prog.meta if len(prog.meta) else None, amps=psi)
+            got, t = run_c64(gpu, prog, psi, "bf16", "dense")
+            assert ("k_dsweep" in t) == (n >= 13), (n, t)
+            assert infidelity(got, want) <= 1e-6, (seed, n)
+
+
+def test_dense_sweeps_repeat_and_switch_programs(gpu):
+    """The cached dense plan follows the program: alternating two programs on one state."""
+    n = 15
+    rng = np.random.default_rng(5)
+    a = gpu.transpile_gpu(gpu.layered(n, 6, 1), max_width=5, rotations=True)
+    b = gpu.transpile_gpu(gpu.layered(n, 5, 2), max_width=4, rotations=True)
+    psi = rand_state(rng, n)
+    want = psi
+    st = gpu.State(amps=psi, dtype="c64")
+    for prog in (a, b, a, a, b):
+        want, _ = oracle.execute(n, prog.instrs, prog.pool, prog.meta, amps=want)
+        st.execute(prog, gpu.exec_opts("dense", dense="f32"))
+    got = st.to_host()
+    st.close()
+    assert infidelity(got, want) <= 1e-9

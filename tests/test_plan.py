@@ -81,3 +81,43 @@ def test_cheap_sweeps_run_as_plain_kernels():
 def test_small_states_plan_nothing():
     order, sweep, info = qlx.plan_windowed(qlx.layered(8, 3, 1))
     assert len(order) == 0 and info["sweeps"] == 0
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_dense_sweep_plan_is_valid_reordering(seed):
+    """Dense sweeps (every gate, up to 5 qubits, inside a 13-qubit window holding qubits
+    0-3): the execution order is a valid reordering and every window holds its gates."""
+    rng = np.random.default_rng(70 + seed)
+    n = int(rng.integers(14, 18))
+    ins, pool, meta = random_program(n, 120, rng)
+    p = qlx.Program(n, ins, pool if pool is not None else np.zeros(0, complex),
+                    meta if meta is not None else np.zeros(0, qlx.META_DTYPE))
+    fused = qlx.transpile_gpu(qlx.layered(n, 6, seed), max_width=5, rotations=True)
+    for prog in (p, fused):
+        order, sweep, info = qlx.plan_windowed(prog, "c64", qlx.exec_opts("dense"))
+        assert sorted(order.tolist()) == list(range(len(prog))) and info["window_qubits"] == 13
+        assert (sweep >= 0).all() and info["sweeps"] == sweep.max() + 1
+        init = bf.random_state(n, np.random.default_rng(seed))
+        pl = prog.pool if prog.pool.size else None
+        me = prog.meta if len(prog.meta) else None
+        a, _ = oracle.execute(n, prog.instrs, pl, me, amps=init)
+        b, _ = oracle.execute(n, prog.instrs[order], pl, me, amps=init)
+        np.testing.assert_allclose(b, a, atol=1e-12)
+        for s_ in range(info["sweeps"]):
+            qs = set()
+            for g in prog.instrs[order[sweep == s_]]:
+                a_ = int(g["flags_arity"]) & 15
+                t = int(g["type"])
+                k = 2 if t in (0x20, 0x21, 0xE1) else 3 if t in (0x30, 0x31) else max(a_, 1)
+                qs |= set(int(x) for x in g["qubits"][:min(k, 4)]) | ({int(g["pad"]) & 255} if k == 5 else set())
+            assert len(qs | {0, 1, 2, 3}) <= 13
+
+
+def test_dense_sweeps_of_the_fused_32q_circuit():
+    """The 32-qubit depth-30 layered circuit fused to width 5 needs far fewer dense sweeps
+    than its dense gates (one HBM pass each when run gate by gate)."""
+    f = qlx.transpile_gpu(qlx.layered(32, 30, 1003), max_width=5, rotations=True)
+    n_dense = int(((f.instrs["type"] == 0xE0) & ((f.instrs["flags_arity"] & 15) >= 4)).sum())
+    _, _, info = qlx.plan_windowed(f, "c64", qlx.exec_opts("dense"))
+    print(len(f), n_dense, info)
+    assert info["sweeps"] * 3 < n_dense
