@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -55,9 +56,6 @@ struct DenseOpDesc {
     uint64_t frag_off;      // DOP_MFMA: byte offset of the A fragments
     uint32_t bits[5];       // local bit of gate qubit i (matrix index bit i)
     uint32_t sorted[5];     // the same, ascending (group base by zero-bit insertion)
-    uint32_t in_off[16];    // DOP_MFMA: local offset of group amplitude r < 2^(k-1)
-    uint32_t out_off[16];   // DOP_MFMA: local offset of output slot 16m + 4a + b
-    uint32_t hbit_in, hbit_out;
 };
 
 __device__ __forceinline__ uint32_t ins0_32(uint32_t idx, uint32_t t) {
@@ -75,7 +73,19 @@ __device__ void mfma_gate(float2* __restrict__ lds, const DenseOpDesc& op, const
     constexpr int NTILES = (kDChunk >> K) / 32;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int h = lane >> 5, j = lane & 31;
-    const uint32_t hin = h ? op.hbit_in : 0u, hout = h ? op.hbit_out : 0u;
+    // local offset of group amplitude l: the sum of 2^bits[i] over the set bits i of l, formed
+    // where it is used from five values (a table of 32 offsets kept the scalar registers full)
+    uint32_t pb[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) pb[i] = 1u << op.bits[i];
+    auto off = [&](int l) {
+        uint32_t o = 0;
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            if ((l >> i) & 1) o += pb[i];
+        return o;
+    };
+    const uint32_t hin = h ? pb[K - 1] : 0u, hout = h ? pb[1] : 0u;
     const unsigned char* fr = frags + op.frag_off;
     // the gate's A fragments, once per wave (the wave then takes 2 (K = 5) or 4 (K = 4) tiles)
     bf16x8 ah[F32 ? 1 : MT][F32 ? 1 : TB], am[F32 ? 1 : MT][F32 ? 1 : TB], al[F32 ? 1 : MT][F32 ? 1 : TB];
@@ -105,7 +115,7 @@ __device__ void mfma_gate(float2* __restrict__ lds, const DenseOpDesc& op, const
         for (int q = 0; q < K; ++q) base = ins0_32(base, op.sorted[q]);
         float2 x[HALF];
 #pragma unroll
-        for (int r = 0; r < HALF; ++r) x[r] = lds[base + hin + op.in_off[r]];
+        for (int r = 0; r < HALF; ++r) x[r] = lds[base + hin + off(r)];
         f32x16 acc[MT];
 #pragma unroll
         for (int m = 0; m < MT; ++m)
@@ -153,7 +163,7 @@ __device__ void mfma_gate(float2* __restrict__ lds, const DenseOpDesc& op, const
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b)
-                    lds[base + hout + op.out_off[m * 8 + a * 2 + b]] =
+                    lds[base + hout + off(16 * m + 4 * a + b)] =
                         float2{acc[m][4 * a + 2 * b], acc[m][4 * a + 2 * b + 1]};
     }
 }
@@ -194,6 +204,47 @@ __device__ void valu_gate(float2* __restrict__ lds, const DenseOpDesc& op, const
     }
 }
 
+// 3-qubit gates (CCX, CSWAP, 3-qubit matrices; rare in fused programs): the 64 matrix
+// entries are staged in LDS and read row by row, which keeps them out of the registers of
+// the main loop (uniform entries held in scalar registers spilled the others).
+__device__ void valu_gate3(float2* __restrict__ lds, const DenseOpDesc& op, const float2* __restrict__ mats,
+                           float2* __restrict__ mbuf) {
+    constexpr int K = 3, D = 8, NG = kDChunk >> K;
+    if (threadIdx.x < D * D) mbuf[threadIdx.x] = mats[op.mat + threadIdx.x];
+    __syncthreads();
+    uint32_t off[D];
+#pragma unroll
+    for (int l = 0; l < D; ++l) {
+        uint32_t o = 0;
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            if ((l >> i) & 1) o |= 1u << op.bits[i];
+        off[l] = o;
+    }
+#pragma unroll 1
+    for (int g = threadIdx.x; g < NG; g += kDThreads) {
+        uint32_t base = (uint32_t)g;
+#pragma unroll
+        for (int q = 0; q < K; ++q) base = ins0_32(base, op.sorted[q]);
+        float2 v[D];
+#pragma unroll
+        for (int l = 0; l < D; ++l) v[l] = lds[base + off[l]];
+        // the group's amplitudes are all in registers: each row is written as it is formed
+#pragma unroll 1
+        for (int r = 0; r < D; ++r) {
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                const float2 m = mbuf[r * D + c];
+                re = fmaf(m.x, v[c].x, fmaf(-m.y, v[c].y, re));
+                im = fmaf(m.x, v[c].y, fmaf(m.y, v[c].x, im));
+            }
+            const uint32_t o = ((r & 1) ? off[1] : 0u) + ((r & 2) ? off[2] : 0u) + ((r & 4) ? off[4] : 0u);
+            lds[base + o] = float2{re, im};
+        }
+    }
+}
+
 // One chunk per work-group: load into LDS, every gate of the sweep, store.
 template <bool F32>
 __global__ void __launch_bounds__(kDThreads) k_dsweep(float2* __restrict__ amp, const DenseSweepDesc* __restrict__ sw,
@@ -201,6 +252,7 @@ __global__ void __launch_bounds__(kDThreads) k_dsweep(float2* __restrict__ amp, 
                                                      const float2* __restrict__ mats,
                                                      const unsigned char* __restrict__ frags) {
     __shared__ float2 lds[kDChunk];
+    __shared__ float2 mbuf[64];  // the current 3-qubit gate's matrix
     const unsigned t = threadIdx.x;
     uint64_t base = blockIdx.x;
 #pragma unroll
@@ -233,17 +285,22 @@ __global__ void __launch_bounds__(kDThreads) k_dsweep(float2* __restrict__ amp, 
             switch (op.k) {
                 case 1: valu_gate<1>(lds, op, mats); break;
                 case 2: valu_gate<2>(lds, op, mats); break;
-                default: valu_gate<3>(lds, op, mats); break;
+                default: valu_gate3(lds, op, mats, mbuf); break;
             }
         }
         __syncthreads();
     }
+    // the store's register-bit offsets are formed again from positions read again (volatile):
+    // kept from the load, the 32 64-bit offsets would hold scalar registers through every gate
+    uint32_t sp[5];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) sp[l] = ((const volatile uint32_t*)sw->S)[8 + l];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         uint64_t ro = 0;
 #pragma unroll
         for (int l = 0; l < 5; ++l)
-            if ((r >> l) & 1) ro |= 1ull << sw->S[8 + l];
+            if ((r >> l) & 1) ro |= 1ull << sp[l];
         amp[base | tl | ro] = lds[t + kDThreads * r];
     }
 }
@@ -302,13 +359,89 @@ std::vector<double> gate_matrix(const qlx_gate_instr& g, const double* pool, con
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// ---- merging the gates of one sweep ------------------------------------------------------
+// Every application costs a pass over the chunk in LDS whatever its width, so the gates of a
+// sweep are merged (FP64 products, on the host) into as few blocks of at most five qubits as a
+// greedy rule finds: a gate joins the latest block whose qubits together with its own number at
+// most five, passing over later blocks that share none of its qubits (with which it commutes).
+constexpr int kMaxBlock = 5;
+
+struct Block {
+    std::vector<int> q;                  // window bits; matrix index bit i <-> q[i]
+    std::vector<std::complex<double>> m;  // row-major 2^|q| x 2^|q|
+};
+
+// The matrix m (on bits gq) as an operator on the ordered bit list u, a superset of gq.
+std::vector<std::complex<double>> embed_on(const std::vector<int>& gq, const std::vector<std::complex<double>>& m,
+                                           const std::vector<int>& u) {
+    const int kg = (int)gq.size(), ku = (int)u.size();
+    const int DG = 1 << kg, DU = 1 << ku;
+    int pos[kMaxBlock];
+    uint32_t gmask = 0;
+    for (int i = 0; i < kg; ++i) {
+        pos[i] = (int)(std::find(u.begin(), u.end(), gq[i]) - u.begin());
+        if (pos[i] >= ku) fail(QLX_EHIP, "dense sweep block merge: qubit missing from the union");
+        gmask |= 1u << pos[i];
+    }
+    auto gather = [&](uint32_t x) {
+        uint32_t r = 0;
+        for (int i = 0; i < kg; ++i) r |= ((x >> pos[i]) & 1u) << i;
+        return r;
+    };
+    std::vector<std::complex<double>> e((size_t)DU * DU);
+    for (uint32_t r = 0; r < (uint32_t)DU; ++r)
+        for (uint32_t c = 0; c < (uint32_t)DU; ++c)
+            if ((r & ~gmask) == (c & ~gmask)) e[(size_t)r * DU + c] = m[(size_t)gather(r) * DG + gather(c)];
+    return e;
+}
+
+// b <- g b on the union of their bits (g applied after b).
+void absorb(Block& b, const Block& g) {
+    std::vector<int> u = b.q;
+    for (int x : g.q)
+        if (std::find(u.begin(), u.end(), x) == u.end()) u.push_back(x);
+    const std::vector<std::complex<double>> B = embed_on(b.q, b.m, u), G = embed_on(g.q, g.m, u);
+    const size_t D = size_t{1} << u.size();
+    std::vector<std::complex<double>> C(D * D);
+    for (size_t r = 0; r < D; ++r)
+        for (size_t k = 0; k < D; ++k) {
+            const std::complex<double> a = G[r * D + k];
+            if (a == 0.0) continue;
+            for (size_t c = 0; c < D; ++c) C[r * D + c] += a * B[k * D + c];
+        }
+    b.q = std::move(u);
+    b.m = std::move(C);
+}
+
+std::vector<Block> merge_blocks(std::vector<Block> gates, int max_q) {
+    std::vector<Block> out;
+    for (Block& g : gates) {
+        bool merged = false;
+        for (size_t j = out.size(); j-- > 0;) {
+            Block& b = out[j];
+            int shared = 0;
+            for (int x : g.q) shared += std::find(b.q.begin(), b.q.end(), x) != b.q.end();
+            if ((int)(b.q.size() + g.q.size()) - shared <= max_q) {
+                absorb(b, g);
+                merged = true;
+                break;
+            }
+            if (shared) break;
+        }
+        if (!merged) out.push_back(std::move(g));
+    }
+    return out;
+}
+
 struct DensePlan {
     uint64_t key = 0;
     std::vector<DenseSweepDesc> sweeps;
     std::vector<DenseOpDesc> ops;
     std::vector<float> mats;     // interleaved complex FP32
-    size_t gates = 0;
+    size_t gates = 0;            // program gates in the sweeps
+    size_t blocks = 0;           // applications after merging
     DevBuf dev;                  // sweeps | ops | mats
+    DevBuf fpool, frags, fdesc;  // FP64 matrices of the matrix-core blocks, their A fragments
     size_t off_ops = 0, off_mats = 0;
 };
 
@@ -328,12 +461,10 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
         execute_windowed(st, instrs, n, pool, pool_elems, meta, n_meta, o, stats);
         return;
     }
-    bool need_pool = false;
-    for (size_t i = 0; i < n; ++i)
-        if (instrs[i].type == QLX_UNITARY || instrs[i].type == QLX_FUSED_BLOCK) need_pool = true;
-    const double* dpool = need_pool ? upload_pool(st, pool, pool_elems) : nullptr;
-    prepare_dense(st, instrs, n, dpool, meta, n_meta, opts);
-    const bool f32 = st->frag_variant == QLX_DENSE_MFMA_F32;
+    const uint32_t dk = opts.dense_kernel;
+    if (dk > QLX_DENSE_MFMA_F32) fail(QLX_EINVAL, "unknown dense_kernel");
+    const int variant = dk == QLX_DENSE_MFMA_F32 ? QLX_DENSE_MFMA_F32 : QLX_DENSE_MFMA_BF16;
+    const bool f32 = variant == QLX_DENSE_MFMA_F32;
 
     uint64_t key = 1469598103934665603ull;
     auto mix = [&](const void* p, size_t b) {
@@ -351,6 +482,9 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
     if (!st->dense_cache) st->dense_cache = std::shared_ptr<void>(new DensePlan(), [](void* p) {
         auto* d = static_cast<DensePlan*>(p);
         d->dev.release();
+        d->fpool.release();
+        d->frags.release();
+        d->fdesc.release();
         delete d;
     });
     DensePlan& P = *static_cast<DensePlan*>(st->dense_cache.get());
@@ -359,6 +493,10 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
         P.ops.clear();
         P.mats.clear();
         P.gates = 0;
+        P.blocks = 0;
+        std::vector<double> fpool;             // FP64 matrices of the matrix-core blocks
+        std::vector<MfmaSplitJob> jobs;
+        size_t frag_total = 0;
         for (const DenseWindow& w : plan_dense_windows(st->nq, instrs, n, opts, kDW, kDC)) {
             DenseSweepDesc d{};
             for (int l = 0; l < kDW; ++l) d.S[l] = (uint32_t)w.S[l];
@@ -368,46 +506,57 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
             d.op_begin = (uint32_t)P.ops.size();
             std::vector<int> loc(st->nq, -1);
             for (int l = 0; l < kDW; ++l) loc[w.S[l]] = l;
+            std::vector<Block> gates;
             for (int src : w.instrs) {
                 const qlx_gate_instr& g = instrs[src];
                 int k = 0;
                 const std::vector<double> m = gate_matrix(g, pool, meta, k);
+                Block b;
+                for (int i = 0; i < k; ++i) {
+                    const int q = (int)instr_qubit(g, i);
+                    if (loc[q] < 0) fail(QLX_EHIP, "dense sweep gate outside its window");
+                    b.q.push_back(loc[q]);
+                }
+                b.m.resize(m.size() / 2);
+                for (size_t e = 0; e < b.m.size(); ++e) b.m[e] = {m[2 * e], m[2 * e + 1]};
+                gates.push_back(std::move(b));
+                ++P.gates;
+            }
+            for (Block& b : merge_blocks(std::move(gates), kMaxBlock)) {
+                // blocks of 1-3 qubits run on the matrix cores as 4-qubit blocks (identity on
+                // the highest free window bits) except in FP32, where the vector form costs
+                // no more than an FP32 4-qubit product
+                if (b.q.size() < 4 && !f32) {
+                    std::vector<int> u = b.q;
+                    for (int l = kDW - 1; l >= 0 && u.size() < 4; --l)
+                        if (std::find(u.begin(), u.end(), l) == u.end()) u.push_back(l);
+                    b.m = embed_on(b.q, b.m, u);
+                    b.q = u;
+                }
+                const int k = (int)b.q.size();
                 DenseOpDesc o{};
                 o.k = (uint32_t)k;
-                for (int i = 0; i < k; ++i) {
-                    const int lb = loc[instr_qubit(g, i)];
-                    if (lb < 0) fail(QLX_EHIP, "dense sweep gate outside its window");
-                    o.bits[i] = (uint32_t)lb;
-                    o.sorted[i] = (uint32_t)lb;
-                }
+                for (int i = 0; i < k; ++i) o.bits[i] = o.sorted[i] = (uint32_t)b.q[i];
                 std::sort(o.sorted, o.sorted + k);
-                const uint32_t mi = g.payload.matrix_idx;
-                const bool mfma = (g.type == QLX_UNITARY || g.type == QLX_FUSED_BLOCK) && (k == 4 || k == 5) &&
-                                  mi < st->frag_index.size() && st->frag_index[mi] >= 0;
-                if (mfma) {
+                if (k >= 4) {
                     o.kind = DOP_MFMA;
-                    o.frag_off = (uint64_t)st->frag_index[mi];
-                    auto off = [&](int l) {
-                        uint32_t v = 0;
-                        for (int i = 0; i < k; ++i)
-                            if ((l >> i) & 1) v |= 1u << o.bits[i];
-                        return v;
-                    };
-                    for (int r = 0; r < (1 << (k - 1)); ++r) o.in_off[r] = off(r);
-                    const int mt = (1 << (k + 1)) / 32;
-                    for (int mm = 0; mm < mt; ++mm)
-                        for (int a = 0; a < 4; ++a)
-                            for (int b = 0; b < 2; ++b) o.out_off[mm * 8 + a * 2 + b] = off(16 * mm + 4 * a + b);
-                    o.hbit_in = 1u << o.bits[k - 1];
-                    o.hbit_out = 1u << o.bits[1];
+                    o.frag_off = frag_total;
+                    jobs.push_back(MfmaSplitJob{fpool.size() / 2, frag_total, k, variant});
+                    frag_total += (mfma_fragment_bytes(k, variant) + 255) & ~size_t{255};
+                    for (const std::complex<double>& v : b.m) {
+                        fpool.push_back(v.real());
+                        fpool.push_back(v.imag());
+                    }
                 } else {
-                    if (k > 3) fail(QLX_EUNSUPPORTED, "dense sweeps run 4/5-qubit gates on the matrix cores only");
                     o.kind = DOP_VALU;
                     o.mat = (uint32_t)(P.mats.size() / 2);
-                    for (double v : m) P.mats.push_back((float)v);
+                    for (const std::complex<double>& v : b.m) {
+                        P.mats.push_back((float)v.real());
+                        P.mats.push_back((float)v.imag());
+                    }
                 }
                 P.ops.push_back(o);
-                ++P.gates;
+                ++P.blocks;
             }
             d.n_ops = (uint32_t)(P.ops.size() - d.op_begin);
             P.sweeps.push_back(d);
@@ -423,6 +572,13 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
         QLX_HIP_CHECK(hipStreamSynchronize(st->stream));  // the old image may still be in use
         P.dev.reserve(total, st->stream);
         QLX_HIP_CHECK(hipMemcpyAsync(P.dev.ptr, img.data(), total, hipMemcpyHostToDevice, st->stream));
+        if (!jobs.empty()) {
+            P.fpool.reserve(fpool.size() * sizeof(double), st->stream);
+            QLX_HIP_CHECK(hipMemcpyAsync(P.fpool.ptr, fpool.data(), fpool.size() * sizeof(double),
+                                         hipMemcpyHostToDevice, st->stream));
+            P.frags.reserve(frag_total, st->stream);
+            mfma_split((const double*)P.fpool.ptr, jobs.data(), jobs.size(), P.frags.ptr, P.fdesc, st->stream);
+        }
         QLX_HIP_CHECK(hipStreamSynchronize(st->stream));
         P.key = key;
     }
@@ -430,7 +586,7 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
     const uint64_t chunks = 1ull << (st->nq - kDW);
     if (chunks > 0xffffffffull) fail(QLX_EUNSUPPORTED, "state too large for one dense-sweep launch");
     const double pass = pass_bytes(st);
-    const unsigned char* frags = (const unsigned char*)st->frags.ptr;
+    const unsigned char* frags = (const unsigned char*)P.frags.ptr;
     for (size_t s = 0; s < P.sweeps.size(); ++s) {
         TimedLaunch tl(st, "k_dsweep", pass);
         const DenseSweepDesc* sw = (const DenseSweepDesc*)dev + s;
