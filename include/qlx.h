@@ -123,6 +123,8 @@ typedef struct qlx_exec_opts {
 #define QLX_DENSE_MFMA_BF16   2  /* BF16 split operands (matrix hi+mid+lo, state hi+lo), four
                                     products accumulated in FP32 */
 #define QLX_DENSE_MFMA_F32    3  /* FP32 MFMA (v_mfma_f32_32x32x2_f32) */
+#define QLX_DENSE_MFMA_F16    4  /* FP16 split operands (matrix hi+lo, state scaled by 2^14 then
+                                    hi+lo: 22 bits each), three products accumulated in FP32 */
 
 /* Statistics of the last qlx_execute on a state (sweep accounting). */
 typedef struct qlx_exec_stats {

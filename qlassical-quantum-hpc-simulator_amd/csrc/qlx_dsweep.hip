@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "qlx_internal.hpp"
+#include "qlx_mfma_core.hpp"
 #include "qlx_sweep_core.hpp"
 
 namespace qlx {
@@ -36,8 +37,6 @@ constexpr int kDC = 4;               // low qubits in every window (128-byte run
 constexpr int kDThreads = 256;
 constexpr int kDChunk = 1 << kDW;
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 struct DenseSweepDesc {
     uint32_t S[kDW];       // window qubits ascending (local bit l <-> global qubit S[l])
@@ -58,6 +57,18 @@ struct DenseOpDesc {
     uint32_t sorted[5];     // the same, ascending (group base by zero-bit insertion)
 };
 
+// LDS position of local index x: bits 5-9 folded by XOR into bits 0-4 (a bijection on the
+// chunk). Groups of a gate on low bits sit 2^5 or more apart, and unswizzled all 32 columns of
+// a wave half would hit one bank; swz is linear over GF(2), so swz(base | off) = swz(base) ^
+// swz(off) for disjoint bit sets and every offset is swizzled once per gate.
+__device__ __forceinline__ uint32_t swz(uint32_t x) { return x ^ ((x >> 5) & 31u); }
+
+// The amplitude at byte offset `byte` of the chunk (offsets kept in bytes, so a swizzled
+// address costs one XOR and no shift).
+__device__ __forceinline__ float2& at(float2* lds, uint32_t byte) {
+    return *reinterpret_cast<float2*>(reinterpret_cast<char*>(lds) + byte);
+}
+
 __device__ __forceinline__ uint32_t ins0_32(uint32_t idx, uint32_t t) {
     const uint32_t lo = idx & ((1u << t) - 1);
     return lo | ((idx ^ lo) << 1);
@@ -65,30 +76,35 @@ __device__ __forceinline__ uint32_t ins0_32(uint32_t idx, uint32_t t) {
 
 // One 4/5-qubit gate on the chunk in LDS, on the matrix cores. Wave w takes tiles w, w+4, ...
 // of 32 groups; lane (h, j) holds group j's amplitudes [h 2^(K-1), (h+1) 2^(K-1)).
-template <int K, bool F32>
+template <int K, int FORM>
 __device__ void mfma_gate(float2* __restrict__ lds, const DenseOpDesc& op, const unsigned char* __restrict__ frags) {
     constexpr int D = 1 << K, HALF = D / 2, MT = (2 * D) / 32;
     constexpr int TB = (2 * D) / 16, TF = D;
     constexpr int NB = MT * TB * 64;
     constexpr int NTILES = (kDChunk >> K) / 32;
+    constexpr bool F32 = FORM == FORM_F32, BF16 = FORM == FORM_BF16, F16 = FORM == FORM_F16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int h = lane >> 5, j = lane & 31;
-    // local offset of group amplitude l: the sum of 2^bits[i] over the set bits i of l, formed
-    // where it is used from five values (a table of 32 offsets kept the scalar registers full)
+    // swizzled byte offset of group amplitude l: the XOR of 8 swz(2^bits[i]) over the set bits i
+    // of l, formed where it is used from five values (a table of 32 offsets kept the scalar
+    // registers full)
     uint32_t pb[K];
 #pragma unroll
-    for (int i = 0; i < K; ++i) pb[i] = 1u << op.bits[i];
+    for (int i = 0; i < K; ++i) pb[i] = swz(1u << op.bits[i]) << 3;
     auto off = [&](int l) {
         uint32_t o = 0;
 #pragma unroll
         for (int i = 0; i < K; ++i)
-            if ((l >> i) & 1) o += pb[i];
+            if ((l >> i) & 1) o ^= pb[i];
         return o;
     };
     const uint32_t hin = h ? pb[K - 1] : 0u, hout = h ? pb[1] : 0u;
     const unsigned char* fr = frags + op.frag_off;
     // the gate's A fragments, once per wave (the wave then takes 2 (K = 5) or 4 (K = 4) tiles)
-    bf16x8 ah[F32 ? 1 : MT][F32 ? 1 : TB], am[F32 ? 1 : MT][F32 ? 1 : TB], al[F32 ? 1 : MT][F32 ? 1 : TB];
+    constexpr int RB = BF16 ? MT : 1, TBB = BF16 ? TB : 1;
+    constexpr int RH = F16 ? MT : 1, TBH = F16 ? TB : 1;
+    bf16x8 ah[RB][TBB], am[RB][TBB], al[RB][TBB];
+    f16x8 hh[RH][TBH], hl[RH][TBH];
     float af[F32 ? MT : 1][F32 ? TF : 1];
     if constexpr (F32) {
         const float* f = (const float*)fr;
@@ -96,7 +112,7 @@ __device__ void mfma_gate(float2* __restrict__ lds, const DenseOpDesc& op, const
         for (int m = 0; m < MT; ++m)
 #pragma unroll
             for (int t = 0; t < TF; ++t) af[m][t] = f[(m * TF + t) * 64 + lane];
-    } else {
+    } else if constexpr (BF16) {
         const bf16x8* fh = (const bf16x8*)fr;
         const bf16x8* fm = fh + NB;
         const bf16x8* fl = fm + NB;
@@ -108,14 +124,25 @@ __device__ void mfma_gate(float2* __restrict__ lds, const DenseOpDesc& op, const
                 am[m][t] = fm[(m * TB + t) * 64 + lane];
                 al[m][t] = fl[(m * TB + t) * 64 + lane];
             }
+    } else {
+        const f16x8* fh = (const f16x8*)fr;
+        const f16x8* fl = fh + NB;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int t = 0; t < TB; ++t) {
+                hh[m][t] = fh[(m * TB + t) * 64 + lane];
+                hl[m][t] = fl[(m * TB + t) * 64 + lane];
+            }
     }
     for (int tile = wave; tile < NTILES; tile += kDThreads / 64) {
         uint32_t base = (uint32_t)tile * 32 + j;
 #pragma unroll
         for (int q = 0; q < K; ++q) base = ins0_32(base, op.sorted[q]);
+        base = swz(base) << 3;
         float2 x[HALF];
 #pragma unroll
-        for (int r = 0; r < HALF; ++r) x[r] = lds[base + hin + off(r)];
+        for (int r = 0; r < HALF; ++r) x[r] = at(lds, base ^ hin ^ off(r));
         f32x16 acc[MT];
 #pragma unroll
         for (int m = 0; m < MT; ++m)
@@ -129,17 +156,10 @@ __device__ void mfma_gate(float2* __restrict__ lds, const DenseOpDesc& op, const
                 for (int m = 0; m < MT; ++m)
                     acc[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[m][t], xv, acc[m], 0, 0, 0);
             }
-        } else {
+        } else if constexpr (BF16) {
             bf16x8 xh[TB], xl[TB];
 #pragma unroll
-            for (int t = 0; t < TB; ++t)
-#pragma unroll
-                for (int s = 0; s < 8; ++s) {
-                    const float v = (s & 1) ? x[4 * t + (s >> 1)].y : x[4 * t + (s >> 1)].x;
-                    const __bf16 hv = (__bf16)v;
-                    xh[t][s] = hv;
-                    xl[t][s] = (__bf16)(v - (float)hv);
-                }
+            for (int t = 0; t < TB; ++t) split_bf16(x + 4 * t, xh[t], xl[t]);
             // small terms first (as k_dense_mfma)
 #pragma unroll
             for (int m = 0; m < MT; ++m) {
@@ -156,6 +176,23 @@ __device__ void mfma_gate(float2* __restrict__ lds, const DenseOpDesc& op, const
                 for (int t = 0; t < TB; ++t)
                     acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m][t], xh[t], acc[m], 0, 0, 0);
             }
+        } else {
+            // the chunk is held scaled by 2^14 in LDS (applied at the load, removed at the store)
+            f16x8 xh[TB], xl[TB];
+#pragma unroll
+            for (int t = 0; t < TB; ++t) split_f16(x + 4 * t, 1.0f, xh[t], xl[t]);
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hl[m][t], xh[t], acc[m], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hh[m][t], xl[t], acc[m], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hh[m][t], xh[t], acc[m], 0, 0, 0);
+            }
         }
 #pragma unroll
         for (int m = 0; m < MT; ++m)
@@ -163,7 +200,7 @@ __device__ void mfma_gate(float2* __restrict__ lds, const DenseOpDesc& op, const
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b)
-                    lds[base + hout + off(16 * m + 4 * a + b)] =
+                    at(lds, base ^ hout ^ off(16 * m + 4 * a + b)) =
                         float2{acc[m][4 * a + 2 * b], acc[m][4 * a + 2 * b + 1]};
     }
 }
@@ -181,15 +218,16 @@ __device__ void valu_gate(float2* __restrict__ lds, const DenseOpDesc& op, const
 #pragma unroll
         for (int i = 0; i < K; ++i)
             if ((l >> i) & 1) o |= 1u << op.bits[i];
-        off[l] = o;
+        off[l] = swz(o);
     }
     for (int g = threadIdx.x; g < NG; g += kDThreads) {
         uint32_t base = (uint32_t)g;
 #pragma unroll
         for (int q = 0; q < K; ++q) base = ins0_32(base, op.sorted[q]);
+        base = swz(base);
         float2 v[D];
 #pragma unroll
-        for (int l = 0; l < D; ++l) v[l] = lds[base + off[l]];
+        for (int l = 0; l < D; ++l) v[l] = lds[base ^ off[l]];
 #pragma unroll
         for (int r = 0; r < D; ++r) {
             float re = 0.f, im = 0.f;
@@ -199,7 +237,7 @@ __device__ void valu_gate(float2* __restrict__ lds, const DenseOpDesc& op, const
                 re = fmaf(m.x, v[c].x, fmaf(-m.y, v[c].y, re));
                 im = fmaf(m.x, v[c].y, fmaf(m.y, v[c].x, im));
             }
-            lds[base + off[r]] = float2{re, im};
+            lds[base ^ off[r]] = float2{re, im};
         }
     }
 }
@@ -219,16 +257,17 @@ __device__ void valu_gate3(float2* __restrict__ lds, const DenseOpDesc& op, cons
 #pragma unroll
         for (int i = 0; i < K; ++i)
             if ((l >> i) & 1) o |= 1u << op.bits[i];
-        off[l] = o;
+        off[l] = swz(o);
     }
 #pragma unroll 1
     for (int g = threadIdx.x; g < NG; g += kDThreads) {
         uint32_t base = (uint32_t)g;
 #pragma unroll
         for (int q = 0; q < K; ++q) base = ins0_32(base, op.sorted[q]);
+        base = swz(base);
         float2 v[D];
 #pragma unroll
-        for (int l = 0; l < D; ++l) v[l] = lds[base + off[l]];
+        for (int l = 0; l < D; ++l) v[l] = lds[base ^ off[l]];
         // the group's amplitudes are all in registers: each row is written as it is formed
 #pragma unroll 1
         for (int r = 0; r < D; ++r) {
@@ -239,14 +278,15 @@ __device__ void valu_gate3(float2* __restrict__ lds, const DenseOpDesc& op, cons
                 re = fmaf(m.x, v[c].x, fmaf(-m.y, v[c].y, re));
                 im = fmaf(m.x, v[c].y, fmaf(m.y, v[c].x, im));
             }
-            const uint32_t o = ((r & 1) ? off[1] : 0u) + ((r & 2) ? off[2] : 0u) + ((r & 4) ? off[4] : 0u);
-            lds[base + o] = float2{re, im};
+            const uint32_t o = ((r & 1) ? off[1] : 0u) ^ ((r & 2) ? off[2] : 0u) ^ ((r & 4) ? off[4] : 0u);
+            lds[base ^ o] = float2{re, im};
         }
     }
 }
 
-// One chunk per work-group: load into LDS, every gate of the sweep, store.
-template <bool F32>
+// One chunk per work-group: load into LDS, every gate of the sweep, store. In the FP16 form
+// the chunk is held in LDS scaled by 2^14 (exact; every gate is linear).
+template <int FORM>
 __global__ void __launch_bounds__(kDThreads) k_dsweep(float2* __restrict__ amp, const DenseSweepDesc* __restrict__ sw,
                                                      const DenseOpDesc* __restrict__ ops,
                                                      const float2* __restrict__ mats,
@@ -272,15 +312,17 @@ __global__ void __launch_bounds__(kDThreads) k_dsweep(float2* __restrict__ amp, 
 #pragma unroll
         for (int l = 0; l < 5; ++l)
             if ((r >> l) & 1) ro |= 1ull << sw->S[8 + l];
-        lds[t + kDThreads * r] = amp[base | tl | ro];
+        float2 v = amp[base | tl | ro];
+        if constexpr (FORM == FORM_F16) v = float2{v.x * kF16Scale, v.y * kF16Scale};
+        lds[swz(t + kDThreads * r)] = v;
     }
     __syncthreads();
     const uint32_t n_ops = sw->n_ops;
     for (uint32_t o = 0; o < n_ops; ++o) {
         const DenseOpDesc& op = ops[sw->op_begin + o];
         if (op.kind == DOP_MFMA) {
-            if (op.k == 5) mfma_gate<5, F32>(lds, op, frags);
-            else mfma_gate<4, F32>(lds, op, frags);
+            if (op.k == 5) mfma_gate<5, FORM>(lds, op, frags);
+            else mfma_gate<4, FORM>(lds, op, frags);
         } else {
             switch (op.k) {
                 case 1: valu_gate<1>(lds, op, mats); break;
@@ -301,7 +343,9 @@ __global__ void __launch_bounds__(kDThreads) k_dsweep(float2* __restrict__ amp, 
 #pragma unroll
         for (int l = 0; l < 5; ++l)
             if ((r >> l) & 1) ro |= 1ull << sp[l];
-        amp[base | tl | ro] = lds[t + kDThreads * r];
+        float2 v = lds[swz(t + kDThreads * r)];
+        if constexpr (FORM == FORM_F16) v = float2{v.x * kF16Unscale, v.y * kF16Unscale};
+        amp[base | tl | ro] = v;
     }
 }
 
@@ -462,8 +506,8 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
         return;
     }
     const uint32_t dk = opts.dense_kernel;
-    if (dk > QLX_DENSE_MFMA_F32) fail(QLX_EINVAL, "unknown dense_kernel");
-    const int variant = dk == QLX_DENSE_MFMA_F32 ? QLX_DENSE_MFMA_F32 : QLX_DENSE_MFMA_BF16;
+    if (dk > QLX_DENSE_MFMA_F16) fail(QLX_EINVAL, "unknown dense_kernel");
+    const int variant = dk == QLX_DENSE_AUTO ? QLX_DENSE_MFMA_BF16 : (int)dk;
     const bool f32 = variant == QLX_DENSE_MFMA_F32;
 
     uint64_t key = 1469598103934665603ull;
@@ -592,10 +636,12 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
         const DenseSweepDesc* sw = (const DenseSweepDesc*)dev + s;
         const DenseOpDesc* ops = (const DenseOpDesc*)(dev + P.off_ops);
         const float2* mats = (const float2*)(dev + P.off_mats);
-        if (f32)
-            k_dsweep<true><<<(unsigned)chunks, kDThreads, 0, st->stream>>>((float2*)st->amp, sw, ops, mats, frags);
+        if (variant == QLX_DENSE_MFMA_F32)
+            k_dsweep<FORM_F32><<<(unsigned)chunks, kDThreads, 0, st->stream>>>((float2*)st->amp, sw, ops, mats, frags);
+        else if (variant == QLX_DENSE_MFMA_F16)
+            k_dsweep<FORM_F16><<<(unsigned)chunks, kDThreads, 0, st->stream>>>((float2*)st->amp, sw, ops, mats, frags);
         else
-            k_dsweep<false><<<(unsigned)chunks, kDThreads, 0, st->stream>>>((float2*)st->amp, sw, ops, mats, frags);
+            k_dsweep<FORM_BF16><<<(unsigned)chunks, kDThreads, 0, st->stream>>>((float2*)st->amp, sw, ops, mats, frags);
         QLX_HIP_CHECK(hipGetLastError());
         stats.sweeps_full++;
         stats.algorithmic_bytes += pass;

@@ -166,7 +166,7 @@ struct MfmaSplitJob {
     uint64_t pool_off;  // complex elements into the device pool
     uint64_t frag_off;  // bytes into the fragment buffer
     int k;              // 4 or 5
-    int variant;        // QLX_DENSE_MFMA_BF16 | QLX_DENSE_MFMA_F32
+    int variant;        // QLX_DENSE_MFMA_BF16 | QLX_DENSE_MFMA_F32 | QLX_DENSE_MFMA_F16
 };
 size_t mfma_fragment_bytes(int k, int variant);
 // Builds the A-operand fragments of each job from the FP64 pool (synchronous).

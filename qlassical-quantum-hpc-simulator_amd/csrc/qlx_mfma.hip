@@ -35,12 +35,11 @@
 #include <algorithm>
 
 #include "qlx_internal.hpp"
+#include "qlx_mfma_core.hpp"
 
 namespace qlx {
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kWaves = 4;
 constexpr int kBlock = 64 * kWaves;
@@ -71,7 +70,7 @@ struct SplitDesc {
     uint64_t pool_off;   // complex elements into the device pool
     uint64_t frag_off;   // bytes into the fragment buffer
     uint32_t k;          // 4 or 5
-    uint32_t f32;        // 1: FP32 fragments, 0: BF16 hi/mid/lo fragments
+    uint32_t form;       // FORM_BF16 (hi/mid/lo), FORM_F32, FORM_F16 (hi/lo)
 };
 
 // One work-group per matrix: writes the A-operand fragments of every (M-tile, K-step, lane).
@@ -82,12 +81,29 @@ __global__ void __launch_bounds__(kBlock) k_mfma_split(const double* __restrict_
     const int K = (int)d.k, D = 1 << K;
     const double* U = pool + 2 * d.pool_off;
     const int MT = (2 * D) / 32;
-    if (d.f32) {
+    if (d.form == FORM_F32) {
         const int T = D;  // 2D / 2 K-steps
         float* out = (float*)(frags + d.frag_off);
         for (int e = threadIdx.x; e < MT * T * 64; e += kBlock) {
             const int lane = e & 63, t = (e >> 6) % T, m = (e >> 6) / T;
             out[e] = (float)embed(U, D, 32 * m + (lane & 31), kcol_f32(K, t, lane >> 5));
+        }
+    } else if (d.form == FORM_F16) {
+        const int T = (2 * D) / 16;
+        f16x8* hi = (f16x8*)(frags + d.frag_off);
+        f16x8* lo = hi + MT * T * 64;
+        for (int e = threadIdx.x; e < MT * T * 64; e += kBlock) {
+            const int lane = e & 63, t = (e >> 6) % T, m = (e >> 6) / T;
+            f16x8 vh, vl;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const double v = embed(U, D, 32 * m + (lane & 31), kcol_bf16(K, t, lane >> 5, s));
+                const _Float16 h = (_Float16)(float)v;
+                vh[s] = h;
+                vl[s] = (_Float16)(float)(v - (double)(float)h);   // the residual is exact in FP64
+            }
+            hi[e] = vh;
+            lo[e] = vl;
         }
     } else {
         const int T = (2 * D) / 16;
@@ -123,9 +139,9 @@ struct MfmaArgs {
     uint32_t hbit_out;    // 1 << qubits[1]:   lane half 1 writes locals with bit 1 set
 };
 
-// ALDS: the A fragments stay in LDS (read per tile) instead of VGPRs: fewer registers per
-// wave, more waves per SIMD and more HBM loads in flight.
-template <int K, bool F32, bool ALDS>
+// ALDS (FP32 form only): the A fragments stay in LDS (read per tile) instead of VGPRs: fewer
+// registers per wave, more waves per SIMD and more HBM loads in flight.
+template <int K, int FORM, bool ALDS>
 __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp, uint64_t ngroups,
                                                       const unsigned char* __restrict__ frags, MfmaArgs ma) {
     constexpr int D = 1 << K;          // amplitudes per group
@@ -135,24 +151,21 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
     const int h = lane >> 5, j = lane & 31;
     const uint64_t hin = h ? ma.hbit_in : 0u, hout = h ? ma.hbit_out : 0u;
 
-    constexpr int TB = (2 * D) / 16;   // BF16 K-steps
+    constexpr bool F32 = FORM == FORM_F32, BF16 = FORM == FORM_BF16, F16 = FORM == FORM_F16;
+    static_assert(!ALDS || F32, "LDS-resident fragments are used in the FP32 form only");
+    constexpr int TB = (2 * D) / 16;   // 16-bit K-steps
     constexpr int TF = D;              // F32 K-steps
-    constexpr int NB = MT * TB * 64;   // bf16x8 fragments per split term
+    constexpr int NB = MT * TB * 64;   // 8 x 16-bit fragments per split term
     constexpr int NF = MT * TF * 64;   // float fragments
     constexpr int RA = ALDS ? 1 : MT;  // register copies
-    bf16x8 ah[F32 ? 1 : RA][F32 ? 1 : TB], am[F32 ? 1 : RA][F32 ? 1 : TB], al[F32 ? 1 : RA][F32 ? 1 : TB];
+    bf16x8 ah[BF16 ? MT : 1][BF16 ? TB : 1], am[BF16 ? MT : 1][BF16 ? TB : 1], al[BF16 ? MT : 1][BF16 ? TB : 1];
+    f16x8 hh[F16 ? MT : 1][F16 ? TB : 1], hl[F16 ? MT : 1][F16 ? TB : 1];
     float af[F32 ? RA : 1][F32 ? TF : 1];
-    __shared__ __align__(16) unsigned char sA[ALDS ? (F32 ? NF * 4 : 3 * NB * 16) : 16];
+    __shared__ __align__(16) unsigned char sA[ALDS ? NF * 4 : 16];
     if constexpr (ALDS) {
-        if constexpr (F32) {
-            const float* f = (const float*)frags;
-            float* d = (float*)sA;
-            for (int e = threadIdx.x; e < NF; e += kBlock) d[e] = f[e];
-        } else {
-            const bf16x8* f = (const bf16x8*)frags;
-            bf16x8* d = (bf16x8*)sA;
-            for (int e = threadIdx.x; e < 3 * NB; e += kBlock) d[e] = f[e];
-        }
+        const float* f = (const float*)frags;
+        float* d = (float*)sA;
+        for (int e = threadIdx.x; e < NF; e += kBlock) d[e] = f[e];
         __syncthreads();
     } else if constexpr (F32) {
         const float* f = (const float*)frags;
@@ -160,7 +173,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
         for (int m = 0; m < MT; ++m)
 #pragma unroll
             for (int t = 0; t < TF; ++t) af[m][t] = f[(m * TF + t) * 64 + lane];
-    } else {
+    } else if constexpr (BF16) {
         const bf16x8* fh = (const bf16x8*)frags;
         const bf16x8* fm = fh + NB;
         const bf16x8* fl = fm + NB;
@@ -172,12 +185,17 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
                 am[m][t] = fm[(m * TB + t) * 64 + lane];
                 al[m][t] = fl[(m * TB + t) * 64 + lane];
             }
+    } else {
+        const f16x8* fh = (const f16x8*)frags;
+        const f16x8* fl = fh + NB;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int t = 0; t < TB; ++t) {
+                hh[m][t] = fh[(m * TB + t) * 64 + lane];
+                hl[m][t] = fl[(m * TB + t) * 64 + lane];
+            }
     }
-    // fragment accessors (term: 0 hi, 1 mid, 2 lo)
-    auto fragb = [&](int term, int m, int t) -> bf16x8 {
-        if constexpr (ALDS) return ((const bf16x8*)sA)[term * NB + (m * TB + t) * 64 + lane];
-        else return term == 0 ? ah[m][t] : (term == 1 ? am[m][t] : al[m][t]);
-    };
     auto fragf = [&](int m, int t) -> float {
         if constexpr (ALDS) return ((const float*)sA)[(m * TF + t) * 64 + lane];
         else return af[m][t];
@@ -218,33 +236,44 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
                 for (int m = 0; m < MT; ++m)
                     acc[m] = __builtin_amdgcn_mfma_f32_32x32x2f32(fragf(m, t), xv, acc[m], 0, 0, 0);
             }
-        } else {
+        } else if constexpr (BF16) {
             bf16x8 xh[TB], xl[TB];
 #pragma unroll
-            for (int t = 0; t < TB; ++t)
-#pragma unroll
-                for (int s = 0; s < 8; ++s) {
-                    const float v = (s & 1) ? x[4 * t + (s >> 1)].y : x[4 * t + (s >> 1)].x;
-                    const __bf16 hv = (__bf16)v;
-                    xh[t][s] = hv;
-                    xl[t][s] = (__bf16)(v - (float)hv);
-                }
+            for (int t = 0; t < TB; ++t) split_bf16(x + 4 * t, xh[t], xl[t]);
             // small terms first
 #pragma unroll
             for (int m = 0; m < MT; ++m) {
 #pragma unroll
                 for (int t = 0; t < TB; ++t)
-                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fragb(2, m, t), xh[t], acc[m], 0, 0, 0);
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[m][t], xh[t], acc[m], 0, 0, 0);
 #pragma unroll
                 for (int t = 0; t < TB; ++t)
-                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fragb(1, m, t), xh[t], acc[m], 0, 0, 0);
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am[m][t], xh[t], acc[m], 0, 0, 0);
 #pragma unroll
                 for (int t = 0; t < TB; ++t)
-                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fragb(0, m, t), xl[t], acc[m], 0, 0, 0);
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m][t], xl[t], acc[m], 0, 0, 0);
 #pragma unroll
                 for (int t = 0; t < TB; ++t)
-                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fragb(0, m, t), xh[t], acc[m], 0, 0, 0);
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m][t], xh[t], acc[m], 0, 0, 0);
             }
+        } else {
+            f16x8 xh[TB], xl[TB];
+#pragma unroll
+            for (int t = 0; t < TB; ++t) split_f16(x + 4 * t, kF16Scale, xh[t], xl[t]);
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hl[m][t], xh[t], acc[m], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hh[m][t], xl[t], acc[m], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < TB; ++t)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hh[m][t], xh[t], acc[m], 0, 0, 0);
+            }
+#pragma unroll
+            for (int m = 0; m < MT; ++m) acc[m] *= kF16Unscale;
         }
 #pragma unroll
         for (int m = 0; m < MT; ++m)
@@ -274,16 +303,21 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(float2* __restrict__ amp,
     }
 }
 
-size_t frag_bytes(int k, bool f32) {
+int form_of(int variant) {
+    return variant == QLX_DENSE_MFMA_F32 ? FORM_F32 : variant == QLX_DENSE_MFMA_F16 ? FORM_F16 : FORM_BF16;
+}
+
+size_t frag_bytes(int k, int form) {
     const size_t D = size_t{1} << k, MT = 2 * D / 32;
-    return f32 ? MT * D * 64 * sizeof(float) : 3 * MT * (2 * D / 16) * 64 * 16;
+    if (form == FORM_F32) return MT * D * 64 * sizeof(float);
+    return (form == FORM_F16 ? 2 : 3) * MT * (2 * D / 16) * 64 * 16;
 }
 
 }  // namespace
 
 size_t mfma_fragment_bytes(int k, int variant) {
     if (k != 4 && k != 5) fail(QLX_EUNSUPPORTED, "MFMA dense gates take 4 or 5 qubits");
-    return frag_bytes(k, variant == QLX_DENSE_MFMA_F32);
+    return frag_bytes(k, form_of(variant));
 }
 
 void mfma_split(const double* d_pool, const MfmaSplitJob* jobs, size_t n, void* d_frags, DevBuf& desc_buf,
@@ -292,8 +326,7 @@ void mfma_split(const double* d_pool, const MfmaSplitJob* jobs, size_t n, void* 
     std::vector<SplitDesc> h(n);
     for (size_t i = 0; i < n; ++i) {
         if (jobs[i].k != 4 && jobs[i].k != 5) fail(QLX_EUNSUPPORTED, "MFMA dense gates take 4 or 5 qubits");
-        h[i] = SplitDesc{jobs[i].pool_off, jobs[i].frag_off, (uint32_t)jobs[i].k,
-                         jobs[i].variant == QLX_DENSE_MFMA_F32 ? 1u : 0u};
+        h[i] = SplitDesc{jobs[i].pool_off, jobs[i].frag_off, (uint32_t)jobs[i].k, (uint32_t)form_of(jobs[i].variant)};
     }
     desc_buf.reserve(n * sizeof(SplitDesc), stream);
     QLX_HIP_CHECK(hipMemcpyAsync(desc_buf.ptr, h.data(), n * sizeof(SplitDesc), hipMemcpyHostToDevice, stream));
@@ -335,16 +368,21 @@ void launch_dense_mfma(qlx_state* st, const void* d_frag, const uint8_t* qubits,
     const unsigned grid = (unsigned)std::min<uint64_t>(want, 256ull * 8);
     float2* amp = (float2*)st->amp;
     const auto* f = (const unsigned char*)d_frag;
-    const bool f32 = variant == QLX_DENSE_MFMA_F32;
     // measured on MI355X (32-qubit complex64, 5-qubit gates): the FP32 form gains from the
     // extra occupancy of LDS-resident fragments (18.2 -> 15.5 ms per gate), the BF16 form does
     // not (14.6 ms either way) and keeps them in registers
-    if (f32) {
-        if (k == 5) k_dense_mfma<5, true, true><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
-        else k_dense_mfma<4, true, true><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
-    } else {
-        if (k == 5) k_dense_mfma<5, false, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
-        else k_dense_mfma<4, false, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+    switch (form_of(variant)) {
+        case FORM_F32:
+            if (k == 5) k_dense_mfma<5, FORM_F32, true><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+            else k_dense_mfma<4, FORM_F32, true><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+            break;
+        case FORM_F16:
+            if (k == 5) k_dense_mfma<5, FORM_F16, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+            else k_dense_mfma<4, FORM_F16, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+            break;
+        default:
+            if (k == 5) k_dense_mfma<5, FORM_BF16, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
+            else k_dense_mfma<4, FORM_BF16, false><<<grid, kBlock, 0, st->stream>>>(amp, ngroups, f, ma);
     }
     QLX_HIP_CHECK(hipGetLastError());
 }
@@ -353,9 +391,9 @@ void prepare_dense(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const 
                    const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts& opts) {
     st->frag_index.clear();
     const uint32_t dk = opts.dense_kernel;
-    if (dk > QLX_DENSE_MFMA_F32) fail(QLX_EINVAL, "unknown dense_kernel");
+    if (dk > QLX_DENSE_MFMA_F16) fail(QLX_EINVAL, "unknown dense_kernel");
     if (st->dtype != QLX_C64 || dk == QLX_DENSE_VALU || !dpool || !meta) return;
-    const int variant = dk == QLX_DENSE_MFMA_F32 ? QLX_DENSE_MFMA_F32 : QLX_DENSE_MFMA_BF16;
+    const int variant = dk == QLX_DENSE_AUTO ? QLX_DENSE_MFMA_BF16 : (int)dk;
     std::vector<MfmaSplitJob> jobs;
     size_t total = 0;
     for (size_t i = 0; i < n; ++i) {

@@ -32,6 +32,7 @@ QLX_DENSE_AUTO = 0
 QLX_DENSE_VALU = 1
 QLX_DENSE_MFMA_BF16 = 2
 QLX_DENSE_MFMA_F32 = 3
+QLX_DENSE_MFMA_F16 = 4
 
 
 class QlxError(RuntimeError):

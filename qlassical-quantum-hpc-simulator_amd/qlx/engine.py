@@ -29,7 +29,7 @@ def _dt(dtype) -> int:
 
 
 DENSE = {"auto": _abi.QLX_DENSE_AUTO, "valu": _abi.QLX_DENSE_VALU, "bf16": _abi.QLX_DENSE_MFMA_BF16,
-         "f32": _abi.QLX_DENSE_MFMA_F32}
+         "f32": _abi.QLX_DENSE_MFMA_F32, "f16": _abi.QLX_DENSE_MFMA_F16}
 
 
 def exec_opts(mode: str = "plain", window_qubits: int = 0, coalesce_qubits: int = 0,

@@ -43,7 +43,7 @@ def test_generator_circuits_vs_oracle(gpu, kind, n, d):
     got = run(gpu, prog, init, "c64", gpu.exec_opts("windowed"))
     assert infidelity(want, got) <= 1e-6
     fused = gpu.transpile_gpu(prog, max_width=5, rotations=True)
-    for dense in ("bf16", "f32"):
+    for dense in ("bf16", "f32", "f16"):
         got = run(gpu, fused, init, "c64", gpu.exec_opts("plain", dense=dense))
         assert infidelity(want, got) <= 1e-6, dense
         # norm drift of the matrix-core path (BF16: the state's 16-bit split; FP32: rounding)

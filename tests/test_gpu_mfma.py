@@ -12,8 +12,11 @@ FP32; the split is made once per gate in FP64 on the device) and each state comp
 into two (hi + lo: 16 bits, relative error <= 2^-16), so one gate moves the state by a
 relative error of order 1e-5 that differs from amplitude to amplitude (stated bound:
 max |diff| <= 2e-4 * max |amp|, infidelity <= 1e-8 per gate). FP32 MFMA is plain FP32
-arithmetic: max |diff| <= 2e-6 * max |amp|, infidelity <= 1e-11 per gate. Infidelity is
-the state infidelity 1 - |<a|b>|^2 / (<a|a><b|b>); the norm is checked on its own.
+arithmetic: max |diff| <= 2e-6 * max |amp|, infidelity <= 1e-11 per gate. FP16 splits both
+operands into two fp16 terms (22 significand bits each; the state scaled by 2^14 first) and
+drops the lo x lo product (<= 2^-22 relative): max |diff| <= 4e-6 * max |amp|, infidelity
+<= 1e-10 per gate. Infidelity is the state infidelity 1 - |<a|b>|^2 / (<a|a><b|b>); the norm
+is checked on its own.
 """
 import numpy as np
 import pytest
@@ -22,7 +25,9 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"bf16": (2e-4, 1e-8), "f32": (2e-6, 1e-11), "valu": (2e-6, 1e-11)}
+TOL = {"bf16": (2e-4, 1e-8), "f32": (2e-6, 1e-11), "f16": (4e-6, 1e-10), "valu": (2e-6, 1e-11)}
+# infidelity bounds for whole circuits
+CIRCUIT_BOUND = {"bf16": 1e-6, "f16": 1e-8, "f32": 1e-9, "valu": 1e-9}
 
 
 def rand_unitary(rng, k):
@@ -74,7 +79,7 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("dense", ["bf16", "f32"])
+@pytest.mark.parametrize("dense", ["bf16", "f32", "f16"])
 @pytest.mark.parametrize("n,qubits", CASES)
 def test_dense_gate_matches_c128(gpu, dense, n, qubits):
     rng = np.random.default_rng(1000 * n + len(qubits))
@@ -108,13 +113,13 @@ def test_dense_mfma_repeated_gates_keep_norm(gpu):
     prog = c.release()
     psi = rand_state(rng, n)
     want = run_c128(gpu, prog, psi)
-    for dense, bound in [("bf16", 1e-6), ("f32", 1e-9)]:
+    for dense, bound in [("bf16", 1e-6), ("f32", 1e-9), ("f16", 1e-8)]:
         got, _ = run_c64(gpu, prog, psi, dense)
         assert infidelity(got, want) <= bound, dense
         assert abs(np.vdot(got, got).real - 1.0) <= 1e-4
 
 
-@pytest.mark.parametrize("dense", ["bf16", "f32", "valu"])
+@pytest.mark.parametrize("dense", ["bf16", "f32", "f16", "valu"])
 @pytest.mark.parametrize("mode", ["plain", "windowed"])
 def test_fused_layered_circuit_c64_vs_oracle(gpu, dense, mode):
     n, d = 16, 12
@@ -126,8 +131,7 @@ def test_fused_layered_circuit_c64_vs_oracle(gpu, dense, mode):
     want, _ = oracle.execute(n, prog.instrs, amps=psi)
     got, t = run_c64(gpu, fused, psi, dense, mode)
     assert ("k_dense_mfma" in t) == (dense != "valu"), t
-    bound = 1e-6 if dense == "bf16" else 1e-9
-    assert infidelity(got, want) <= bound
+    assert infidelity(got, want) <= CIRCUIT_BOUND[dense]
     assert np.abs(got - want).max() <= 50 * TOL[dense][0] * np.abs(want).max()
 
 
@@ -161,7 +165,7 @@ def test_dense_kernel_choice_rejected_when_unknown(gpu):
 
 
 # ---- dense window sweeps (QLX_EXEC_DENSE_SWEEPS) ---------------------------------------------
-@pytest.mark.parametrize("dense", ["bf16", "f32"])
+@pytest.mark.parametrize("dense", ["bf16", "f32", "f16"])
 def test_dense_sweeps_fused_circuits_vs_oracle(gpu, dense):
     """Fused programs (widths 5 and 4) in 13-qubit LDS windows, 4/5-qubit gates on the matrix
     cores, the rest as FP32 matrices: against the oracle's unfused circuit."""
@@ -173,8 +177,7 @@ def test_dense_sweeps_fused_circuits_vs_oracle(gpu, dense):
         want, _ = oracle.execute(n, prog.instrs, amps=psi)
         got, t = run_c64(gpu, fused, psi, dense, "dense")
         assert "k_dsweep" in t and "k_dense_mfma" not in t, t
-        bound = 1e-6 if dense == "bf16" else 1e-9
-        assert infidelity(got, want) <= bound, (n, w)
+        assert infidelity(got, want) <= CIRCUIT_BOUND[dense], (n, w)
         assert np.abs(got - want).max() <= 50 * TOL[dense][0] * np.abs(want).max()
         assert abs(np.vdot(got, got).real - 1.0) <= 1e-5
 
@@ -199,9 +202,10 @@ def test_dense_sweeps_every_gate_kind_vs_oracle(gpu):
             psi = rand_state(rng, n)
             want, _ = oracle.execute(n, prog.instrs, prog.pool if prog.pool.size else None,
                                      prog.meta if len(prog.meta) else None, amps=psi)
-            got, t = run_c64(gpu, prog, psi, "bf16", "dense")
-            assert ("k_dsweep" in t) == (n >= 13), (n, t)
-            assert infidelity(got, want) <= 1e-6, (seed, n)
+            for dense in ("bf16", "f16"):
+                got, t = run_c64(gpu, prog, psi, dense, "dense")
+                assert ("k_dsweep" in t) == (n >= 13), (n, t)
+                assert infidelity(got, want) <= CIRCUIT_BOUND[dense], (seed, n, dense)
 
 
 def test_dense_sweeps_repeat_and_switch_programs(gpu):
@@ -219,3 +223,31 @@ def test_dense_sweeps_repeat_and_switch_programs(gpu):
     got = st.to_host()
     st.close()
     assert infidelity(got, want) <= 1e-9
+
+
+@pytest.mark.parametrize("mode", ["plain", "dense"])
+def test_f16_form_on_concentrated_and_basis_states(gpu, mode):
+    """The FP16 form scales amplitudes by 2^14 before the split: a state with one amplitude of
+    modulus ~1 (scaled: ~2^14, below the FP16 maximum 65504) and a basis state (zeros
+    everywhere else) stay exact to the FP16 bound; a state whose amplitudes span many orders
+    of magnitude keeps the small ones to the absolute bound."""
+    rng = np.random.default_rng(17)
+    n = 15
+    c = gpu.Circuit(n)
+    for _ in range(8):
+        qs = [int(x) for x in rng.choice(n, 5, replace=False)]
+        c.unitary(qs, rand_unitary(rng, 5), max_arity=5)
+    prog = c.release()
+    basis = np.zeros(1 << n, complex)
+    basis[12345] = 1.0
+    peaked = rand_state(rng, n) * 1e-3
+    peaked[77] = 1.0
+    peaked /= np.linalg.norm(peaked)
+    graded = rand_state(rng, n) * np.exp(-np.arange(1 << n) / 2000.0)
+    graded /= np.linalg.norm(graded)
+    for psi in (basis, peaked, graded):
+        want, _ = oracle.execute(n, prog.instrs, prog.pool, prog.meta, amps=psi)
+        got, t = run_c64(gpu, prog, psi, "f16", mode)
+        assert ("k_dsweep" in t) == (mode == "dense"), t
+        assert infidelity(got, want) <= CIRCUIT_BOUND["f16"]
+        assert np.abs(got - want).max() <= 8 * TOL["f16"][0] * np.abs(want).max()
