@@ -97,8 +97,9 @@ def measured_traffic(kernel, workload):
 def run_precision(args):
     """The 32-qubit depth-30 layered circuit (seed 1003) as a complex64 state (32 GiB),
     its width-5 fused blocks (matrices formed in FP64 on the GPU) applied as MFMA
-    contractions inside 13-qubit LDS windows (the reported variant; the others are timed
-    beside it), checked against the complex128 state (64 GiB) of the unfused circuit
+    contractions inside 13-qubit LDS windows with FP16 hi/lo split operands accumulated in
+    FP32 (the reported variant; the BF16 split, FP32 and vector-unit forms are timed beside
+    it), checked against the complex128 state (64 GiB) of the unfused circuit
     run on the GPU in FP64 (windowed path, pinned to the CPU oracle at smaller sizes):
     max |amplitude error| and infidelity 1 - |<psi64|psi128>|^2, FP64 reductions on the GPU."""
     import numpy as np
@@ -126,7 +127,9 @@ def run_precision(args):
             return qlx.exec_opts("windowed", **spec, **win)
         if mode == "windowed":
             return qlx.exec_opts(mode, dense=dense, **spec, **win)
-        return qlx.exec_opts(mode, dense=dense)   # plain, or dense sweeps
+        if mode == "dense":
+            return qlx.exec_opts(mode, dense=dense, relabel=args.relabel)
+        return qlx.exec_opts(mode, dense=dense)   # plain
     def prog_of(v):
         return prog if v.startswith("unfused") else fused
     res = {}
@@ -425,7 +428,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", choices=["circuit30", "precision", "batch", "serve", "clustered"],
                     default="circuit30")
-    ap.add_argument("--dense-variants", default="dense:bf16,windowed:bf16,dense:f32,windowed:valu,unfused:windowed",
+    ap.add_argument("--dense-variants", default="dense:f16,dense:bf16,windowed:f16,dense:f32,windowed:valu,unfused:windowed",
                     help="precision config: mode:dense-arithmetic forms to time (the first one is reported)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)

@@ -99,13 +99,16 @@ typedef struct qlx_exec_opts {
     uint32_t dense_kernel;    /* QLX_DENSE_*: arithmetic of UNITARY / FUSED_BLOCK gates */
     uint32_t sweep_workgroups; /* window sweeps with <= 16 registers per work-item: work-groups
                                   per launch (each loops over chunks); 0 = every resident slot */
-    uint32_t specialize;      /* windowed: QLX_SPECIALIZE_* flags; 0 = the generic sweep kernel */
+    uint32_t specialize;      /* QLX_SPECIALIZE_* flags. windowed: 0 = the generic sweep kernel;
+                                 dense sweeps: QLX_SPECIALIZE_RELABEL alone selects out-of-place
+                                 relabelled sweeps */
 } qlx_exec_opts;
 
 /* qlx_exec_opts::specialize flags.
  * QLX_SPECIALIZE: compile the program's sweeps into straight-line kernels (hipRTC, once per
  *   program and state; later executions of the same program reuse them).
- * QLX_SPECIALIZE_RELABEL (with QLX_SPECIALIZE): every sweep reads the state from one buffer
+ * QLX_SPECIALIZE_RELABEL (windowed: with QLX_SPECIALIZE; dense sweeps: on its own): every
+ *   sweep reads the state from one buffer
  *   and writes it to a second one with its qubits re-laid out so that the next sweep's window
  *   sits on the low address bits (short strides, fewer DRAM pages per chunk); the last sweep
  *   writes the logical order back. Needs a second device buffer of the state's size (allocated
@@ -118,7 +121,7 @@ typedef struct qlx_exec_opts {
  * 5-qubit gates on complex64 states; every other dense gate runs on the vector units
  * (FP64 for complex128, FP32 for complex64). The matrices stay FP64 in the pool and are
  * converted on the device. */
-#define QLX_DENSE_AUTO        0  /* complex64 4/5-qubit gates on states of >= k+5 qubits: MFMA BF16 */
+#define QLX_DENSE_AUTO        0  /* complex64 4/5-qubit gates on states of >= k+5 qubits: MFMA FP16 */
 #define QLX_DENSE_VALU        1  /* vector units only */
 #define QLX_DENSE_MFMA_BF16   2  /* BF16 split operands (matrix hi+mid+lo, state hi+lo), four
                                     products accumulated in FP32 */

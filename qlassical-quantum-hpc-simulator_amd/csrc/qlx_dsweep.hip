@@ -20,9 +20,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <complex>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "qlx_internal.hpp"
@@ -38,12 +40,24 @@ constexpr int kDThreads = 256;
 constexpr int kDChunk = 1 << kDW;
 
 
+constexpr int kMaxChunkBits = 32;    // chunk-index bits of one launch (2^32 work-groups at most)
+
+// A sweep's addressing: local bit l of the chunk (local bit l <-> window qubit S[l], S
+// ascending) and chunk-index bit j (the qubits outside the window, in the order of their load
+// address bits) sit at address bit ld_*[.] when the chunk is read and st_*[.] when it is
+// written. In place both layouts are the identity; with relabelling (out of place, between two
+// buffers) the store layout puts the next window on the low address bits.
 struct DenseSweepDesc {
-    uint32_t S[kDW];       // window qubits ascending (local bit l <-> global qubit S[l])
     uint32_t op_begin, n_ops;
-    uint32_t pad[1];
+    uint32_t nc;                    // chunk-index bits (state qubits - kDW)
+    uint32_t pad;
+    uint32_t ld_l[kDW], st_l[kDW];
+    uint32_t ld_c[kMaxChunkBits], st_c[kMaxChunkBits];
+    uint32_t pad2[2];
 };
-static_assert(sizeof(DenseSweepDesc) % 4 == 0, "descriptor layout");
+static_assert(sizeof(DenseSweepDesc) % 16 == 0, "descriptor layout");
+static_assert(offsetof(DenseSweepDesc, st_c) == offsetof(DenseSweepDesc, ld_c) + 4 * kMaxChunkBits,
+              "st_c follows ld_c (read as one 64-entry array)");
 
 enum : uint32_t { DOP_VALU = 0, DOP_MFMA = 1 };
 
@@ -285,34 +299,49 @@ __device__ void valu_gate3(float2* __restrict__ lds, const DenseOpDesc& op, cons
 }
 
 // One chunk per work-group: load into LDS, every gate of the sweep, store. In the FP16 form
-// the chunk is held in LDS scaled by 2^14 (exact; every gate is linear).
+// the chunk is held in LDS scaled by 2^14 (exact; every gate is linear). `src` and `dst` are
+// the same buffer in place; a work-group reads all of its chunk before writing any of it.
 template <int FORM>
-__global__ void __launch_bounds__(kDThreads) k_dsweep(float2* __restrict__ amp, const DenseSweepDesc* __restrict__ sw,
+__global__ void __launch_bounds__(kDThreads) __attribute__((amdgpu_waves_per_eu(2))) k_dsweep(const float2* src, float2* dst,
+                                                     const DenseSweepDesc* __restrict__ sw,
                                                      const DenseOpDesc* __restrict__ ops,
                                                      const float2* __restrict__ mats,
                                                      const unsigned char* __restrict__ frags) {
     __shared__ float2 lds[kDChunk];
     __shared__ float2 mbuf[64];  // the current 3-qubit gate's matrix
     const unsigned t = threadIdx.x;
-    uint64_t base = blockIdx.x;
+    // the chunk's base address at the load and at the store: lane j < 32 of every wave places
+    // bit j of the work-group index at its load address bit, lane 32 + j at its store address bit
+    // (bits at or above nc are zero), and each half of the wave ORs its values together
+    uint64_t base, sbase;
+    {
+        const uint32_t lane = t & 63u;
+        const uint64_t bit = (blockIdx.x >> (lane & 31u)) & 1u;
+        const uint32_t* cpos = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(sw) +
+                                                                 offsetof(DenseSweepDesc, ld_c));
+        uint64_t v = bit << cpos[lane];   // cpos[32 + j] is st_c[j]
 #pragma unroll
-    for (int l = 0; l < kDW; ++l) {
-        const uint32_t q = sw->S[l];
-        const uint64_t lo = base & ((1ull << q) - 1);
-        base = lo | ((base ^ lo) << 1);
+        for (int m = 1; m < 32; m <<= 1) v |= __shfl_xor(v, m);
+        base = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), 0) << 32) |
+               (uint32_t)__builtin_amdgcn_readlane((uint32_t)v, 0);
+        sbase = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), 32) << 32) |
+                (uint32_t)__builtin_amdgcn_readlane((uint32_t)v, 32);
     }
     // local index i = t + 256 r: bits 0-7 from the work-item, bits 8-12 from r
-    uint64_t tl = 0;
+    uint64_t tl = 0, stl = 0;
 #pragma unroll
-    for (int l = 0; l < 8; ++l) tl |= (uint64_t)((t >> l) & 1u) << sw->S[l];
+    for (int l = 0; l < 8; ++l) {
+        tl |= (uint64_t)((t >> l) & 1u) << sw->ld_l[l];
+        stl |= (uint64_t)((t >> l) & 1u) << sw->st_l[l];
+    }
     constexpr int R = kDChunk / kDThreads;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         uint64_t ro = 0;
 #pragma unroll
         for (int l = 0; l < 5; ++l)
-            if ((r >> l) & 1) ro |= 1ull << sw->S[8 + l];
-        float2 v = amp[base | tl | ro];
+            if ((r >> l) & 1) ro |= 1ull << sw->ld_l[8 + l];
+        float2 v = src[base | tl | ro];
         if constexpr (FORM == FORM_F16) v = float2{v.x * kF16Scale, v.y * kF16Scale};
         lds[swz(t + kDThreads * r)] = v;
     }
@@ -332,11 +361,13 @@ __global__ void __launch_bounds__(kDThreads) k_dsweep(float2* __restrict__ amp, 
         }
         __syncthreads();
     }
-    // the store's register-bit offsets are formed again from positions read again (volatile):
-    // kept from the load, the 32 64-bit offsets would hold scalar registers through every gate
+    // the store's register-bit positions are read here (volatile): read with the others, the 32
+    // register offsets would hold scalar registers through every gate
+    const volatile uint32_t* vd = reinterpret_cast<const volatile uint32_t*>(sw);
+    constexpr int kStL = offsetof(DenseSweepDesc, st_l) / 4;
     uint32_t sp[5];
 #pragma unroll
-    for (int l = 0; l < 5; ++l) sp[l] = ((const volatile uint32_t*)sw->S)[8 + l];
+    for (int l = 0; l < 5; ++l) sp[l] = vd[kStL + 8 + l];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         uint64_t ro = 0;
@@ -345,7 +376,7 @@ __global__ void __launch_bounds__(kDThreads) k_dsweep(float2* __restrict__ amp, 
             if ((r >> l) & 1) ro |= 1ull << sp[l];
         float2 v = lds[swz(t + kDThreads * r)];
         if constexpr (FORM == FORM_F16) v = float2{v.x * kF16Unscale, v.y * kF16Unscale};
-        amp[base | tl | ro] = v;
+        dst[sbase | stl | ro] = v;
     }
 }
 
@@ -507,8 +538,9 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
     }
     const uint32_t dk = opts.dense_kernel;
     if (dk > QLX_DENSE_MFMA_F16) fail(QLX_EINVAL, "unknown dense_kernel");
-    const int variant = dk == QLX_DENSE_AUTO ? QLX_DENSE_MFMA_BF16 : (int)dk;
+    const int variant = dk == QLX_DENSE_AUTO ? QLX_DENSE_MFMA_F16 : (int)dk;
     const bool f32 = variant == QLX_DENSE_MFMA_F32;
+    const bool relabel = (opts.specialize & QLX_SPECIALIZE_RELABEL) != 0;
 
     uint64_t key = 1469598103934665603ull;
     auto mix = [&](const void* p, size_t b) {
@@ -541,12 +573,59 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
         std::vector<double> fpool;             // FP64 matrices of the matrix-core blocks
         std::vector<MfmaSplitJob> jobs;
         size_t frag_total = 0;
-        for (const DenseWindow& w : plan_dense_windows(st->nq, instrs, n, opts, kDW, kDC)) {
-            DenseSweepDesc d{};
-            for (int l = 0; l < kDW; ++l) d.S[l] = (uint32_t)w.S[l];
+        const int nq = (int)st->nq;
+        const std::vector<DenseWindow> wins = plan_dense_windows(st->nq, instrs, n, opts, kDW, kDC);
+        // pos[q]: address bit of logical qubit q before the current sweep (identity at the start
+        // and, after the last sweep, at the end)
+        std::vector<int> pos(nq);
+        for (int q = 0; q < nq; ++q) pos[q] = q;
+        for (size_t si = 0; si < wins.size(); ++si) {
+            const DenseWindow& w = wins[si];
             if (w.S.size() != (size_t)kDW) fail(QLX_EHIP, "dense sweep window of the wrong size");
             for (int l = 0; l < kDC; ++l)
                 if (w.S[l] != l) fail(QLX_EHIP, "dense sweep window without the low qubits");
+            std::vector<int> next(nq);
+            for (int q = 0; q < nq; ++q) next[q] = q;
+            if (relabel && si + 1 < wins.size()) {
+                // the low qubits stay; then the next window, this window, the rest in load order
+                std::vector<char> placed(nq, 0);
+                int at = 0;
+                auto place = [&](int q) {
+                    if (!placed[q]) {
+                        placed[q] = 1;
+                        next[q] = at++;
+                    }
+                };
+                for (int q = 0; q < kDC; ++q) place(q);
+                for (int q : wins[si + 1].S) place(q);
+                for (int q : w.S) place(q);
+                std::vector<int> rest;
+                for (int q = 0; q < nq; ++q)
+                    if (!placed[q]) rest.push_back(q);
+                std::sort(rest.begin(), rest.end(), [&](int a, int b) { return pos[a] < pos[b]; });
+                for (int q : rest) place(q);
+            }
+            DenseSweepDesc d{};
+            std::vector<char> inw(nq, 0);
+            for (int q : w.S) inw[q] = 1;
+            std::vector<int> outside;
+            for (int q = 0; q < nq; ++q)
+                if (!inw[q]) outside.push_back(q);
+            std::sort(outside.begin(), outside.end(), [&](int a, int b) { return pos[a] < pos[b]; });
+            if (outside.size() > (size_t)kMaxChunkBits) fail(QLX_EUNSUPPORTED, "state too large for dense sweeps");
+            d.nc = (uint32_t)outside.size();
+            for (size_t j = 0; j < outside.size(); ++j) {
+                d.ld_c[j] = (uint32_t)pos[outside[j]];
+                d.st_c[j] = (uint32_t)next[outside[j]];
+            }
+            for (int l = 0; l < kDW; ++l) {
+                d.ld_l[l] = (uint32_t)pos[w.S[l]];
+                d.st_l[l] = (uint32_t)next[w.S[l]];
+            }
+            for (int l = 0; l < kDC; ++l)
+                if (d.ld_l[l] != (uint32_t)l || d.st_l[l] != (uint32_t)l)
+                    fail(QLX_EHIP, "dense sweep layout moved a low qubit");
+            pos = next;
             d.op_begin = (uint32_t)P.ops.size();
             std::vector<int> loc(st->nq, -1);
             for (int l = 0; l < kDW; ++l) loc[w.S[l]] = l;
@@ -631,18 +710,28 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
     if (chunks > 0xffffffffull) fail(QLX_EUNSUPPORTED, "state too large for one dense-sweep launch");
     const double pass = pass_bytes(st);
     const unsigned char* frags = (const unsigned char*)P.frags.ptr;
+    if (relabel && !st->alt) {
+        if (hipMalloc(&st->alt, st->bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            st->alt = nullptr;
+            fail(QLX_ENOMEM, "relabelled dense sweeps need a second buffer of " + std::to_string(st->bytes) + " bytes");
+        }
+    }
     for (size_t s = 0; s < P.sweeps.size(); ++s) {
         TimedLaunch tl(st, "k_dsweep", pass);
         const DenseSweepDesc* sw = (const DenseSweepDesc*)dev + s;
         const DenseOpDesc* ops = (const DenseOpDesc*)(dev + P.off_ops);
         const float2* mats = (const float2*)(dev + P.off_mats);
+        const float2* src = (const float2*)st->amp;
+        float2* dst = (float2*)(relabel ? st->alt : st->amp);
         if (variant == QLX_DENSE_MFMA_F32)
-            k_dsweep<FORM_F32><<<(unsigned)chunks, kDThreads, 0, st->stream>>>((float2*)st->amp, sw, ops, mats, frags);
+            k_dsweep<FORM_F32><<<(unsigned)chunks, kDThreads, 0, st->stream>>>(src, dst, sw, ops, mats, frags);
         else if (variant == QLX_DENSE_MFMA_F16)
-            k_dsweep<FORM_F16><<<(unsigned)chunks, kDThreads, 0, st->stream>>>((float2*)st->amp, sw, ops, mats, frags);
+            k_dsweep<FORM_F16><<<(unsigned)chunks, kDThreads, 0, st->stream>>>(src, dst, sw, ops, mats, frags);
         else
-            k_dsweep<FORM_BF16><<<(unsigned)chunks, kDThreads, 0, st->stream>>>((float2*)st->amp, sw, ops, mats, frags);
+            k_dsweep<FORM_BF16><<<(unsigned)chunks, kDThreads, 0, st->stream>>>(src, dst, sw, ops, mats, frags);
         QLX_HIP_CHECK(hipGetLastError());
+        if (relabel) std::swap(st->amp, st->alt);
         stats.sweeps_full++;
         stats.algorithmic_bytes += pass;
     }

@@ -393,7 +393,7 @@ void prepare_dense(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const 
     const uint32_t dk = opts.dense_kernel;
     if (dk > QLX_DENSE_MFMA_F16) fail(QLX_EINVAL, "unknown dense_kernel");
     if (st->dtype != QLX_C64 || dk == QLX_DENSE_VALU || !dpool || !meta) return;
-    const int variant = dk == QLX_DENSE_AUTO ? QLX_DENSE_MFMA_BF16 : (int)dk;
+    const int variant = dk == QLX_DENSE_AUTO ? QLX_DENSE_MFMA_F16 : (int)dk;
     std::vector<MfmaSplitJob> jobs;
     size_t total = 0;
     for (size_t i = 0; i < n; ++i) {

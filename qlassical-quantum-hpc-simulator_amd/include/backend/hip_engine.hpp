@@ -46,9 +46,12 @@ struct HIPEngineOptions {
     // windowed: compile each program's sweeps into straight-line kernels (seconds of hipRTC
     // time on a program's first execution, reused afterwards: for programs run many times)
     bool specialize = false;
-    // with specialize: out-of-place sweeps through a second buffer of the state's size that
-    // keep every sweep's window on the low address bits
+    // with specialize (or dense_sweeps): out-of-place sweeps through a second buffer of the
+    // state's size that keep every sweep's window on the low address bits
     bool relabel = false;
+    // complex64 fused programs: every gate inside 13-qubit LDS windows, 4/5-qubit gates on the
+    // matrix cores (QLX_EXEC_DENSE_SWEEPS; takes precedence over windowed)
+    bool dense_sweeps = false;
 };
 
 class HIPEngine final : public ExecutionEngine {
@@ -91,9 +94,12 @@ public:
 
     void execute(std::span<const GateInstr> program, const UnitaryPool& pool) override {
         qlx_exec_opts o{};
-        o.mode = opts_.windowed ? QLX_EXEC_WINDOWED : QLX_EXEC_PLAIN;
+        o.mode = opts_.dense_sweeps ? QLX_EXEC_DENSE_SWEEPS : opts_.windowed ? QLX_EXEC_WINDOWED : QLX_EXEC_PLAIN;
         o.gswap_mode = opts_.global_swap_noop ? QLX_GSWAP_NOOP : QLX_GSWAP_PERMUTE;
-        o.specialize = opts_.specialize ? (QLX_SPECIALIZE | (opts_.relabel ? QLX_SPECIALIZE_RELABEL : 0u)) : 0u;
+        if (opts_.dense_sweeps)
+            o.specialize = opts_.relabel ? QLX_SPECIALIZE_RELABEL : 0u;
+        else
+            o.specialize = opts_.specialize ? (QLX_SPECIALIZE | (opts_.relabel ? QLX_SPECIALIZE_RELABEL : 0u)) : 0u;
         const auto* instrs = reinterpret_cast<const qlx_gate_instr*>(program.data());
         const auto* pdata = reinterpret_cast<const double*>(pool.data.data());
         const auto* meta = reinterpret_cast<const qlx_matrix_entry*>(pool.metadata.data());

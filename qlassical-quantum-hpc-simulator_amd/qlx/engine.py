@@ -36,9 +36,10 @@ def exec_opts(mode: str = "plain", window_qubits: int = 0, coalesce_qubits: int 
               gswap: str = "permute", register_qubits: int = 0, dense: str = "auto",
               sweep_workgroups: int = 0, specialize: bool = False, relabel: bool = False) -> _abi.ExecOpts:
     """qlx_exec_opts. dense: arithmetic of 4/5-qubit dense gates on complex64 states
-    ('auto' = 'bf16' matrix cores, 'f32' matrix cores, 'valu' vector units). relabel (with
-    specialize): out-of-place sweeps that re-lay out the qubits between two buffers."""
-    if relabel and not specialize:
+    ('auto' = 'f16' matrix cores (FP16 hi/lo split), 'bf16' matrix cores (BF16 split), 'f32'
+    matrix cores, 'valu' vector units). relabel (windowed: with specialize; dense: alone):
+    out-of-place sweeps that re-lay out the qubits between two buffers."""
+    if relabel and not specialize and mode != "dense":
         raise ValueError("relabel needs specialize")
     o = _abi.ExecOpts()
     o.dense_kernel = DENSE[dense]

@@ -46,10 +46,10 @@ def infidelity(a, b):
     return 1.0 - abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
 
 
-def run_c64(qlx, prog, psi, dense, mode="plain"):
+def run_c64(qlx, prog, psi, dense, mode="plain", relabel=False):
     st = qlx.State(amps=psi, dtype="c64")
     st.timing(True)
-    st.execute(prog, qlx.exec_opts(mode, dense=dense))
+    st.execute(prog, qlx.exec_opts(mode, dense=dense, relabel=relabel))
     t = st.timing_read()
     out = st.to_host()
     st.close()
@@ -165,8 +165,9 @@ def test_dense_kernel_choice_rejected_when_unknown(gpu):
 
 
 # ---- dense window sweeps (QLX_EXEC_DENSE_SWEEPS) ---------------------------------------------
+@pytest.mark.parametrize("relabel", [False, True])
 @pytest.mark.parametrize("dense", ["bf16", "f32", "f16"])
-def test_dense_sweeps_fused_circuits_vs_oracle(gpu, dense):
+def test_dense_sweeps_fused_circuits_vs_oracle(gpu, dense, relabel):
     """Fused programs (widths 5 and 4) in 13-qubit LDS windows, 4/5-qubit gates on the matrix
     cores, the rest as FP32 matrices: against the oracle's unfused circuit."""
     rng = np.random.default_rng(91)
@@ -175,7 +176,7 @@ def test_dense_sweeps_fused_circuits_vs_oracle(gpu, dense):
         fused = gpu.transpile_gpu(prog, max_width=w, rotations=True)
         psi = rand_state(rng, n)
         want, _ = oracle.execute(n, prog.instrs, amps=psi)
-        got, t = run_c64(gpu, fused, psi, dense, "dense")
+        got, t = run_c64(gpu, fused, psi, dense, "dense", relabel=relabel)
         assert "k_dsweep" in t and "k_dense_mfma" not in t, t
         assert infidelity(got, want) <= CIRCUIT_BOUND[dense], (n, w)
         assert np.abs(got - want).max() <= 50 * TOL[dense][0] * np.abs(want).max()
@@ -251,3 +252,25 @@ def test_f16_form_on_concentrated_and_basis_states(gpu, mode):
         assert ("k_dsweep" in t) == (mode == "dense"), t
         assert infidelity(got, want) <= CIRCUIT_BOUND["f16"]
         assert np.abs(got - want).max() <= 8 * TOL["f16"][0] * np.abs(want).max()
+
+
+def test_dense_sweeps_relabel_mixed_with_in_place(gpu):
+    """Relabelled (out-of-place) dense sweeps leave the state in logical order in whichever of
+    the two buffers the last sweep wrote: in-place runs, relabelled runs with an odd and an even
+    number of sweeps and host reads interleave on one state."""
+    n = 16
+    rng = np.random.default_rng(23)
+    progs = [gpu.transpile_gpu(gpu.layered(n, d, 40 + d), max_width=5, rotations=True) for d in (3, 7, 9)]
+    psi = rand_state(rng, n)
+    want = psi
+    st = gpu.State(amps=psi, dtype="c64")
+    sweeps = set()
+    for prog, relabel in [(progs[0], True), (progs[1], False), (progs[2], True), (progs[1], True),
+                          (progs[0], False)]:
+        want, _ = oracle.execute(n, prog.instrs, prog.pool, prog.meta, amps=want)
+        st.execute(prog, gpu.exec_opts("dense", dense="f16", relabel=relabel))
+        sweeps.add(st.last_stats()["sweeps_full"] % 2)
+        got = st.to_host()
+        assert infidelity(got, want) <= CIRCUIT_BOUND["f16"], relabel
+    st.close()
+    assert sweeps == {0, 1}          # (depths 3, 7, 9: 2, 3 and 4 sweeps)
