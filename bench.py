@@ -167,6 +167,15 @@ def run_precision(args):
                   "norm_sq_minus_1": n64 - 1.0}
         log(f"precision: {v}: {acc[v]}")
     ref.close()
+    copy_ref = None
+    if args.calib_copy:
+        dst = qlx.State(nq, "c64")
+        dst.timing(True)
+        dst.copy_from(st)
+        t = dst.timing_read()["k_copy"]
+        copy_ref = {"kernel": "k_copy", "bytes": t[2], "ms": round(t[1], 4), "gbps": round(t[2] / (t[1] / 1e3) / 1e9, 1)}
+        log(f"precision: k_copy {copy_ref}")
+        dst.close()
     st.close()
     main_v = variants[0]
     r = res[main_v]
@@ -175,6 +184,9 @@ def run_precision(args):
     launches, tot_ms, tot_bytes = r["timing"][name]
     avg_ms = tot_ms / launches
     achieved = (tot_bytes / launches) / (avg_ms / 1e3)
+    mode, dense = main_v.split(":")
+    traffic = measured_traffic(name, f"layered-{nq}q-depth{depth}-seed{seed}-c64-fused5-{mode}-{dense}"
+                                     f"{'-relabelled' if args.relabel and mode == 'dense' else ''}")
     out = {
         "metric": METRIC, "value": round(gbps, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(r["ms"], 3), "higher_is_better": True, "scaling": "weak",
@@ -189,10 +201,15 @@ def run_precision(args):
         },
         "roofline": {
             "bound": "hbm", "kernel": name, "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9,
-            "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4),
+            "traffic": traffic[0] if traffic else None,
+            "traffic_source": (f"{traffic[1]}: rocprofv3 FETCH_SIZE and WRITE_SIZE per launch of this kernel on "
+                               f"this workload, scaled by a copy kernel's counters" if traffic else None),
             "launches": launches, "avg_ms": round(avg_ms, 4), "bytes_per_launch": tot_bytes / launches,
         },
     }
+    if copy_ref:
+        out["copy_reference"] = copy_ref
     print(json.dumps(out), flush=True)
 
 
