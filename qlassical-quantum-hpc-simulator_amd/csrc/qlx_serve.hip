@@ -9,14 +9,21 @@
 // New requests join their group's next launch, finished ones leave: iteration-level
 // (continuous) batching. Groups run on different streams, so their work overlaps on the
 // device. Up to two launches per group are in flight; a request's sweeps are ordered by
-// its group's stream.
+// its group's stream. Plans are formed on host planner threads (a few ms per circuit of
+// 20-28 qubits): submit returns at once, and admission takes the queue's head as soon as
+// its plan is ready, so planning overlaps the device's work.
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "qlx_sweep_core.hpp"
@@ -105,6 +112,108 @@ private:
     uint64_t size_ = 0;
 };
 
+// A request's window-sweep plan and descriptor image, formed on a planner thread. `state`
+// is written last (release) and read by the server's thread (acquire) before the results.
+struct PlanJob {
+    uint32_t nq = 0;
+    int dtype = QLX_C64;
+    uint64_t pool_bytes = 0;
+    std::vector<qlx_gate_instr> instrs;
+    std::unique_ptr<SweepPlan> plan;
+    std::vector<char> image;
+    uint32_t off_ph = 0, off_op = 0, off_cf = 0, n_sweeps = 0, rb = 0, log_chunks = 0;
+    uint64_t block_bytes = 0;
+    int err_code = QLX_OK;
+    std::string err;
+    std::atomic<int> state{0};  // 0 pending, 1 planned, 2 failed
+};
+
+inline size_t align_to(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+void plan_job(PlanJob& j) {
+    qlx_exec_opts o{};
+    Geometry geo{};
+    geo.logt = 8;
+    geo.w = std::min<int>((int)j.nq, j.dtype == QLX_C64 ? 13 : 12);
+    geo.rb = geo.w - 8;
+    geo.c = j.dtype == QLX_C64 ? 4 : 3;  // one 128-byte cache line of amplitudes
+    j.plan = build_plan_geo(j.nq, geo, j.instrs.data(), j.instrs.size(), nullptr, nullptr, o, /*thorough=*/false);
+    const SweepPlan& p = *j.plan;
+    j.n_sweeps = (uint32_t)p.dsw.size();
+    j.rb = (uint32_t)geo.rb;
+    j.log_chunks = j.nq - (uint32_t)geo.w;
+    const size_t off_ph = align_to(p.dsw.size() * sizeof(DevSweep), 256);
+    const size_t off_op = align_to(off_ph + p.dph.size() * sizeof(DevPhase), 256);
+    const size_t off_cf = align_to(off_op + p.dop.size() * sizeof(DevOp), 256);
+    const size_t total = align_to(off_cf + p.coef.size() * sizeof(double), 256) + 256;
+    if (total > j.pool_bytes) fail(QLX_ENOMEM, "descriptor image larger than the pool");
+    j.image.assign(total, 0);
+    std::memcpy(j.image.data(), p.dsw.data(), p.dsw.size() * sizeof(DevSweep));
+    std::memcpy(j.image.data() + off_ph, p.dph.data(), p.dph.size() * sizeof(DevPhase));
+    if (!p.dop.empty()) std::memcpy(j.image.data() + off_op, p.dop.data(), p.dop.size() * sizeof(DevOp));
+    if (!p.coef.empty()) std::memcpy(j.image.data() + off_cf, p.coef.data(), p.coef.size() * sizeof(double));
+    j.off_ph = (uint32_t)off_ph;
+    j.off_op = (uint32_t)off_op;
+    j.off_cf = (uint32_t)off_cf;
+    j.block_bytes = total;
+    j.plan.reset();  // the image holds everything the device needs
+}
+
+// Planner threads: take jobs in submission order until stopped.
+class Planner {
+public:
+    void start(unsigned n) {
+        for (unsigned i = 0; i < n; ++i) threads_.emplace_back([this] { loop(); });
+    }
+    void push(std::shared_ptr<PlanJob> j) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            todo_.push_back(std::move(j));
+        }
+        cv_.notify_one();
+    }
+    void stop() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread& t : threads_) t.join();
+        threads_.clear();
+    }
+
+private:
+    void loop() {
+        for (;;) {
+            std::shared_ptr<PlanJob> j;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [this] { return stop_ || !todo_.empty(); });
+                if (stop_) return;
+                j = std::move(todo_.front());
+                todo_.pop_front();
+            }
+            try {
+                plan_job(*j);
+                j->state.store(1, std::memory_order_release);
+            } catch (const Error& e) {
+                j->err_code = e.code;
+                j->err = e.msg;
+                j->state.store(2, std::memory_order_release);
+            } catch (const std::exception& e) {
+                j->err_code = QLX_EHIP;
+                j->err = e.what();
+                j->state.store(2, std::memory_order_release);
+            }
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::shared_ptr<PlanJob>> todo_;
+    std::vector<std::thread> threads_;
+    bool stop_ = false;
+};
+
 }  // namespace qlx
 
 using namespace qlx;
@@ -120,10 +229,8 @@ struct qlx_server {
     struct Req {
         uint64_t id;
         uint32_t nq;
-        std::vector<qlx_gate_instr> instrs;
         Status status = QUEUED;
-        std::unique_ptr<SweepPlan> plan;
-        std::vector<char> image;  // descriptor image (kept until the upload is complete)
+        std::shared_ptr<PlanJob> job;  // the plan and descriptor image (until the upload is complete)
         uint64_t amp_off = 0, state_bytes = 0, block_off = 0, block_bytes = 0;
         uint32_t off_ph = 0, off_op = 0, off_cf = 0;
         uint32_t next_sweep = 0, n_sweeps = 0, rb = 0, log_chunks = 0;
@@ -150,6 +257,7 @@ struct qlx_server {
     std::vector<hipEvent_t> free_events;
     DevBuf scratch;
     hipStream_t aux = nullptr;
+    Planner planner;
 };
 
 namespace {
@@ -172,10 +280,9 @@ hipEvent_t get_event(qlx_server* s) {
     return e;
 }
 
-size_t align_to(size_t x, size_t a) { return (x + a - 1) / a * a; }
-
 void destroy_server(qlx_server* s) {
     if (!s) return;
+    s->planner.stop();
     for (auto& kv : s->groups) {
         Group& g = kv.second;
         if (g.stream) (void)hipStreamSynchronize(g.stream);
@@ -194,10 +301,30 @@ void destroy_server(qlx_server* s) {
     delete s;
 }
 
-// Admission: FIFO while the head request's state and descriptors fit.
+// Admission: FIFO while the head request is planned and its state and descriptors fit.
 void admit(qlx_server* s) {
     while (!s->queue.empty()) {
         Req& r = s->reqs.at(s->queue.front());
+        const int ps = r.job->state.load(std::memory_order_acquire);
+        if (ps == 0) return;
+        if (ps == 2) {
+            // a request that cannot be planned leaves the queue; its error is reported once
+            const int code = r.job->err_code;
+            const std::string msg = "request " + std::to_string(r.id) + ": " + r.job->err;
+            s->queue.pop_front();
+            s->reqs.erase(r.id);
+            fail(code, msg);
+        }
+        PlanJob& j = *r.job;
+        if (!r.block_bytes) {
+            r.block_bytes = j.block_bytes;
+            r.n_sweeps = j.n_sweeps;
+            r.rb = j.rb;
+            r.log_chunks = j.log_chunks;
+            r.off_ph = j.off_ph;
+            r.off_op = j.off_op;
+            r.off_cf = j.off_cf;
+        }
         uint64_t aoff = 0, boff = 0;
         if (!s->arena_alloc.alloc(r.state_bytes, aoff)) return;
         if (!s->pool_alloc.alloc(r.block_bytes, boff)) {
@@ -210,11 +337,10 @@ void admit(qlx_server* s) {
         if (!g.stream) QLX_HIP_CHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
         // upload on the auxiliary stream (never waits for the groups' launches); it is
         // complete before the request's first launch is enqueued
-        QLX_HIP_CHECK(hipMemcpyAsync((char*)s->pool.ptr + boff, r.image.data(), r.image.size(),
+        QLX_HIP_CHECK(hipMemcpyAsync((char*)s->pool.ptr + boff, j.image.data(), j.image.size(),
                                      hipMemcpyHostToDevice, s->aux));
         QLX_HIP_CHECK(hipStreamSynchronize(s->aux));
-        r.image.clear();
-        r.image.shrink_to_fit();
+        r.job.reset();
         r.status = qlx_server::ACTIVE;
         r.admit_s = now_s(s);
         g.active.push_back(r.id);
@@ -328,6 +454,7 @@ int qlx_server_create(int dtype, uint64_t hbm_budget_bytes, qlx_server** out) {
             s->pool_alloc.reset(s->pool_bytes);
             QLX_HIP_CHECK(hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
             s->t0 = std::chrono::steady_clock::now();
+            s->planner.start(std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2)));
         } catch (...) {
             destroy_server(s);
             throw;
@@ -353,36 +480,17 @@ int qlx_server_submit(qlx_server* s, uint32_t nq, const qlx_gate_instr* instrs, 
                 fail(QLX_EUNSUPPORTED, "served circuits take 1-qubit gates, CX, SWAP and GLOBAL_SWAP");
             validate_instr(g, nq, nullptr, 0, 0);
         }
-        r.instrs.assign(instrs, instrs + n);
-        qlx_exec_opts o{};
-        Geometry geo{};
-        geo.logt = 8;
-        geo.w = std::min<int>((int)nq, s->dtype == QLX_C64 ? 13 : 12);
-        geo.rb = geo.w - 8;
-        geo.c = s->dtype == QLX_C64 ? 4 : 3;  // one 128-byte cache line of amplitudes
-        r.plan = build_plan_geo(nq, geo, r.instrs.data(), n, nullptr, nullptr, o, /*thorough=*/false);
-        const SweepPlan& p = *r.plan;
-        r.n_sweeps = (uint32_t)p.dsw.size();
-        r.rb = (uint32_t)geo.rb;
-        r.log_chunks = nq - (uint32_t)geo.w;
-        const size_t off_ph = align_to(p.dsw.size() * sizeof(DevSweep), 256);
-        const size_t off_op = align_to(off_ph + p.dph.size() * sizeof(DevPhase), 256);
-        const size_t off_cf = align_to(off_op + p.dop.size() * sizeof(DevOp), 256);
-        const size_t total = align_to(off_cf + p.coef.size() * sizeof(double), 256) + 256;
-        r.image.assign(total, 0);
-        std::memcpy(r.image.data(), p.dsw.data(), p.dsw.size() * sizeof(DevSweep));
-        std::memcpy(r.image.data() + off_ph, p.dph.data(), p.dph.size() * sizeof(DevPhase));
-        if (!p.dop.empty()) std::memcpy(r.image.data() + off_op, p.dop.data(), p.dop.size() * sizeof(DevOp));
-        if (!p.coef.empty()) std::memcpy(r.image.data() + off_cf, p.coef.data(), p.coef.size() * sizeof(double));
-        r.off_ph = (uint32_t)off_ph;
-        r.off_op = (uint32_t)off_op;
-        r.off_cf = (uint32_t)off_cf;
-        r.block_bytes = total;
-        if (total > s->pool_bytes) fail(QLX_ENOMEM, "descriptor image larger than the pool");
+        auto job = std::make_shared<PlanJob>();
+        job->nq = nq;
+        job->dtype = s->dtype;
+        job->pool_bytes = s->pool_bytes;
+        job->instrs.assign(instrs, instrs + n);
+        r.job = job;
         r.submit_s = now_s(s);
         const uint64_t id = r.id;
         s->reqs.emplace(id, std::move(r));
         s->queue.push_back(id);
+        s->planner.push(std::move(job));
         if (request_id) *request_id = id;
     });
 }
