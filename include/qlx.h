@@ -216,6 +216,8 @@ typedef struct qlx_plan_info {
     uint32_t register_qubits;  /* qubits held in each work-item's registers */
     uint32_t threads_per_group;
     uint32_t coalesce_qubits;  /* low qubits always in the window */
+    uint64_t mfma_phases;      /* complex64 generic sweeps: phases fused into one 32 x 32 matrix and
+                                  applied on the matrix cores */
 } qlx_plan_info;
 int qlx_plan_windowed(uint32_t num_qubits, int dtype, const qlx_gate_instr* instrs, size_t n,
                       const double* pool_c128, size_t pool_elems,

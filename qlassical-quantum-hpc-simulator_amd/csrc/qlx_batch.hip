@@ -286,6 +286,7 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
             c.plan = build_plan_geo(c.nq, geo, c.instrs.data(), c.instrs.size(),
                                     c.pool.empty() ? nullptr : c.pool.data(), c.meta.empty() ? nullptr : c.meta.data(), o,
                                     /*thorough=*/false);
+            if (b->dtype == QLX_C64) densify_phases(*c.plan);
             for (const PlanStep& s : c.plan->steps)
                 if (!s.sweep) fail(QLX_EUNSUPPORTED, "a batched circuit needs a kernel outside the window sweep");
             c.amp_off = amp_total;
@@ -300,6 +301,7 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
             d.rb = (uint32_t)geo.rb;
             d.log_chunks = c.nq - (uint32_t)geo.w;
             max_sweeps = std::max(max_sweeps, d.n_sweeps);
+            if (coef.size() & 1) coef.push_back(0.0);  // matrix-core fragments stay 16-byte aligned
             const uint32_t ph0 = (uint32_t)dph.size(), op0 = (uint32_t)dop.size(), cf0 = (uint32_t)coef.size();
             for (DevSweep s : c.plan->dsw) {
                 s.phase_begin += ph0;
@@ -307,6 +309,7 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
             }
             for (DevPhase p : c.plan->dph) {
                 p.op_begin += op0;
+                if (p.mfma) p.mfma += cf0;
                 dph.push_back(p);
             }
             for (DevOp op : c.plan->dop) {

@@ -169,6 +169,10 @@ struct MfmaSplitJob {
     int variant;        // QLX_DENSE_MFMA_BF16 | QLX_DENSE_MFMA_F32 | QLX_DENSE_MFMA_F16
 };
 size_t mfma_fragment_bytes(int k, int variant);
+// The FP16-form A fragments of k_mfma_split (hi block, then lo block) of a row-major
+// interleaved complex 2^k x 2^k matrix, formed on the host; `out` holds
+// mfma_fragment_bytes(k, QLX_DENSE_MFMA_F16) bytes.
+void mfma_f16_fragments_host(const double* U, int k, unsigned char* out);
 // Builds the A-operand fragments of each job from the FP64 pool (synchronous).
 void mfma_split(const double* d_pool, const MfmaSplitJob* jobs, size_t n, void* d_frags, DevBuf& desc_buf,
                 hipStream_t stream);

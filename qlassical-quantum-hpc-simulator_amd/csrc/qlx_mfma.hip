@@ -315,6 +315,28 @@ size_t frag_bytes(int k, int form) {
 
 }  // namespace
 
+void mfma_f16_fragments_host(const double* U, int k, unsigned char* out) {
+    if (k != 4 && k != 5) fail(QLX_EUNSUPPORTED, "MFMA dense gates take 4 or 5 qubits");
+    const int D = 1 << k, MT = (2 * D) / 32, T = (2 * D) / 16, NB = MT * T * 64;
+    auto embed_h = [&](int row, int col) {
+        const int o = row >> 1, l = col >> 1;
+        const double ur = U[2 * (o * D + l)], ui = U[2 * (o * D + l) + 1];
+        if ((row & 1) == 0) return (col & 1) ? -ui : ur;
+        return (col & 1) ? ur : ui;
+    };
+    _Float16* hi = reinterpret_cast<_Float16*>(out);
+    _Float16* lo = hi + 8 * NB;
+    for (int e = 0; e < NB; ++e) {
+        const int lane = e & 63, t = (e >> 6) % T, m = (e >> 6) / T;
+        for (int s = 0; s < 8; ++s) {
+            const double v = embed_h(32 * m + (lane & 31), kcol_bf16(k, t, lane >> 5, s));
+            const _Float16 h = (_Float16)(float)v;
+            hi[8 * e + s] = h;
+            lo[8 * e + s] = (_Float16)(float)(v - (double)(float)h);
+        }
+    }
+}
+
 size_t mfma_fragment_bytes(int k, int variant) {
     if (k != 4 && k != 5) fail(QLX_EUNSUPPORTED, "MFMA dense gates take 4 or 5 qubits");
     return frag_bytes(k, form_of(variant));

@@ -138,6 +138,7 @@ void plan_job(PlanJob& j) {
     geo.rb = geo.w - 8;
     geo.c = j.dtype == QLX_C64 ? 4 : 3;  // one 128-byte cache line of amplitudes
     j.plan = build_plan_geo(j.nq, geo, j.instrs.data(), j.instrs.size(), nullptr, nullptr, o, /*thorough=*/false);
+    if (j.dtype == QLX_C64) densify_phases(*j.plan);
     const SweepPlan& p = *j.plan;
     j.n_sweeps = (uint32_t)p.dsw.size();
     j.rb = (uint32_t)geo.rb;

@@ -19,6 +19,7 @@
 // CCX/CSWAP and dense gates on 3+ qubits run as their own plain kernels between sweeps.
 
 #include <algorithm>
+#include <complex>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -878,6 +879,108 @@ std::unique_ptr<SweepPlan> build_plan_geo(uint32_t nq, const Geometry& geo, cons
     return best;
 }
 
+namespace {
+
+// One op of a generic phase applied to 32 register amplitudes in FP64 (the same action as the
+// kernel's g_* functions).
+void apply_op_host(std::complex<double> (&v)[32], const DevOp& op, const std::vector<double>& coef) {
+    using C = std::complex<double>;
+    const int kind = (int)(op.key >> 8), b0 = (int)((op.key >> 4) & 15), b1 = (int)(op.key & 15);
+    const int m0 = 1 << b0, m1 = 1 << b1;
+    switch (kind) {
+        case OP_RX: {
+            const C c(op.c0, 0.0), ms(0.0, -op.c1);   // [[c, -i s], [-i s, c]]
+            for (int i = 0; i < 32; ++i)
+                if (!(i & m0)) {
+                    const C a0 = v[i], a1 = v[i | m0];
+                    v[i] = c * a0 + ms * a1;
+                    v[i | m0] = ms * a0 + c * a1;
+                }
+            break;
+        }
+        case OP_H:
+            for (int i = 0; i < 32; ++i)
+                if (!(i & m0)) {
+                    const C a0 = v[i], a1 = v[i | m0];
+                    v[i] = op.c0 * (a0 + a1);
+                    v[i | m0] = op.c0 * (a0 - a1);
+                }
+            break;
+        case OP_U2: {
+            const double* m = coef.data() + op.coef;
+            const C u00(m[0], m[1]), u01(m[2], m[3]), u10(m[4], m[5]), u11(m[6], m[7]);
+            for (int i = 0; i < 32; ++i)
+                if (!(i & m0)) {
+                    const C a0 = v[i], a1 = v[i | m0];
+                    v[i] = u00 * a0 + u01 * a1;
+                    v[i | m0] = u10 * a0 + u11 * a1;
+                }
+            break;
+        }
+        case OP_CX:
+            for (int i = 0; i < 32; ++i)
+                if ((i & m0) && !(i & m1)) std::swap(v[i], v[i | m1]);
+            break;
+        case OP_SWAP:
+            for (int i = 0; i < 32; ++i)
+                if ((i & m0) && !(i & m1)) std::swap(v[i], v[i ^ m0 ^ m1]);
+            break;
+        case OP_U4: {
+            const double* m = coef.data() + op.coef;
+            for (int i = 0; i < 32; ++i) {
+                if ((i & m0) || (i & m1)) continue;
+                const int idx[4] = {i, i | m0, i | m1, i | m0 | m1};
+                C x[4];
+                for (int c = 0; c < 4; ++c) x[c] = v[idx[c]];
+                for (int r = 0; r < 4; ++r) {
+                    C acc = 0.0;
+                    for (int c = 0; c < 4; ++c) acc += C(m[2 * (4 * r + c)], m[2 * (4 * r + c) + 1]) * x[c];
+                    v[idx[r]] = acc;
+                }
+            }
+            break;
+        }
+        default:
+            fail(QLX_EHIP, "phase fusion: op kind without a host form");
+    }
+}
+
+}  // namespace
+
+void densify_phases(SweepPlan& p) {
+    if (p.geo.rb != 5 || p.geo.logt != 8) return;
+    const size_t fdoubles = mfma_fragment_bytes(5, QLX_DENSE_MFMA_F16) / sizeof(double);
+    for (DevPhase& ph : p.dph) {
+        double cost = 0.0;
+        bool fusable = true;
+        for (uint32_t o = 0; o < ph.n_ops && fusable; ++o) {
+            switch (p.dop[ph.op_begin + o].key >> 8) {
+                case OP_RX: case OP_H: case OP_U2: cost += 1.0; break;
+                case OP_CX: case OP_SWAP: cost += 0.25; break;
+                case OP_U4: cost += 4.0; break;
+                default: fusable = false;
+            }
+        }
+        if (!fusable || cost < kDensePhaseCost) continue;
+        std::vector<double> U(2 * 32 * 32);
+        for (int c = 0; c < 32; ++c) {
+            std::complex<double> v[32] = {};
+            v[c] = 1.0;
+            for (uint32_t o = 0; o < ph.n_ops; ++o) apply_op_host(v, p.dop[ph.op_begin + o], p.coef);
+            for (int r = 0; r < 32; ++r) {
+                U[2 * (32 * r + c)] = v[r].real();
+                U[2 * (32 * r + c) + 1] = v[r].imag();
+            }
+        }
+        if (p.coef.size() & 1) p.coef.push_back(0.0);  // 16-byte fragments
+        const size_t at = p.coef.size();
+        if (at + 1 > 0xffffffffull) fail(QLX_EUNSUPPORTED, "phase fusion: coefficient array too large");
+        p.coef.resize(at + fdoubles);
+        mfma_f16_fragments_host(U.data(), 5, reinterpret_cast<unsigned char*>(p.coef.data() + at));
+        ph.mfma = (uint32_t)(at + 1);
+    }
+}
+
 void assign_layouts(SweepPlan& p, uint32_t nq) {
     const Geometry& g = p.geo;
     const int n = (int)nq;
@@ -1041,6 +1144,7 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
     if (!cp.plan || cp.key != key) {
         cp.plan = build_plan(st->nq, st->dtype, instrs, n, pool, meta, opts);
         cp.plan->tag = key;
+        if (st->dtype == QLX_C64 && !(opts.specialize & QLX_SPECIALIZE)) densify_phases(*cp.plan);
         if (relabel) assign_layouts(*cp.plan, st->nq);
         cp.key = key;
         cp.jit.reset();
@@ -1162,6 +1266,7 @@ extern "C" int qlx_plan_windowed(uint32_t num_qubits, int dtype, const qlx_gate_
         const Geometry geo = geometry_for(dtype, num_qubits, o);
         std::unique_ptr<SweepPlan> p;
         if ((int)num_qubits >= geo.w) p = build_plan(num_qubits, dtype, instrs, n, pool, meta, o);
+        if (p && dtype == QLX_C64 && !(o.specialize & QLX_SPECIALIZE)) densify_phases(*p);
         const size_t cnt = p ? p->order.size() : 0;
         if (n_out) *n_out = cnt;
         for (size_t k = 0; p && k < cnt && k < cap; ++k) {
@@ -1179,6 +1284,7 @@ extern "C" int qlx_plan_windowed(uint32_t num_qubits, int dtype, const qlx_gate_
                 info->phases = p->phases;
                 info->plain_steps = p->steps.size() - p->sweeps.size();
                 info->gates_in_sweeps = p->gates_in_sweeps;
+                for (const DevPhase& ph : p->dph) info->mfma_phases += ph.mfma ? 1 : 0;
             }
         }
     });

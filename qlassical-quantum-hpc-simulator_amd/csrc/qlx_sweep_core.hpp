@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "qlx_internal.hpp"
+#include "qlx_mfma_core.hpp"
 
 namespace qlx {
 
@@ -43,7 +44,11 @@ static_assert(sizeof(DevOp) == 24, "DevOp layout");
 
 // Descriptors use 32/64-bit fields throughout so that kernels read them with scalar loads.
 struct DevPhase {
-    uint32_t n_ops, op_begin, remap, pad;
+    uint32_t n_ops, op_begin, remap;
+    // complex64 phases of 5 register bits (256 work-items): 0, or 1 + the index into the
+    // coefficient array of the phase's fused 32 x 32 matrix as FP16 matrix-core fragments (the
+    // phase then runs as one dense product instead of its ops)
+    uint32_t mfma;
     uint32_t roff[kMaxR];        // local-index contribution of register index i
     uint32_t tlocal[kMaxLogT];   // local bit held by work-item bit b
     uint32_t pad2[2];
@@ -257,6 +262,77 @@ __device__ __forceinline__ void store_chunk(const A (&v)[1 << RB], A* __restrict
     for (int i = 0; i < (1 << RB); ++i) amp[base | toff | sw->st_roff[i]] = v[i];
 }
 
+// A phase's ops fused into one 32 x 32 matrix U over the register bits, applied on the matrix
+// cores (complex64, 5 register bits, waves of 64 work-items): work-item j of a wave holds the
+// 32 amplitudes of its group. v_permlane32_swap of register r with register r + 16 gives the
+// two MFMA B operands of the wave's 64 groups (tile A: groups 0-31, lane half h holding their
+// amplitudes 16h + r; tile B: groups 32-63), the FP16 split (state scaled by 2^14) and three
+// products per K-step accumulate U X in FP32, and the same swap on the results returns every
+// group's 32 outputs to its own work-item. The fragments are those of k_mfma_split's FP16 form
+// (matrix hi + lo terms; M-tile m, K-step t, lane: 16-byte entry (m 4 + t) 64 + lane).
+__device__ __forceinline__ void mfma_phase(float2 (&v)[32], const unsigned char* __restrict__ frags) {
+    const int lane = threadIdx.x & 63;
+    const f16x8* fh = reinterpret_cast<const f16x8*>(frags);
+    const f16x8* fl = fh + 2 * 4 * 64;
+    f16x8 xh[2][4], xl[2][4];   // [tile][K-step]
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        float2 a[4], b[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const int r = 4 * t + p;
+            const auto sx = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, v[r].x),
+                                                             __builtin_bit_cast(uint32_t, v[r + 16].x), false, false);
+            const auto sy = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, v[r].y),
+                                                             __builtin_bit_cast(uint32_t, v[r + 16].y), false, false);
+            a[p] = float2{__builtin_bit_cast(float, (uint32_t)sx[0]), __builtin_bit_cast(float, (uint32_t)sy[0])};
+            b[p] = float2{__builtin_bit_cast(float, (uint32_t)sx[1]), __builtin_bit_cast(float, (uint32_t)sy[1])};
+        }
+        split_f16(a, kF16Scale, xh[0][t], xl[0][t]);
+        split_f16(b, kF16Scale, xh[1][t], xl[1][t]);
+    }
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        f32x16 acc[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[k][e] = 0.f;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const f16x8 ah = fh[(m * 4 + t) * 64 + lane], al = fl[(m * 4 + t) * 64 + lane];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, xh[k][t], acc[k], 0, 0, 0);
+                acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, xl[k][t], acc[k], 0, 0, 0);
+                acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, xh[k][t], acc[k], 0, 0, 0);
+            }
+        }
+        // acc[k][4a + 2b + c], lane half h: output 16m + 4a + 2h + b (component c) of tile k's
+        // group; after the swap tile A's register holds bit-1-clear outputs of the work-item's
+        // own group, tile B's the bit-1-set ones
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                float o[2][2];
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    // scaled first: the swap then takes ordinary VGPR values (taken straight from the
+                    // accumulator vectors, the swaps of different elements were merged into one)
+                    const float y0 = acc[0][4 * a + 2 * b + c] * kF16Unscale;
+                    const float y1 = acc[1][4 * a + 2 * b + c] * kF16Unscale;
+                    const auto sw = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, y0),
+                                                                     __builtin_bit_cast(uint32_t, y1), false, false);
+                    o[0][c] = __builtin_bit_cast(float, (uint32_t)sw[0]);
+                    o[1][c] = __builtin_bit_cast(float, (uint32_t)sw[1]);
+                }
+                v[16 * m + 4 * a + b] = float2{o[0][0], o[0][1]};
+                v[16 * m + 4 * a + 2 + b] = float2{o[1][0], o[1][1]};
+            }
+    }
+}
+
 // The register phases of one sweep. No global-memory access: HBM loads issued before
 // the call (the next chunk's) stay in flight across it.
 template <class A, int LOGT, int RB>
@@ -286,6 +362,12 @@ __device__ __forceinline__ void compute_chunk(A (&v)[1 << RB], const DevSweep* _
             __syncthreads();
         }
         cur = P;
+        if constexpr (std::is_same_v<A, float2> && LOGT == 8 && RB == 5) {
+            if (P->mfma) {
+                mfma_phase(v, reinterpret_cast<const unsigned char*>(coef + (P->mfma - 1)));
+                continue;
+            }
+        }
         const uint32_t nops = P->n_ops, ob = P->op_begin;
         // the next descriptor's scalar loads are issued before the current op runs
         DevOp op = nops ? ops[ob] : DevOp{};
@@ -376,6 +458,14 @@ std::vector<DenseWindow> plan_dense_windows(uint32_t nq, const qlx_gate_instr* i
 // the low C qubits never move and the last sweep stores through the identity. Also rewrites
 // the sweeps' device descriptors (work-item and register address bits) to the new layouts.
 void assign_layouts(SweepPlan& p, uint32_t nq);
+
+// Complex64 plans of 5 register bits and 256 work-items (generic kernels): every phase whose ops
+// would cost at least kDensePhaseCost single-qubit ops on the vector units (CX/SWAP a quarter,
+// a 2-qubit matrix four) is fused into one 32 x 32 matrix over its register bits (FP64 product
+// on the host) and applied on the matrix cores (DevPhase::mfma; FP16 fragments appended to the
+// coefficient array). Not for the circuit-specialised sweeps.
+constexpr double kDensePhaseCost = 5.0;
+void densify_phases(SweepPlan& p);
 
 // Plans windowed execution of a MEASURE-free stream with an explicit geometry (W <= nq).
 // thorough: also try the classic greedy window chooser and a beam search, let sweeps that

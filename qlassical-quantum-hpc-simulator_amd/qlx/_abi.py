@@ -108,6 +108,7 @@ class PlanInfo(C.Structure):
         ("register_qubits", C.c_uint32),
         ("threads_per_group", C.c_uint32),
         ("coalesce_qubits", C.c_uint32),
+        ("mfma_phases", C.c_uint64),
     ]
 
 

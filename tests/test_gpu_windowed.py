@@ -115,3 +115,32 @@ def test_c64_windowed_fidelity(gpu, n, d, seed):
     inf = 1.0 - abs(np.vdot(ref, got)) ** 2
     print(f"c64 windowed {n}q: infidelity {inf:.2e}, max abs {np.abs(ref - got).max():.2e}")
     assert inf <= 1e-6
+
+
+def _infidelity(a, b):
+    return 1.0 - abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_c64_phases_on_matrix_cores_vs_oracle(gpu, seed):
+    """complex64 generic sweeps (13-qubit windows, 5 register qubits): each register phase with
+    enough work runs as one fused 32 x 32 product on the matrix cores (FP64 product of its ops on
+    the host, FP16 split operands, FP32 accumulation). Random programs with every gate kind and a
+    layered circuit, from random states, against the FP64 oracle."""
+    from test_oracle import random_program
+    rng = np.random.default_rng(900 + seed)
+    n = int(rng.integers(13, 17))
+    ins, pool, meta = random_program(n, 200, rng)
+    rand = gpu.Program(n, ins, pool if pool is not None else np.zeros(0, complex),
+                       meta if meta is not None else np.zeros(0, gpu.META_DTYPE))
+    for p in (rand, gpu.layered(n, 12, 40 + seed)):
+        _, _, info = gpu.plan_windowed(p, "c64", gpu.exec_opts("windowed"))
+        init = bf.random_state(n, rng)
+        got, _ = run(gpu, p, init, "c64")
+        want, _ = oracle.execute(n, p.instrs, p.pool if p.pool.size else None,
+                                 p.meta if len(p.meta) else None, amps=init)
+        inf = _infidelity(got, want)
+        print(f"c64 {n}q {len(p)} gates: {info['mfma_phases']} of {info['phases']} phases fused, infidelity {inf:.2e}")
+        assert inf <= 1e-10
+        assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+    assert info["mfma_phases"] > 0
