@@ -121,3 +121,23 @@ def test_dense_sweeps_of_the_fused_32q_circuit():
     _, _, info = qlx.plan_windowed(f, "c64", qlx.exec_opts("dense"))
     print(len(f), n_dense, info)
     assert info["sweeps"] * 3 < n_dense
+
+
+def test_c64_generic_plans_fuse_heavy_register_phases():
+    """complex64 generic sweeps (5 register qubits, 256 work-items) run every phase with at least
+    five 1-qubit ops' worth of work as one 32 x 32 product on the matrix cores; the specialised
+    sweeps and complex128 keep their ops."""
+    p = qlx.layered(28, 40, 5)
+    _, _, info = qlx.plan_windowed(p, "c64", qlx.exec_opts("windowed"))
+    assert (info["register_qubits"], info["threads_per_group"]) == (5, 256)
+    assert 0 < info["mfma_phases"] < info["phases"]
+    print(info["mfma_phases"], "of", info["phases"])
+    _, _, spec = qlx.plan_windowed(p, "c64", qlx.exec_opts("windowed", specialize=True))
+    _, _, c128 = qlx.plan_windowed(p, "c128", qlx.exec_opts("windowed"))
+    assert spec["mfma_phases"] == 0 and c128["mfma_phases"] == 0
+    # a phase of CX gates alone stays on the vector units
+    c = qlx.Circuit(14)
+    for q in range(13):
+        c.cx(q, q + 1)
+    _, _, cx = qlx.plan_windowed(c.release(), "c64", qlx.exec_opts("windowed"))
+    assert cx["mfma_phases"] == 0
