@@ -2,7 +2,7 @@
 
 Bars: permutation gates (X, Y, Z, S, CX, SWAP, CCX, CSWAP, GLOBAL_SWAP) and H are
 bit-exact; every floating-point result is within 1e-12 max abs amplitude error of
-the FP64 oracle (the north-star tolerance for complex128).
+the FP64 oracle (the stated tolerance for complex128).
 """
 import os
 import zlib
