@@ -149,14 +149,26 @@ __device__ void mfma_gate(float2* __restrict__ lds, const DenseOpDesc& op, const
                 hl[m][t] = fl[(m * TB + t) * 64 + lane];
             }
     }
-    for (int tile = wave; tile < NTILES; tile += kDThreads / 64) {
-        uint32_t base = (uint32_t)tile * 32 + j;
+    // The wave reads all of its tiles before it writes any (its groups are its own), so a tile's
+    // reads and operand split overlap the previous tile's products.
+    constexpr int TPW = NTILES / (kDThreads / 64);
+    auto tile_base = [&](int tile) {
+        uint32_t b = (uint32_t)tile * 32 + j;
 #pragma unroll
-        for (int q = 0; q < K; ++q) base = ins0_32(base, op.sorted[q]);
-        base = swz(base) << 3;
-        float2 x[HALF];
+        for (int q = 0; q < K; ++q) b = ins0_32(b, op.sorted[q]);
+        return swz(b) << 3;
+    };
+    float2 xa[TPW][HALF];
 #pragma unroll
-        for (int r = 0; r < HALF; ++r) x[r] = at(lds, base ^ hin ^ off(r));
+    for (int i = 0; i < TPW; ++i) {
+        const uint32_t b = tile_base(wave + i * (kDThreads / 64));
+#pragma unroll
+        for (int r = 0; r < HALF; ++r) xa[i][r] = at(lds, b ^ hin ^ off(r));
+    }
+#pragma unroll
+    for (int ti = 0; ti < TPW; ++ti) {
+        const uint32_t base = tile_base(wave + ti * (kDThreads / 64));
+        const float2* x = xa[ti];
         f32x16 acc[MT];
 #pragma unroll
         for (int m = 0; m < MT; ++m)
