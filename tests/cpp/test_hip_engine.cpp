@@ -12,6 +12,7 @@
 #include <backend/hip_engine.hpp>
 #include <frontend/circuit_runner.hpp>
 #include <frontend/frontend.hpp>
+#include <middleware/gpu_middleware.hpp>
 
 #include "mini_test.hpp"
 
@@ -346,6 +347,44 @@ TEST_CASE("uniform superposition 4q and 14q (test_circuit_runner.cpp:355-395)") 
         }
         CHECK(err < 1e-12);
         CHECK_NEAR(total, 1.0, 1e-12);
+    }
+}
+
+TEST_CASE("complex64 dense sweeps on a width-5 fused module (HIPEngineOptions::dense_sweeps)") {
+    // H on every qubit, then 8 layers of RX and brick CX; fused to width 5 with rotations
+    const int n = 15;
+    QuantumCircuit qc(n);
+    for (int q = 0; q < n; ++q) qc.h(q);
+    for (int l = 0; l < 8; ++l) {
+        for (int q = 0; q < n; ++q) qc.rx(q, 0.1 + 0.37 * q + 0.91 * l);
+        for (int q = l % 2; q + 1 < n; q += 2) qc.cx(q, q + 1);
+    }
+    IRModule m = std::move(qc).release();
+    gpu::GpuTranspilerConfig cfg;
+    cfg.max_fusion_width = 5;
+    cfg.fuse_rotations = true;
+    cfg.precision = Precision::C64;
+    gpu::transpile_gpu(m, cfg);
+    HIPEngine ref(n);
+    ref.execute(m);
+    const auto want = ref.state();
+    for (bool relabel : {false, true}) {
+        HIPEngineOptions o;
+        o.precision = Precision::C64;
+        o.dense_sweeps = true;
+        o.relabel = relabel;
+        HIPEngine e(n, std::nullopt, 0, o);
+        e.execute(m);
+        const auto got = e.state();
+        std::complex<double> ip = 0.0;
+        double na = 0.0, nb = 0.0;
+        for (std::size_t i = 0; i < got.size(); ++i) {
+            ip += std::conj(want[i]) * got[i];
+            na += std::norm(want[i]);
+            nb += std::norm(got[i]);
+        }
+        CHECK(1.0 - std::norm(ip) / (na * nb) <= 1e-9);
+        CHECK_NEAR(nb, 1.0, 1e-5);
     }
 }
 
