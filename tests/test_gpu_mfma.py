@@ -190,7 +190,7 @@ def test_dense_sweeps_every_gate_kind_vs_oracle(gpu):
     from test_oracle import random_program
     for seed in range(4):
         rng = np.random.default_rng(300 + seed)
-        n = int(rng.integers(11, 17))
+        n = 13 if seed == 0 else int(rng.integers(11, 17))   # 13 qubits: one chunk per sweep
         ins, pool, meta = random_program(n, 150, rng)
         p = gpu.Program(n, ins, pool if pool is not None else np.zeros(0, complex),
                         meta if meta is not None else np.zeros(0, gpu.META_DTYPE))
@@ -203,10 +203,10 @@ def test_dense_sweeps_every_gate_kind_vs_oracle(gpu):
             psi = rand_state(rng, n)
             want, _ = oracle.execute(n, prog.instrs, prog.pool if prog.pool.size else None,
                                      prog.meta if len(prog.meta) else None, amps=psi)
-            for dense in ("bf16", "f16"):
-                got, t = run_c64(gpu, prog, psi, dense, "dense")
+            for dense, relabel in (("bf16", False), ("f16", False), ("f16", True)):
+                got, t = run_c64(gpu, prog, psi, dense, "dense", relabel=relabel)
                 assert ("k_dsweep" in t) == (n >= 13), (n, t)
-                assert infidelity(got, want) <= CIRCUIT_BOUND[dense], (seed, n, dense)
+                assert infidelity(got, want) <= CIRCUIT_BOUND[dense], (seed, n, dense, relabel)
 
 
 def test_dense_sweeps_repeat_and_switch_programs(gpu):
