@@ -274,3 +274,29 @@ def test_dense_sweeps_relabel_mixed_with_in_place(gpu):
         assert infidelity(got, want) <= CIRCUIT_BOUND["f16"], relabel
     st.close()
     assert sweeps == {0, 1}          # (depths 3, 7, 9: 2, 3 and 4 sweeps)
+
+
+@pytest.mark.parametrize("gswap", ["permute", "noop"])
+def test_dense_sweeps_global_swap_modes(gpu, gswap):
+    """GLOBAL_SWAP inside dense sweeps: a qubit permutation (default) or the reference's no-op,
+    against the complex128 plain path in the same mode."""
+    n = 15
+    rng = np.random.default_rng(61)
+    base = gpu.transpile_gpu(gpu.layered(n, 6, 11), max_width=5, rotations=True)
+    gs = np.zeros(3, gpu.GATE_DTYPE)
+    gs["type"] = 0xE1
+    gs["flags_arity"] = 2
+    gs["qubits"] = [[14, 0, 255, 255], [13, 2, 255, 255], [7, 12, 255, 255]]
+    half = len(base) // 2
+    p = gpu.Program(n, np.concatenate([base.instrs[:half], gs, base.instrs[half:]]), base.pool, base.meta)
+    psi = rand_state(rng, n)
+    st = gpu.State(amps=psi, dtype="c128")
+    st.execute(p, gpu.exec_opts("plain", gswap=gswap))
+    want = st.to_host()
+    st.close()
+    for relabel in (False, True):
+        st = gpu.State(amps=psi, dtype="c64")
+        st.execute(p, gpu.exec_opts("dense", dense="f16", gswap=gswap, relabel=relabel))
+        got = st.to_host()
+        st.close()
+        assert infidelity(got, want) <= CIRCUIT_BOUND["f16"], (gswap, relabel)
