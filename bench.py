@@ -9,8 +9,14 @@ evolving it, so every step does identical work). Prints one JSON line.
              f * 2 * 16 B * 2^30 (f = 1 for full sweeps, 1/2 for CX/SWAP) / step time
   roofline   the dominant kernel's algorithmic bytes per launch / its mean HIP-event
              duration on the state's stream, against 8 TB/s
-  cpu_baseline  the oracle (C++ restatement of CPUOpenMPEngine, Release flags) on a
-             bounded sample of the same 30q circuit, on this host's cores
+  cpu_baseline  the oracle (C++ restatement of CPUOpenMPEngine, Release flags) running the
+             whole 30q circuit on this host's cores, from the same seeded random state as a
+             GPU run of the benchmarked mode
+  parity     that GPU result against the oracle's (max |amplitude difference| <= 1e-12;
+             the process exits with status 3 if it is not)
+  serve      the serving workload (Poisson requests of 12-28 qubits, complex64): circuits/s,
+             p50/p99 latency, infidelity of spot checks against complex128
+  config.dropin_default  the same circuit through the drop-in default options (QLX_EXEC_AUTO)
 """
 import argparse
 import json
@@ -41,9 +47,9 @@ def algorithmic_bytes(instrs, nq, amp_bytes=16):
     return sum(pass_fraction(t) for t in instrs["type"]) * bp
 
 
-def cpu_baseline(prog, nq, budget_s):
+def cpu_baseline_layer(prog, nq, budget_s):
     """Times the oracle (Release build) on the first layer (RX on every qubit + its CX
-    brick) of the same circuit: a bounded sample with the circuit's gate mix."""
+    brick) of the same circuit: a bounded sample with the circuit's gate mix (--cpu-sample layer)."""
     import numpy as np
 
     import oracle
@@ -72,6 +78,68 @@ def cpu_baseline(prog, nq, budget_s):
                   f"{int((sample['type'] == 0x20).sum())} CX) of the {nq}q circuit's first layer, "
                   f"execute() only, {t:.2f} s; oracle/_build/liboracle_fast.so (-O3 -ffast-math -fopenmp)",
         "circuit_s_est": round(full / rate, 1),
+    }
+
+
+def cpu_baseline_and_parity(qlx, prog, nq, opts, input_seed):
+    """The whole circuit on the CPU and on the GPU from the same seeded random state.
+
+    CPU: the oracle's Release build (the reference's -O3 -ffast-math flags, OpenMP on every host
+    core) runs execute() of all the circuit's gates in place on one host array (16 GiB at 30
+    qubits); the timer brackets execute() only, as benchmarks/bench_hw_awareness.cpp:102-107 does.
+    GPU: the benchmarked mode runs on the same input meanwhile. The two results are then
+    compared on the device (max |amplitude difference|): the full-size check of the timed form."""
+    import oracle
+    host = oracle.random_state(nq, input_seed)
+    st = qlx.State(dtype="c128", amps=host)
+    st.execute(prog, opts)
+    log(f"bench: cpu baseline: whole {len(prog)}-gate circuit on the oracle ({oracle.num_threads('fast')} threads) ...")
+    t0 = time.perf_counter()
+    oracle.execute_inplace(nq, prog.instrs, host, lib="fast")
+    t_cpu = time.perf_counter() - t0
+    st.sync()
+    want = qlx.State(dtype="c128", amps=host)
+    del host
+    err = st.max_abs_diff(want)
+    want.close()
+    st.close()
+    b = algorithmic_bytes(prog.instrs, nq)
+    log(f"bench: cpu baseline {t_cpu:.1f} s ({b / t_cpu / 1e9:.1f} GB/s); GPU vs oracle max |d| = {err:.3e}")
+    cpu = {
+        "value": round(b / t_cpu / 1e9, 3), "unit": "GB/s", "cores": int(oracle.num_threads("fast")), "kind": "port",
+        "sample": f"the whole {nq}-qubit circuit ({len(prog)} gates, {algorithmic_bytes(prog.instrs, nq) / (2 * 16 * (1 << nq)):.0f} "
+                  f"full-state passes) from a seeded random state, execute() only: {t_cpu:.2f} s; "
+                  f"oracle/_build/liboracle_fast.so (-O3 -march=x86-64-v3 -ffast-math -fopenmp)",
+        "circuit_s": round(t_cpu, 3),
+    }
+    parity = {"vs": "oracle (Release build), whole circuit from the same seeded random state",
+              "input_seed": input_seed, "max_abs_err": err, "tolerance": 1e-12, "ok": bool(err <= 1e-12)}
+    return cpu, parity
+
+
+def serve_summary(qlx, requests, budget_gb):
+    """The serving workload (N ~ U[12,28], D ~ U[5,40], seeds 500000+j, complex64): first every
+    request at t = 0 (saturated capacity), then a Poisson stream at 80% of that capacity;
+    latency from arrival to completion; every ~10th result compared with complex128."""
+    import numpy as np
+    work = qlx.serve_workload(requests)
+    progs = [qlx.layered(n, d, s) for n, d, s in work]
+    budget = int(budget_gb * (1 << 30))
+    cap = qlx.run_server(progs, np.zeros(len(progs)), budget, "c64")
+    rate = 0.8 * cap["circuits_per_s"]
+    log(f"serve: saturated capacity {cap['circuits_per_s']:.1f} circuits/s; running at {rate:.1f}/s")
+    arr = qlx.poisson_arrivals(len(progs), rate, seed=5006)
+    res = qlx.run_server(progs, arr, budget, "c64", check_every=max(1, len(progs) // 40))
+    log(f"serve: {res['circuits_per_s']:.1f} circuits/s, p50 {res['p50_ms']:.1f} ms, p99 {res['p99_ms']:.1f} ms, "
+        f"max infidelity {res['max_infidelity']:.2e}")
+    return {
+        "workload": f"{len(progs)} requests, N~U[12,28], D~U[5,40] (seeds 500000+j), complex64, HBM budget "
+                    f"{budget_gb:g} GiB; Poisson arrivals at 80% of the saturated rate",
+        "offered_per_s": round(rate, 2), "circuits_per_s": round(res["circuits_per_s"], 2),
+        "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3), "mean_ms": round(res["mean_ms"], 3),
+        "saturated_circuits_per_s": round(cap["circuits_per_s"], 2), "saturated_p99_ms": round(cap["p99_ms"], 3),
+        "spot_checks": len(res["checks"]), "max_infidelity_vs_c128": res["max_infidelity"],
+        "max_abs_vs_c128": res["max_abs"], "infidelity_ok": bool(res["max_infidelity"] <= 1e-6),
     }
 
 
@@ -397,28 +465,14 @@ def run_serve(args):
     """Poisson stream of layered circuits (N ~ U[12,28], D ~ U[5,40], mt19937_64(5005) shapes,
     seeds 500000+j), complex64, HBM-budgeted continuous batching. The rate is 80% of the
     capacity measured with every request arriving at t = 0."""
-    import numpy as np
-
     import qlx
-    work = qlx.serve_workload(args.serve_requests)
-    progs = [qlx.layered(n, d, s) for n, d, s in work]
-    budget = int(args.serve_budget_gb * (1 << 30))
-    cap = qlx.run_server(progs, np.zeros(len(progs)), budget, "c64")
-    rate = 0.8 * cap["circuits_per_s"]
-    log(f"serve: saturated capacity {cap['circuits_per_s']:.1f} circuits/s; running at {rate:.1f}/s")
-    arr = qlx.poisson_arrivals(len(progs), rate, seed=5006)
-    res = qlx.run_server(progs, arr, budget, "c64", check_every=max(1, len(progs) // 40))
+    sv = serve_summary(qlx, args.serve_requests, args.serve_budget_gb)
     out = {
         "metric": "served circuits/s + p99 latency (Poisson layered circuits, complex64)",
-        "value": round(res["circuits_per_s"], 2), "unit": "circuits/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": round(res["wall_s"] * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+        "value": sv["circuits_per_s"], "unit": "circuits/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{len(progs)} requests, N~U[12,28], D~U[5,40], Poisson rate {rate:.1f}/s "
-                               f"(80% of saturated {cap['circuits_per_s']:.1f}/s), HBM budget {args.serve_budget_gb} GiB",
-                   "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
-                   "mean_ms": round(res["mean_ms"], 3), "saturated_circuits_per_s": round(cap["circuits_per_s"], 2),
-                   "saturated_p99_ms": round(cap["p99_ms"], 3), "spot_checks": len(res["checks"]),
-                   "max_infidelity_vs_c128": res["max_infidelity"], "max_abs_vs_c128": res["max_abs"]},
+        "config": dict(sv),
     }
     print(json.dumps(out), flush=True)
 
@@ -441,8 +495,12 @@ def main():
                          "re-laying out the qubits so that every window sits on the low address bits)")
     ap.add_argument("--sweep-wg", type=int, default=0,
                     help="work-groups per window-sweep launch (0 = every resident slot)")
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="--cpu-sample layer: seconds of CPU work")
+    ap.add_argument("--cpu-sample", choices=["circuit", "layer"], default="circuit",
+                    help="circuit30: time the oracle on the whole circuit (and check the GPU result against it), "
+                         "or on the first layer only")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="circuit30: skip the serving leg")
     ap.add_argument("--config", choices=["circuit30", "precision", "batch", "serve", "clustered"],
                     default="circuit30")
     ap.add_argument("--dense-variants", default="dense:f16,dense:bf16,windowed:f16,dense:f32,windowed:valu,unfused:windowed",
@@ -477,8 +535,6 @@ def main():
         run_clustered(args)
         return
 
-    import numpy as np
-
     import qlx
 
     nq, depth = args.qubits or 30, args.depth or 20
@@ -486,6 +542,44 @@ def main():
     prog = qlx.layered(nq, depth, args.seed)
     modes = ["windowed", "plain"] if args.mode == "auto" else [args.mode]
     st = qlx.State(nq, "c128")
+
+    # the drop-in default: what HIPEngine / CircuitRunner run with HIPEngineOptions{} and the default
+    # GpuTranspilerConfig (its fusion pass leaves this circuit unchanged): QLX_EXEC_AUTO. Measured
+    # first, while the compiler's cache is cold: the first execution runs the generic sweep kernel
+    # and starts the background compilation; once it is done every execution runs the compiled
+    # sweeps.
+    dropin = None
+    if args.mode == "auto" and not (args.window or args.regs or args.generic):
+        dp = qlx.transpile_gpu(prog)          # GpuTranspilerConfig{} (fusion width 4, H/CX blocks)
+        oa = qlx.exec_opts("auto")
+        t0 = time.perf_counter()
+        st.execute(dp, oa)
+        st.sync()
+        first = time.perf_counter() - t0
+        s_first = st.last_stats()
+        t0 = time.perf_counter()
+        qlx.specialize_wait()
+        wait_s = time.perf_counter() - t0
+        st.execute(dp, oa)
+        st.sync()
+        k = max(1, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(k):
+            st.execute(dp, oa)
+        st.sync()
+        dms = (time.perf_counter() - t0) / k * 1e3
+        s_after = st.last_stats()
+        dropin = {"ms_per_step": round(dms, 3), "steps": k, "first_execution_ms": round(first * 1e3, 3),
+                  "first_execution_kernels": "compiled" if s_first["specialised"] else "generic",
+                  "compile_wait_after_first_s": round(wait_s, 3),
+                  "kernels": "compiled" if s_after["specialised"] else "generic",
+                  "relabelled": bool(s_after["relabelled"]), "instructions_after_default_transpile": len(dp),
+                  "note": "HIPEngineOptions{} = QLX_EXEC_AUTO: generic sweeps until the background "
+                          "compilation of the program's specialised sweeps is done, then those"}
+        log(f"bench: drop-in default: first execution {first * 1e3:.1f} ms ({dropin['first_execution_kernels']}), "
+            f"compile done {wait_s:.2f} s later, then {dms:.2f} ms/step ({dropin['kernels']})")
+        st.reset()
+
     mode, opts = None, None
     for m in modes:
         spec = m == "windowed" and not args.generic
@@ -545,13 +639,18 @@ def main():
     fused = qlx.transpile_gpu(prog, max_width=5, rotations=True)
     st.execute(fused, opts)
     st.sync()
+    st.timing(True)
+    st.timing_reset()
     tf = time.perf_counter()
     for _ in range(2):
         st.execute(fused, opts)
     st.sync()
     fused_ms = (time.perf_counter() - tf) / 2 * 1e3
+    fused_timing = st.timing_read()
+    st.timing(False)
     fused_stats = st.last_stats()
-    log(f"bench: fused width 5 + windowed {fused_ms:.2f} ms/step ({len(fused)} instructions, {fused_stats})")
+    log(f"bench: fused width 5 + windowed {fused_ms:.2f} ms/step ({len(fused)} instructions, {fused_stats}, "
+        f"kernels {fused_timing})")
     name, (launches, tot_ms, tot_bytes) = dom
     avg_ms = tot_ms / launches
     achieved = (tot_bytes / launches) / (avg_ms / 1e3)
@@ -581,10 +680,13 @@ def main():
                                 "quarter": stats["sweeps_quarter"]},
             "circuit_ms": round(ms, 3),
             "sweep_kernels": ("circuit-specialised (compiled at first execution)" if opts.specialize else "generic") +
-                             ("; out of place, qubits re-laid out between two buffers" if opts.specialize & 2 else ""),
+                             ("; out of place, qubits re-laid out between two buffers" if stats["relabelled"] else ""),
             "first_execution_s": round(first_s, 3),
+            "dropin_default": dropin,
+            "dropin_ms": dropin["ms_per_step"] if dropin else None,
             "generic_kernel_ms_per_step": round(generic, 3) if generic else None,
             "fused_width5_ms_per_step": round(fused_ms, 3),
+            "fused_width5_kernels_ms": {k: round(v[1] / max(1, v[0]), 4) for k, v in fused_timing.items()},
             "fused_width5_note": f"pass_gate_fusion (width 5, rotations fused) gives {len(fused)} instructions; "
                                  f"its 3-5-qubit blocks are dense FP64 passes, so the window sweeps over the "
                                  f"unfused gates are the faster form of fusion here",
@@ -612,10 +714,16 @@ def main():
         dst.close()
     st.close()
     if not args.no_cpu:
-        log("bench: cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(prog, nq, args.cpu_budget)
+        if args.cpu_sample == "layer":
+            log("bench: cpu baseline (first layer) ...")
+            out["cpu_baseline"] = cpu_baseline_layer(prog, nq, args.cpu_budget)
+        else:
+            out["cpu_baseline"], out["parity"] = cpu_baseline_and_parity(qlx, prog, nq, opts, 300000 + args.seed)
+    if not args.no_serve:
+        out["serve"] = serve_summary(qlx, args.serve_requests, args.serve_budget_gb)
     print(json.dumps(out), flush=True)
-
+    if "parity" in out and not out["parity"]["ok"]:
+        sys.exit(3)
 
 if __name__ == "__main__":
     main()

@@ -78,6 +78,13 @@ enum {
 #define QLX_EXEC_DENSE_SWEEPS 2  /* complex64: 13-qubit windows held in LDS; every gate of a window
                                     applied there as a dense matrix, 4/5-qubit UNITARY / FUSED_BLOCK
                                     gates on the matrix cores (dense_kernel) — for fused programs */
+#define QLX_EXEC_AUTO     3  /* the default of the C++ HIPEngine: window sweeps; for states of at
+                                least QLX_AUTO_SPECIALIZE_QUBITS qubits (window and register qubits
+                                left at their defaults) also QLX_SPECIALIZE | QLX_SPECIALIZE_RELABEL |
+                                QLX_SPECIALIZE_ASYNC: a program's first executions run the generic
+                                sweep kernel while its specialised sweeps compile on a host thread,
+                                later executions run the compiled sweeps */
+#define QLX_AUTO_SPECIALIZE_QUBITS 20
 
 /* GLOBAL_SWAP semantics. The reference engine treats GLOBAL_SWAP as a no-op
  * (include/backend/cpu_openmp_engine.hpp:217-219); the GPU backend applies it
@@ -86,7 +93,7 @@ enum {
 #define QLX_GSWAP_NOOP    1
 
 typedef struct qlx_exec_opts {
-    uint32_t mode;            /* QLX_EXEC_PLAIN | QLX_EXEC_WINDOWED | QLX_EXEC_DENSE_SWEEPS */
+    uint32_t mode;            /* QLX_EXEC_PLAIN | QLX_EXEC_WINDOWED | QLX_EXEC_DENSE_SWEEPS | QLX_EXEC_AUTO */
     uint32_t window_qubits;   /* 0 = default for the precision (c128: 12, c64: 13) */
     uint32_t coalesce_qubits; /* low qubits always kept in the window; 0 = default (c128: 3, c64: 4:
                                  one 128-byte cache line) */
@@ -106,16 +113,23 @@ typedef struct qlx_exec_opts {
 
 /* qlx_exec_opts::specialize flags.
  * QLX_SPECIALIZE: compile the program's sweeps into straight-line kernels (hipRTC, once per
- *   program and state; later executions of the same program reuse them).
+ *   program and state; later executions of the same program reuse them). The ROCm code-object
+ *   compiler keeps its own on-disk cache, so a program compiled once is loaded in well under a
+ *   second by later processes.
  * QLX_SPECIALIZE_RELABEL (windowed: with QLX_SPECIALIZE; dense sweeps: on its own): every
  *   sweep reads the state from one buffer
  *   and writes it to a second one with its qubits re-laid out so that the next sweep's window
  *   sits on the low address bits (short strides, fewer DRAM pages per chunk); the last sweep
  *   writes the logical order back. Needs a second device buffer of the state's size (allocated
- *   on first use, kept with the state); qlx_state_device_ptr may return the other buffer after
- *   such an execution. */
+ *   on first use, kept with the state); when that allocation fails the sweeps run in place.
+ *   qlx_state_device_ptr may return the other buffer after such an execution.
+ * QLX_SPECIALIZE_ASYNC (windowed, with QLX_SPECIALIZE): the program's sweeps are planned and
+ *   compiled on a background host thread, shared by every state of the process; until they are
+ *   ready each execution runs the generic sweep kernel in place (the same gates, to rounding).
+ *   qlx_specialize_wait waits for the background work. */
 #define QLX_SPECIALIZE 1u
 #define QLX_SPECIALIZE_RELABEL 2u
+#define QLX_SPECIALIZE_ASYNC 4u
 
 /* Dense-gate arithmetic (qlx_exec_opts::dense_kernel). The MFMA forms apply to 4- and
  * 5-qubit gates on complex64 states; every other dense gate runs on the vector units
@@ -136,6 +150,8 @@ typedef struct qlx_exec_stats {
     uint64_t sweeps_quarter;  /* kernels touching a quarter (CCX/CSWAP, f = 1/4) */
     uint64_t gates_applied;   /* source instructions executed (excl. BARRIER) */
     double   algorithmic_bytes; /* sum of f * 2 * sizeof(amp) * 2^N */
+    uint32_t specialised;     /* 1: the window sweeps ran as circuit-specialised kernels */
+    uint32_t relabelled;      /* 1: out of place through the second buffer */
 } qlx_exec_stats;
 
 /* Device description backing the GPU HardwareTopology
@@ -195,8 +211,10 @@ int qlx_state_copy(qlx_state* dst, qlx_state* src);
  * Enqueues instructions [0, n_instr) up to (not including) the first MEASURE;
  * *stop_at receives its index (or n_instr). The host then draws the outcome with
  * its own mt19937_64 (reference :632-633) via qlx_measure_prob1/qlx_collapse and
- * resumes after it. Unknown opcodes -> QLX_EUNSUPPORTED before anything runs
- * (reference :221-222). opts may be NULL (plain mode, real GLOBAL_SWAP). */
+ * resumes after it. An unknown opcode ends the run like the reference's runtime_error
+ * (:221-222): the instructions before it are executed (as the reference's loop has applied
+ * them when it throws), then QLX_EUNSUPPORTED is returned. opts may be NULL (plain mode,
+ * real GLOBAL_SWAP). */
 int qlx_execute(qlx_state* st,
                 const qlx_gate_instr* instrs, size_t n_instr,
                 const double* pool_c128, size_t pool_elems,
@@ -230,6 +248,12 @@ int qlx_plan_windowed(uint32_t num_qubits, int dtype, const qlx_gate_instr* inst
 int qlx_plan_sweep_source(uint32_t num_qubits, int dtype, const qlx_gate_instr* instrs, size_t n,
                           const double* pool_c128, size_t pool_elems, const qlx_matrix_entry* meta, size_t n_meta,
                           const qlx_exec_opts* opts, uint32_t sweep_index, char* out, size_t cap, size_t* len);
+
+/* Waits up to timeout_s seconds (< 0: no limit) for the background planning and compilation
+ * that QLX_SPECIALIZE_ASYNC executions started; *pending (may be NULL) receives the number of
+ * programs still queued or compiling. Returns the first failure of a finished compilation
+ * (QLX_EHIP, with its log in qlx_last_error) once, else QLX_OK. */
+int qlx_specialize_wait(double timeout_s, uint32_t* pending);
 
 /* Sweep accounting of the last qlx_execute on this state. */
 int qlx_last_exec_stats(const qlx_state* st, qlx_exec_stats* out);

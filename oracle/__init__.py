@@ -56,6 +56,7 @@ def _load(path: str) -> C.CDLL:
         L.orc_fuse_matrices.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
                                         C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
         L.orc_num_threads.restype = C.c_int
+        L.orc_random_state.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64]
     _libs[path] = L
     return L
 
@@ -111,6 +112,39 @@ def execute(num_qubits: int, instrs: np.ndarray, pool=None, meta=None, amps=None
         msg = L.orc_last_error().decode()
         raise {-1: ValueError, -2: IndexError}.get(rc, RuntimeError)(msg)
     return a, [int(x) for x in meas[:nm.value]]
+
+
+def random_state(num_qubits: int, seed: int, out: np.ndarray = None) -> np.ndarray:
+    """Seeded random normalised complex128 state (orc_random_state: SplitMix64 of (seed, index),
+    OpenMP fill, thread-count independent). Fills `out` when given (a C-contiguous complex128
+    array of 2^num_qubits elements)."""
+    a = np.empty(1 << num_qubits, np.complex128) if out is None else out
+    if a.dtype != np.complex128 or a.size != (1 << num_qubits) or not a.flags.c_contiguous:
+        raise ValueError("random_state: out must be C-contiguous complex128 of 2^num_qubits elements")
+    strict().orc_random_state(a.ctypes.data, num_qubits, seed)
+    return a
+
+
+def execute_inplace(num_qubits: int, instrs: np.ndarray, amps: np.ndarray, pool=None, meta=None,
+                    gswap: str = "noop", seed: int = 5489, lib: str = "strict") -> list:
+    """As execute(), but evolves `amps` (C-contiguous complex128, 2^num_qubits) in place, so a
+    30-qubit run needs one 16 GiB host buffer. Returns the measurements."""
+    if amps.dtype != np.complex128 or amps.size != (1 << num_qubits) or not amps.flags.c_contiguous:
+        raise ValueError("execute_inplace: amps must be C-contiguous complex128 of 2^num_qubits elements")
+    L = strict() if lib == "strict" else fast()
+    ins = np.ascontiguousarray(instrs)
+    pool = np.zeros(0, np.complex128) if pool is None else np.ascontiguousarray(pool, np.complex128)
+    meta_b = np.zeros(0, np.uint8) if meta is None else np.ascontiguousarray(meta)
+    meas = np.zeros(max(1, len(ins)), np.uint8)
+    nm = C.c_size_t(0)
+    rc = L.orc_execute(amps.ctypes.data, num_qubits, ins.ctypes.data if len(ins) else None, len(ins),
+                       pool.ctypes.data if pool.size else None, pool.size,
+                       meta_b.ctypes.data if len(meta_b) else None, len(meta_b) if meta is not None else 0,
+                       1 if gswap == "permute" else 0, seed, meas.ctypes.data, meas.size, C.byref(nm))
+    if rc != 0:
+        msg = L.orc_last_error().decode()
+        raise {-1: ValueError, -2: IndexError}.get(rc, RuntimeError)(msg)
+    return [int(x) for x in meas[:nm.value]]
 
 
 def time_execute(num_qubits: int, instrs: np.ndarray, reps: int = 1, lib: str = "fast", pool=None,

@@ -457,6 +457,41 @@ int orc_execute(double* amps, uint32_t nq, const Instr* prog, size_t n, const do
     }
 }
 
+// Seeded random input state for full-size parity runs (test input; the reference has no such
+// routine). Amplitude i is (u(2i), u(2i+1)) with u(k) a SplitMix64 hash of (seed, k) mapped to
+// [-1, 1), scaled by 1/sqrt(sum |a_i|^2). The norm is summed over a fixed partition of 4096
+// blocks, so the bytes do not depend on the thread count.
+int orc_random_state(double* amps, uint32_t nq, uint64_t seed) {
+    const int64_t dim = int64_t{1} << nq;
+    auto mix = [](uint64_t z) {
+        z += 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    const uint64_t s = mix(seed ^ 0x5DEECE66Dull);
+    const int64_t nblk = dim < 4096 ? dim : 4096, per = dim / nblk;
+    std::vector<double> part(nblk, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t b = 0; b < nblk; ++b) {
+        double acc = 0.0;
+        for (int64_t i = b * per; i < (b + 1) * per; ++i) {
+            const double re = (double)(mix(s + 2 * (uint64_t)i) >> 11) * 0x1.0p-52 - 1.0;
+            const double im = (double)(mix(s + 2 * (uint64_t)i + 1) >> 11) * 0x1.0p-52 - 1.0;
+            amps[2 * i] = re;
+            amps[2 * i + 1] = im;
+            acc += re * re + im * im;
+        }
+        part[b] = acc;
+    }
+    double ns = 0.0;
+    for (double p : part) ns += p;
+    const double k = 1.0 / std::sqrt(ns);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < 2 * dim; ++i) amps[i] *= k;
+    return 0;
+}
+
 // Wall-clock seconds of orc_execute over `reps` runs from |0...0> (init outside the timer),
 // as bench_hw_awareness.cpp:102-107 times execute() only. Returns the median.
 double orc_time_execute(double* amps, uint32_t nq, const Instr* prog, size_t n, const double* pool,

@@ -552,7 +552,14 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
     if (dk > QLX_DENSE_MFMA_F16) fail(QLX_EINVAL, "unknown dense_kernel");
     const int variant = dk == QLX_DENSE_AUTO ? QLX_DENSE_MFMA_F16 : (int)dk;
     const bool f32 = variant == QLX_DENSE_MFMA_F32;
-    const bool relabel = (opts.specialize & QLX_SPECIALIZE_RELABEL) != 0;
+    bool relabel = (opts.specialize & QLX_SPECIALIZE_RELABEL) != 0;
+    // relabelled sweeps need a second buffer of the state's size; without it they run in place
+    if (relabel && !st->alt && hipMalloc(&st->alt, st->bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        st->alt = nullptr;
+        relabel = false;
+    }
+    const uint32_t eff_specialize = relabel ? QLX_SPECIALIZE_RELABEL : 0u;
 
     uint64_t key = 1469598103934665603ull;
     auto mix = [&](const void* p, size_t b) {
@@ -566,6 +573,7 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
     if (pool) mix(pool, pool_elems * 16);
     if (meta) mix(meta, n_meta * sizeof(qlx_matrix_entry));
     mix(&opts, sizeof(opts));
+    mix(&eff_specialize, sizeof(eff_specialize));
     mix(&st->nq, sizeof(st->nq));
     if (!st->dense_cache) st->dense_cache = std::shared_ptr<void>(new DensePlan(), [](void* p) {
         auto* d = static_cast<DensePlan*>(p);
@@ -722,13 +730,7 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
     if (chunks > 0xffffffffull) fail(QLX_EUNSUPPORTED, "state too large for one dense-sweep launch");
     const double pass = pass_bytes(st);
     const unsigned char* frags = (const unsigned char*)P.frags.ptr;
-    if (relabel && !st->alt) {
-        if (hipMalloc(&st->alt, st->bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            st->alt = nullptr;
-            fail(QLX_ENOMEM, "relabelled dense sweeps need a second buffer of " + std::to_string(st->bytes) + " bytes");
-        }
-    }
+    if (relabel && !st->alt) fail(QLX_EHIP, "relabelled dense sweeps without a second buffer");
     for (size_t s = 0; s < P.sweeps.size(); ++s) {
         TimedLaunch tl(st, "k_dsweep", pass);
         const DenseSweepDesc* sw = (const DenseSweepDesc*)dev + s;
@@ -748,6 +750,7 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
         stats.algorithmic_bytes += pass;
     }
     stats.gates_applied += P.gates;
+    stats.relabelled = relabel && !P.sweeps.empty() ? 1u : 0u;
 }
 
 }  // namespace qlx

@@ -531,8 +531,17 @@ int qlx_execute(qlx_state* st, const qlx_gate_instr* instrs, size_t n_instr, con
         if (n_instr && !instrs) fail(QLX_EINVAL, "null instruction stream");
         qlx_exec_opts o{};
         if (opts) o = *opts;
+        if (o.mode == QLX_EXEC_AUTO) {
+            // window sweeps; large states also get specialised, relabelled sweeps compiled in the
+            // background (the default geometry only: the generic kernel runs until they are ready)
+            o.mode = QLX_EXEC_WINDOWED;
+            if (st->nq >= QLX_AUTO_SPECIALIZE_QUBITS && !o.window_qubits && !o.register_qubits && !o.coalesce_qubits)
+                o.specialize |= QLX_SPECIALIZE | QLX_SPECIALIZE_RELABEL | QLX_SPECIALIZE_ASYNC;
+        }
         if (o.mode != QLX_EXEC_PLAIN && o.mode != QLX_EXEC_WINDOWED && o.mode != QLX_EXEC_DENSE_SWEEPS)
             fail(QLX_EINVAL, "unknown execution mode");
+        if ((o.specialize & QLX_SPECIALIZE_ASYNC) && o.mode != QLX_EXEC_WINDOWED)
+            fail(QLX_EINVAL, "QLX_SPECIALIZE_ASYNC applies to window sweeps");
         // The run ends at the first MEASURE (handled by the host) or the first
         // unknown opcode (executed prefix, then the reference's runtime_error).
         size_t stop = n_instr;

@@ -146,6 +146,10 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
                       const qlx_matrix_entry* meta, size_t n_meta,
                       const qlx_exec_opts& opts, qlx_exec_stats& stats);
 
+// Waits for background sweep compilations (QLX_SPECIALIZE_ASYNC), qlx_sweep.hip; returns the
+// programs still pending and the first unreported failure in *failure.
+uint32_t specialize_wait(double timeout_s, std::string* failure);
+
 // Dense-sweep executor (QLX_EXEC_DENSE_SWEEPS), qlx_dsweep.hip.
 void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const double* pool,
                           size_t pool_elems, const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts& opts,

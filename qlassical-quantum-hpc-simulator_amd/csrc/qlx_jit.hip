@@ -14,6 +14,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -343,8 +344,7 @@ JitSweeps::~JitSweeps() {
         if (m) (void)hipModuleUnload(m);
 }
 
-std::shared_ptr<JitSweeps> build_jit_sweeps(const SweepPlan& p, int dtype) {
-    auto t0 = std::chrono::steady_clock::now();
+std::vector<std::vector<char>> compile_jit_sweeps(const SweepPlan& p, int dtype, const std::atomic<bool>* cancel) {
     const size_t n = p.dsw.size();
     std::vector<std::string> src(n);
     for (size_t s = 0; s < n; ++s) src[s] = jit_sweep_source(p, (int)s, dtype);
@@ -363,6 +363,10 @@ std::shared_ptr<JitSweeps> build_jit_sweeps(const SweepPlan& p, int dtype) {
                     if (next >= n) return;
                     s = next++;
                 }
+                if (cancel && cancel->load(std::memory_order_relaxed)) {
+                    err[s] = "specialised sweep compilation cancelled";
+                    continue;
+                }
                 try {
                     code[s] = compile(src[s], jit_kernel_name(p, (int)s) + ".hip");
                 } catch (const Error& e) {
@@ -373,6 +377,13 @@ std::shared_ptr<JitSweeps> build_jit_sweeps(const SweepPlan& p, int dtype) {
     for (auto& th : pool) th.join();
     for (size_t s = 0; s < n; ++s)
         if (!err[s].empty()) fail(QLX_EHIP, err[s]);
+    return code;
+}
+
+std::shared_ptr<JitSweeps> load_jit_sweeps(const SweepPlan& p, int dtype, const std::vector<std::vector<char>>& code,
+                                           double compile_s) {
+    const size_t n = p.dsw.size();
+    if (code.size() != n) fail(QLX_EINVAL, "specialised sweeps: code objects do not match the plan");
     auto j = std::make_shared<JitSweeps>();
     j->fns.resize(n);
     j->modules.resize(n, nullptr);
@@ -395,8 +406,15 @@ std::shared_ptr<JitSweeps> build_jit_sweeps(const SweepPlan& p, int dtype) {
             j->resident[s] = (uint64_t)std::max(per_cu, 1) * (uint64_t)std::max(cus, 1);
         }
     }
-    j->compile_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    j->compile_s = compile_s;
     return j;
+}
+
+std::shared_ptr<JitSweeps> build_jit_sweeps(const SweepPlan& p, int dtype) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const std::vector<std::vector<char>> code = compile_jit_sweeps(p, dtype, nullptr);
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return load_jit_sweeps(p, dtype, code, s);
 }
 
 void launch_jit_sweep(const JitSweeps& j, int sweep_index, qlx_state* st, int w, uint32_t wg_override) {

@@ -1,6 +1,7 @@
 // qlx_jit.hpp — circuit-specialised window-sweep kernels compiled at run time (hipRTC).
 #pragma once
 
+#include <atomic>
 #include <memory>
 #include <string>
 #include <vector>
@@ -31,8 +32,13 @@ std::string jit_kernel_name(const SweepPlan& p, int sweep_index);
 bool jit_persistent(const Geometry& g, int dtype);
 // HIP source of sweep `sweep_index` of the plan, specialised to its descriptors.
 std::string jit_sweep_source(const SweepPlan& p, int sweep_index, int dtype);
-// Generates and compiles every sweep of the plan for gfx950 (parallel host threads),
-// then loads them on the current device.
+// Generates and compiles every sweep of the plan for gfx950 (host threads; no device work):
+// one code object per sweep. `cancel` (may be null) stops before the next sweep's compilation.
+std::vector<std::vector<char>> compile_jit_sweeps(const SweepPlan& p, int dtype, const std::atomic<bool>* cancel);
+// Loads compiled sweeps on the current device.
+std::shared_ptr<JitSweeps> load_jit_sweeps(const SweepPlan& p, int dtype, const std::vector<std::vector<char>>& code,
+                                           double compile_s);
+// compile_jit_sweeps + load_jit_sweeps.
 std::shared_ptr<JitSweeps> build_jit_sweeps(const SweepPlan& p, int dtype);
 // One launch of a compiled sweep over the state: one work-group per chunk, or, for the
 // persistent kernels, as many work-groups as the device holds at once (wg_override: that

@@ -19,7 +19,14 @@
 // CCX/CSWAP and dense gates on 3+ qubits run as their own plain kernels between sweeps.
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <complex>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -1130,22 +1137,273 @@ static CachedPlan& plan_cache(qlx_state* st) {
     return *static_cast<CachedPlan*>(st->sweep_cache.get());
 }
 
+// Host plan of a windowed program: sweeps, phases, descriptors; complex64 generic plans fuse
+// heavy register phases for the matrix cores; out-of-place plans get their qubit layouts.
+static std::unique_ptr<SweepPlan> make_plan(uint32_t nq, int dtype, const qlx_gate_instr* instrs, size_t n,
+                                            const double* pool, const qlx_matrix_entry* meta,
+                                            const qlx_exec_opts& opts, uint64_t key) {
+    std::unique_ptr<SweepPlan> p = build_plan(nq, dtype, instrs, n, pool, meta, opts);
+    p->tag = key;
+    if (dtype == QLX_C64 && !(opts.specialize & QLX_SPECIALIZE)) densify_phases(*p);
+    if ((opts.specialize & QLX_SPECIALIZE) && (opts.specialize & QLX_SPECIALIZE_RELABEL)) assign_layouts(*p, nq);
+    return p;
+}
+
+// The second buffer of out-of-place sweeps; false (and no error) when it does not fit.
+static bool ensure_alt(qlx_state* st) {
+    if (st->alt) return true;
+    if (hipMalloc(&st->alt, st->bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        st->alt = nullptr;
+        return false;
+    }
+    return true;
+}
+
+// ---- background specialisation (QLX_SPECIALIZE_ASYNC) ----------------------------------
+// One entry per program (key: stream, pool, options, size, precision), shared by every state
+// of the process. A single host thread plans each queued program and compiles its sweeps;
+// executions pick the compiled sweeps up once `status` is ready and load them on their device.
+struct AsyncSpec {
+    uint64_t key = 0;
+    std::atomic<int> status{0};  // 0 queued or running, 1 ready, 2 failed
+    bool reported = false;       // failure returned by qlx_specialize_wait already
+    std::string error;
+    double compile_s = 0.0;
+    // inputs (copies: the caller's arrays are borrowed only for its call)
+    uint32_t nq = 0;
+    int dtype = QLX_C128;
+    qlx_exec_opts opts{};
+    std::vector<qlx_gate_instr> instrs;
+    std::vector<double> pool;
+    std::vector<qlx_matrix_entry> meta;
+    // results (written by the worker before status = 1)
+    std::unique_ptr<SweepPlan> plan;
+    std::vector<std::vector<char>> code;
+    // loaded modules per device (executing threads, under mu)
+    std::mutex mu;
+    std::vector<std::pair<int, std::shared_ptr<JitSweeps>>> loaded;
+};
+
+class SpecService {
+public:
+    static SpecService& get() {
+        static SpecService s;
+        return s;
+    }
+    ~SpecService() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+            cancel_.store(true);
+        }
+        cv_.notify_all();
+        if (worker_.joinable()) worker_.join();
+    }
+
+    // The program's entry, queued for planning and compilation on first sight.
+    std::shared_ptr<AsyncSpec> find_or_queue(uint64_t key, uint32_t nq, int dtype, const qlx_gate_instr* instrs,
+                                             size_t n, const double* pool, size_t pool_elems,
+                                             const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts& opts) {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto it = entries_.begin(); it != entries_.end(); ++it)
+            if ((*it)->key == key) {
+                std::shared_ptr<AsyncSpec> e = *it;
+                entries_.erase(it);
+                entries_.push_back(e);  // most recently used last
+                return e;
+            }
+        auto e = std::make_shared<AsyncSpec>();
+        e->key = key;
+        e->nq = nq;
+        e->dtype = dtype;
+        e->opts = opts;
+        e->instrs.assign(instrs, instrs + n);
+        if (pool && pool_elems) e->pool.assign(pool, pool + 2 * pool_elems);
+        if (meta && n_meta) e->meta.assign(meta, meta + n_meta);
+        entries_.push_back(e);
+        // keep the most recent programs; entries still queued or compiling stay
+        while (entries_.size() > kMaxEntries) {
+            auto victim = std::find_if(entries_.begin(), entries_.end(),
+                                       [](const std::shared_ptr<AsyncSpec>& x) { return x->status.load() != 0; });
+            if (victim == entries_.end()) break;
+            entries_.erase(victim);
+        }
+        queue_.push_back(e);
+        if (!worker_.joinable()) worker_ = std::thread([this] { run(); });
+        cv_.notify_all();
+        return e;
+    }
+
+    // Waits for the queue to drain (timeout_s < 0: no limit); returns the programs still pending
+    // and the first unreported failure.
+    uint32_t wait(double timeout_s, std::string* failure) {
+        std::unique_lock<std::mutex> lk(mu_);
+        auto idle = [&] { return queue_.empty() && !busy_; };
+        if (timeout_s < 0) cv_.wait(lk, idle);
+        else cv_.wait_for(lk, std::chrono::duration<double>(timeout_s), idle);
+        for (auto& e : entries_)
+            if (e->status.load() == 2 && !e->reported) {
+                e->reported = true;
+                if (failure && failure->empty()) *failure = e->error;
+            }
+        return (uint32_t)queue_.size() + (busy_ ? 1u : 0u);
+    }
+
+private:
+    static constexpr size_t kMaxEntries = 32;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::shared_ptr<AsyncSpec>> queue_;
+    std::vector<std::shared_ptr<AsyncSpec>> entries_;
+    std::thread worker_;
+    bool stop_ = false;
+    bool busy_ = false;
+    std::atomic<bool> cancel_{false};
+
+    void run() {
+        for (;;) {
+            std::shared_ptr<AsyncSpec> e;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || !queue_.empty(); });
+                if (stop_) return;
+                e = queue_.front();
+                queue_.pop_front();
+                busy_ = true;
+            }
+            try {
+                const auto t0 = std::chrono::steady_clock::now();
+                std::unique_ptr<SweepPlan> p =
+                    make_plan(e->nq, e->dtype, e->instrs.data(), e->instrs.size(),
+                              e->pool.empty() ? nullptr : e->pool.data(), e->meta.empty() ? nullptr : e->meta.data(),
+                              e->opts, e->key);
+                std::vector<std::vector<char>> code = compile_jit_sweeps(*p, e->dtype, &cancel_);
+                e->compile_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                e->plan = std::move(p);
+                e->code = std::move(code);
+                e->status.store(1, std::memory_order_release);
+            } catch (const Error& ex) {
+                e->error = ex.msg;
+                e->status.store(2, std::memory_order_release);
+            } catch (const std::exception& ex) {
+                e->error = ex.what();
+                e->status.store(2, std::memory_order_release);
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                busy_ = false;
+            }
+            cv_.notify_all();
+        }
+    }
+};
+
+// Compiled sweeps of a ready entry, loaded on the calling thread's current device.
+static std::shared_ptr<JitSweeps> async_kernels(AsyncSpec& e, int device) {
+    std::lock_guard<std::mutex> lk(e.mu);
+    for (auto& d : e.loaded)
+        if (d.first == device) return d.second;
+    std::shared_ptr<JitSweeps> j = load_jit_sweeps(*e.plan, e.dtype, e.code, e.compile_s);
+    e.loaded.emplace_back(device, j);
+    return j;
+}
+
+// Runs the steps of a plan: window sweeps through the compiled kernels (`jit`) or the generic
+// kernel (descriptor image at `dev`), plain kernels between them.
+static void run_plan(qlx_state* st, const SweepPlan& p, const JitSweeps* jit, const char* dev, const CachedPlan* cp,
+                     const qlx_gate_instr* instrs, size_t n, const double* pool, size_t pool_elems,
+                     const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts& opts, qlx_exec_stats& stats) {
+    bool need_pool = false;
+    for (const PlanStep& s : p.steps)
+        if (!s.sweep && (instrs[s.src].type == QLX_UNITARY || instrs[s.src].type == QLX_FUSED_BLOCK)) need_pool = true;
+    const double* dpool = need_pool ? upload_pool(st, pool, pool_elems) : nullptr;
+    prepare_dense(st, instrs, n, dpool, meta, n_meta, opts);
+    const double pass = pass_bytes(st);
+    const Geometry& geo = p.geo;
+    if (p.relabel && !p.sweeps.empty() && !st->alt) fail(QLX_EHIP, "out-of-place plan without a second buffer");
+    for (size_t k = 0; k < p.steps.size(); ++k) {
+        const PlanStep& s = p.steps[k];
+        if (!s.sweep) {
+            if (p.relabel) {
+                // the gate in the current qubit layout
+                qlx_gate_instr g = instrs[s.src];
+                const std::vector<int>& pos = p.step_pos[k];
+                const int na = operand_count(g);
+                for (int i = 0; i < na && i < 4; ++i) g.qubits[i] = (uint8_t)pos[g.qubits[i]];
+                if (na == 5) g.pad = (uint16_t)((g.pad & 0xFF00) | (uint16_t)pos[g.pad & 0xFF]);
+                launch_plain_instr(st, g, dpool, meta, opts, stats);
+            } else {
+                launch_plain_instr(st, instrs[s.src], dpool, meta, opts, stats);
+            }
+            continue;
+        }
+        if (jit) {
+            TimedLaunch tl(st, "k_sweep_spec", pass);
+            launch_jit_sweep(*jit, s.sweep_index, st, geo.w, opts.sweep_workgroups);
+        } else {
+            TimedLaunch tl(st, "k_sweep", pass);
+            const int key = (st->dtype == QLX_C128 ? 1000 : 0) + geo.w * 10 + geo.rb;
+            const size_t a = cp->off_sw, b = cp->off_ph, c = cp->off_op, d = cp->off_cf;
+            const int si = s.sweep_index;
+            switch (key) {
+                case 1124: launch_sweep<double2, 8, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                case 1125: launch_sweep<double2, 7, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                case 1114: launch_sweep<double2, 7, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                case 1113: launch_sweep<double2, 8, 3>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                case 135: launch_sweep<float2, 8, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                case 125: launch_sweep<float2, 7, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                case 124: launch_sweep<float2, 8, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                default: fail(QLX_EUNSUPPORTED, "sweep geometry not built");
+            }
+        }
+        stats.sweeps_full++;
+        stats.algorithmic_bytes += pass;
+    }
+    stats.gates_applied += p.gates_in_sweeps;
+    stats.specialised = jit ? 1u : 0u;
+    stats.relabelled = p.relabel && !p.sweeps.empty() ? 1u : 0u;
+}
+
 void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, const double* pool, size_t pool_elems,
                       const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts& opts, qlx_exec_stats& stats) {
-    const Geometry geo = geometry_for(st->dtype, st->nq, opts);
+    qlx_exec_opts o = opts;
+    if (o.specialize & QLX_SPECIALIZE_ASYNC) o.specialize |= QLX_SPECIALIZE;
+    const Geometry geo = geometry_for(st->dtype, st->nq, o);
     if ((int)st->nq < geo.w) {
         // the state is smaller than one window: every gate runs as its own kernel
-        execute_plain(st, instrs, n, pool, pool_elems, meta, n_meta, opts, stats);
+        execute_plain(st, instrs, n, pool, pool_elems, meta, n_meta, o, stats);
         return;
     }
-    const uint64_t key = program_key(st->nq, st->dtype, instrs, n, pool, pool_elems, meta, n_meta, opts);
+    // out-of-place sweeps need the second buffer; without it they run in place
+    if ((o.specialize & QLX_SPECIALIZE) && (o.specialize & QLX_SPECIALIZE_RELABEL) && !ensure_alt(st))
+        o.specialize &= ~QLX_SPECIALIZE_RELABEL;
+    if (o.specialize & QLX_SPECIALIZE_ASYNC) {
+        // keyed (and its kernels named) as the same program compiled in the foreground, so the
+        // two share the compiler's code-object cache
+        qlx_exec_opts ko = o;
+        ko.specialize &= ~QLX_SPECIALIZE_ASYNC;
+        ko.mode = QLX_EXEC_WINDOWED;
+        const uint64_t key = program_key(st->nq, st->dtype, instrs, n, pool, pool_elems, meta, n_meta, ko);
+        std::shared_ptr<AsyncSpec> e = SpecService::get().find_or_queue(key, st->nq, st->dtype, instrs, n, pool,
+                                                                        pool_elems, meta, n_meta, o);
+        if (e->status.load(std::memory_order_acquire) == 1) {
+            std::shared_ptr<JitSweeps> j = async_kernels(*e, st->device);
+            run_plan(st, *e->plan, j.get(), nullptr, nullptr, instrs, n, pool, pool_elems, meta, n_meta, o, stats);
+            return;
+        }
+        // until the compiled sweeps are ready (or if their compilation failed): the generic
+        // kernel, in place, on the default geometry when the requested one is not built for it
+        o.specialize = 0;
+        if (!geometry_built(st->dtype, geo.w, geo.rb, false)) {
+            o.window_qubits = 0;
+            o.register_qubits = 0;
+        }
+    }
+    const uint64_t key = program_key(st->nq, st->dtype, instrs, n, pool, pool_elems, meta, n_meta, o);
     CachedPlan& cp = plan_cache(st);
-    const bool relabel = (opts.specialize & QLX_SPECIALIZE) && (opts.specialize & QLX_SPECIALIZE_RELABEL);
     if (!cp.plan || cp.key != key) {
-        cp.plan = build_plan(st->nq, st->dtype, instrs, n, pool, meta, opts);
-        cp.plan->tag = key;
-        if (st->dtype == QLX_C64 && !(opts.specialize & QLX_SPECIALIZE)) densify_phases(*cp.plan);
-        if (relabel) assign_layouts(*cp.plan, st->nq);
+        cp.plan = make_plan(st->nq, st->dtype, instrs, n, pool, meta, o, key);
         cp.key = key;
         cp.jit.reset();
         SweepPlan& p = *cp.plan;
@@ -1165,61 +1423,12 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
         QLX_HIP_CHECK(hipStreamSynchronize(st->stream));
     }
     const SweepPlan& p = *cp.plan;
-    bool need_pool = false;
-    for (const PlanStep& s : p.steps)
-        if (!s.sweep && (instrs[s.src].type == QLX_UNITARY || instrs[s.src].type == QLX_FUSED_BLOCK)) need_pool = true;
-    const double* dpool = need_pool ? upload_pool(st, pool, pool_elems) : nullptr;
-    prepare_dense(st, instrs, n, dpool, meta, n_meta, opts);
-    const char* dev = (const char*)cp.dev.ptr;
-    const double pass = pass_bytes(st);
-    if (p.relabel && !p.sweeps.empty() && !st->alt) {
-        if (hipMalloc(&st->alt, st->bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            st->alt = nullptr;
-            fail(QLX_ENOMEM, "out-of-place sweeps need a second buffer of " + std::to_string(st->bytes) + " bytes");
-        }
-    }
-    for (size_t k = 0; k < p.steps.size(); ++k) {
-        const PlanStep& s = p.steps[k];
-        if (!s.sweep) {
-            if (p.relabel) {
-                // the gate in the current qubit layout
-                qlx_gate_instr g = instrs[s.src];
-                const std::vector<int>& pos = p.step_pos[k];
-                const int na = operand_count(g);
-                for (int i = 0; i < na && i < 4; ++i) g.qubits[i] = (uint8_t)pos[g.qubits[i]];
-                if (na == 5) g.pad = (uint16_t)((g.pad & 0xFF00) | (uint16_t)pos[g.pad & 0xFF]);
-                launch_plain_instr(st, g, dpool, meta, opts, stats);
-            } else {
-                launch_plain_instr(st, instrs[s.src], dpool, meta, opts, stats);
-            }
-            continue;
-        }
-        if (opts.specialize & QLX_SPECIALIZE) {
-            if (!cp.jit) cp.jit = build_jit_sweeps(p, st->dtype);
-            TimedLaunch tl(st, "k_sweep_spec", pass);
-            launch_jit_sweep(*cp.jit, s.sweep_index, st, geo.w, opts.sweep_workgroups);
-        } else {
-            TimedLaunch tl(st, "k_sweep", pass);
-            const int key = (st->dtype == QLX_C128 ? 1000 : 0) + geo.w * 10 + geo.rb;
-            const size_t a = cp.off_sw, b = cp.off_ph, c = cp.off_op, d = cp.off_cf;
-            const int si = s.sweep_index;
-            switch (key) {
-                case 1124: launch_sweep<double2, 8, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
-                case 1125: launch_sweep<double2, 7, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
-                case 1114: launch_sweep<double2, 7, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
-                case 1113: launch_sweep<double2, 8, 3>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
-                case 135: launch_sweep<float2, 8, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
-                case 125: launch_sweep<float2, 7, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
-                case 124: launch_sweep<float2, 8, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
-                default: fail(QLX_EUNSUPPORTED, "sweep geometry not built");
-            }
-        }
-        stats.sweeps_full++;
-        stats.algorithmic_bytes += pass;
-    }
-    stats.gates_applied += p.gates_in_sweeps;
+    if ((o.specialize & QLX_SPECIALIZE) && !cp.jit) cp.jit = build_jit_sweeps(p, st->dtype);
+    run_plan(st, p, (o.specialize & QLX_SPECIALIZE) ? cp.jit.get() : nullptr, (const char*)cp.dev.ptr, &cp, instrs, n,
+             pool, pool_elems, meta, n_meta, o, stats);
 }
+
+uint32_t specialize_wait(double timeout_s, std::string* failure) { return SpecService::get().wait(timeout_s, failure); }
 
 }  // namespace qlx
 
@@ -1287,6 +1496,15 @@ extern "C" int qlx_plan_windowed(uint32_t num_qubits, int dtype, const qlx_gate_
                 for (const DevPhase& ph : p->dph) info->mfma_phases += ph.mfma ? 1 : 0;
             }
         }
+    });
+}
+
+extern "C" int qlx_specialize_wait(double timeout_s, uint32_t* pending) {
+    return guard([&] {
+        std::string failure;
+        const uint32_t left = specialize_wait(timeout_s, &failure);
+        if (pending) *pending = left;
+        if (!failure.empty()) fail(QLX_EHIP, "background sweep compilation failed: " + failure);
     });
 }
 

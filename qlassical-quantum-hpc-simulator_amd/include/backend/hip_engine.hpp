@@ -15,7 +15,9 @@
 // Differences, by design: GLOBAL_SWAP is a real qubit permutation (the reference's is
 // a no-op, :217-219) unless HIPEngineOptions::global_swap_noop is set; qubit operands
 // are validated before launch (out-of-range -> std::out_of_range) instead of being
-// undefined behaviour.
+// undefined behaviour; set_seed reseeds the measurement generator (the reference's
+// body is empty, :130-131, while its seeded-determinism tests construct engines with a
+// seed: reseeding is what a caller of set_seed asks for).
 #ifndef QLASSICAL_BACKEND_HIP_ENGINE_HPP
 #define QLASSICAL_BACKEND_HIP_ENGINE_HPP
 
@@ -52,6 +54,12 @@ struct HIPEngineOptions {
     // complex64 fused programs: every gate inside 13-qubit LDS windows, 4/5-qubit gates on the
     // matrix cores (QLX_EXEC_DENSE_SWEEPS; takes precedence over windowed)
     bool dense_sweeps = false;
+    // windowed without specialize (the default): states of >= QLX_AUTO_SPECIALIZE_QUBITS qubits
+    // get specialised, relabelled sweeps compiled on a background host thread (QLX_EXEC_AUTO).
+    // A program's first executions run the generic sweep kernel; once its kernels are compiled
+    // (once per process; the ROCm compiler's disk cache also serves later processes) every
+    // execution of it, from any engine, runs them. Off: the generic sweep kernel only.
+    bool auto_specialize = true;
 };
 
 class HIPEngine final : public ExecutionEngine {
@@ -94,7 +102,10 @@ public:
 
     void execute(std::span<const GateInstr> program, const UnitaryPool& pool) override {
         qlx_exec_opts o{};
-        o.mode = opts_.dense_sweeps ? QLX_EXEC_DENSE_SWEEPS : opts_.windowed ? QLX_EXEC_WINDOWED : QLX_EXEC_PLAIN;
+        o.mode = opts_.dense_sweeps ? QLX_EXEC_DENSE_SWEEPS
+                 : !opts_.windowed  ? QLX_EXEC_PLAIN
+                 : (opts_.auto_specialize && !opts_.specialize) ? QLX_EXEC_AUTO
+                                                                 : QLX_EXEC_WINDOWED;
         o.gswap_mode = opts_.global_swap_noop ? QLX_GSWAP_NOOP : QLX_GSWAP_PERMUTE;
         if (opts_.dense_sweeps)
             o.specialize = opts_.relabel ? QLX_SPECIALIZE_RELABEL : 0u;

@@ -26,8 +26,11 @@ QLX_EXEC_WINDOWED = 1
 QLX_GSWAP_PERMUTE = 0
 QLX_GSWAP_NOOP = 1
 QLX_EXEC_DENSE_SWEEPS = 2
+QLX_EXEC_AUTO = 3
+QLX_AUTO_SPECIALIZE_QUBITS = 20
 QLX_SPECIALIZE = 1
 QLX_SPECIALIZE_RELABEL = 2
+QLX_SPECIALIZE_ASYNC = 4
 QLX_DENSE_AUTO = 0
 QLX_DENSE_VALU = 1
 QLX_DENSE_MFMA_BF16 = 2
@@ -86,6 +89,8 @@ class ExecStats(C.Structure):
         ("sweeps_quarter", C.c_uint64),
         ("gates_applied", C.c_uint64),
         ("algorithmic_bytes", C.c_double),
+        ("specialised", C.c_uint32),
+        ("relabelled", C.c_uint32),
     ]
 
 
@@ -184,6 +189,7 @@ SIGNATURES = [
     ("qlx_plan_sweep_source", [C.c_uint32, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,
                                C.c_size_t, C.POINTER(ExecOpts), C.c_uint32, C.c_char_p, C.c_size_t,
                                C.POINTER(C.c_size_t)]),
+    ("qlx_specialize_wait", [C.c_double, C.POINTER(C.c_uint32)]),
     ("qlx_last_exec_stats", [C.c_void_p, C.POINTER(ExecStats)]),
     ("qlx_measure_prob1", [C.c_void_p, C.c_uint32, C.POINTER(C.c_double)]),
     ("qlx_collapse", [C.c_void_p, C.c_uint32, C.c_int, C.c_double]),

@@ -34,19 +34,26 @@ DENSE = {"auto": _abi.QLX_DENSE_AUTO, "valu": _abi.QLX_DENSE_VALU, "bf16": _abi.
 
 def exec_opts(mode: str = "plain", window_qubits: int = 0, coalesce_qubits: int = 0,
               gswap: str = "permute", register_qubits: int = 0, dense: str = "auto",
-              sweep_workgroups: int = 0, specialize: bool = False, relabel: bool = False) -> _abi.ExecOpts:
-    """qlx_exec_opts. dense: arithmetic of 4/5-qubit dense gates on complex64 states
+              sweep_workgroups: int = 0, specialize: bool = False, relabel: bool = False,
+              background: bool = False) -> _abi.ExecOpts:
+    """qlx_exec_opts. mode: 'plain', 'windowed', 'dense' or 'auto' (the C++ HIPEngine's default:
+    window sweeps, with specialised relabelled sweeps compiled in the background for states of
+    >= 20 qubits). dense: arithmetic of 4/5-qubit dense gates on complex64 states
     ('auto' = 'f16' matrix cores (FP16 hi/lo split), 'bf16' matrix cores (BF16 split), 'f32'
     matrix cores, 'valu' vector units). relabel (windowed: with specialize; dense: alone):
-    out-of-place sweeps that re-lay out the qubits between two buffers."""
+    out-of-place sweeps that re-lay out the qubits between two buffers. background (windowed,
+    with specialize): compile on a host thread and run the generic kernel until ready."""
     if relabel and not specialize and mode != "dense":
         raise ValueError("relabel needs specialize")
+    if background and not specialize:
+        raise ValueError("background needs specialize")
     o = _abi.ExecOpts()
     o.dense_kernel = DENSE[dense]
     o.sweep_workgroups = sweep_workgroups
-    o.specialize = (_abi.QLX_SPECIALIZE if specialize else 0) | (_abi.QLX_SPECIALIZE_RELABEL if relabel else 0)
+    o.specialize = ((_abi.QLX_SPECIALIZE if specialize else 0) | (_abi.QLX_SPECIALIZE_RELABEL if relabel else 0) |
+                    (_abi.QLX_SPECIALIZE_ASYNC if background else 0))
     o.mode = {"plain": _abi.QLX_EXEC_PLAIN, "windowed": _abi.QLX_EXEC_WINDOWED,
-              "dense": _abi.QLX_EXEC_DENSE_SWEEPS}[mode]
+              "dense": _abi.QLX_EXEC_DENSE_SWEEPS, "auto": _abi.QLX_EXEC_AUTO}[mode]
     o.window_qubits = window_qubits
     o.register_qubits = register_qubits
     o.coalesce_qubits = coalesce_qubits
@@ -89,6 +96,14 @@ def sweep_source(prog: Program, sweep_index: int, dtype="c128", opts: Optional[_
     buf = C.create_string_buffer(n.value + 1)
     _abi.check(L.qlx_plan_sweep_source(*args, buf, n.value + 1, C.byref(n)))
     return buf.value.decode()
+
+
+def specialize_wait(timeout_s: float = -1.0) -> int:
+    """Waits for background sweep compilations (qlx_specialize_wait); returns how many
+    programs are still pending. Raises if a finished compilation failed."""
+    n = C.c_uint32(0)
+    _abi.check(_abi.lib().qlx_specialize_wait(float(timeout_s), C.byref(n)))
+    return n.value
 
 
 def device_info(device: int = 0) -> dict:

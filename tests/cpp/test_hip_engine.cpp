@@ -388,6 +388,78 @@ TEST_CASE("complex64 dense sweeps on a width-5 fused module (HIPEngineOptions::d
     }
 }
 
+TEST_CASE("block hand-off moves the device allocation (circuit_runner.hpp:154-163: move, never copy)") {
+    // the sequence CircuitRunner::run performs between blocks: execute, extract the state
+    // vector, build the next engine from it
+    const int n = 12;
+    QuantumCircuit a(n), b(n);
+    for (int q = 0; q < n; ++q) a.h(q);
+    for (int q = 0; q + 1 < n; ++q) b.cx(q, q + 1);
+    IRModule ma = std::move(a).release(), mb = std::move(b).release();
+    HIPEngine e1(n, std::nullopt, 0, opts());
+    e1.execute(ma);
+    void* p1 = nullptr;
+    CHECK(qlx_state_device_ptr(e1.state_vector().handle(), &p1) == QLX_OK && p1 != nullptr);
+    qlx_state* h1 = e1.state_vector().handle();
+    GPUStateVector carried = std::move(e1).extract_state_vector();
+    HIPEngine e2(std::move(carried), opts());
+    void* p2 = nullptr;
+    CHECK(e2.state_vector().handle() == h1);
+    CHECK(qlx_state_device_ptr(e2.state_vector().handle(), &p2) == QLX_OK && p2 == p1);
+    e2.execute(mb);
+    const auto s = e2.state();
+    double total = 0.0;
+    for (auto v : s) total += std::norm(v);
+    CHECK_NEAR(total, 1.0, 1e-12);
+}
+
+TEST_CASE("HIPEngineOptions{} at 22 qubits: generic sweeps, then the background-compiled ones") {
+    if (g_windowed && !g_relabel) {
+        const int n = 22;
+        QuantumCircuit qc(n);
+        for (int q = 0; q < n; ++q) qc.h(q);
+        for (int l = 0; l < 6; ++l) {
+            for (int q = 0; q < n; ++q) qc.rx(q, 0.3 + 0.11 * q + 0.7 * l);
+            for (int q = l % 2; q + 1 < n; q += 2) qc.cx(q, q + 1);
+        }
+        IRModule m = std::move(qc).release();
+        // a non-uniform start (the circuit maps |+...+> to itself up to a phase)
+        std::vector<std::complex<double>> init(std::size_t(1) << n);
+        double ns = 0.0;
+        for (std::size_t i = 0; i < init.size(); ++i) {
+            init[i] = {std::cos(0.37 * (double)i), std::sin(0.11 * (double)(i % 977))};
+            ns += std::norm(init[i]);
+        }
+        for (auto& v : init) v /= std::sqrt(ns);
+        HIPEngineOptions plain;
+        plain.windowed = false;
+        HIPEngine ref(std::span<const std::complex<double>>(init), std::nullopt, 0, plain);
+        ref.execute(m);
+        const std::vector<std::complex<double>> want(ref.state().begin(), ref.state().end());
+        const std::span<const std::complex<double>> init_span(init);
+        HIPEngine e{init_span};  // HIPEngineOptions{}
+        CHECK(e.options().auto_specialize && e.options().windowed && !e.options().specialize);
+        e.execute(m);
+        double err = 0.0;
+        for (std::size_t i = 0; i < want.size(); ++i) err = std::max(err, std::abs(e.state()[i] - want[i]));
+        CHECK(err <= 1e-12);
+        uint32_t pending = 1;
+        CHECK(qlx_specialize_wait(300.0, &pending) == QLX_OK && pending == 0);
+        e.reset(init_span);
+        e.execute(m);
+        qlx_exec_stats st{};
+        CHECK(qlx_last_exec_stats(e.state_vector().handle(), &st) == QLX_OK);
+        CHECK(st.specialised == 1 && st.relabelled == 1);
+        err = 0.0;
+        for (std::size_t i = 0; i < want.size(); ++i) err = std::max(err, std::abs(e.state()[i] - want[i]));
+        CHECK(err <= 1e-12);
+        // another engine (a later CircuitRunner block) runs the compiled sweeps at once
+        HIPEngine e2{init_span};
+        e2.execute(m);
+        CHECK(qlx_last_exec_stats(e2.state_vector().handle(), &st) == QLX_OK && st.specialised == 1);
+    }
+}
+
 int main(int argc, char** argv) {
     const char* filter = argc > 1 ? argv[1] : nullptr;
     std::printf("== one kernel per gate ==\n");
