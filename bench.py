@@ -550,7 +550,9 @@ def main():
     # sweeps.
     dropin = None
     if args.mode == "auto" and not (args.window or args.regs or args.generic):
-        dp = qlx.transpile_gpu(prog)          # GpuTranspilerConfig{} (fusion width 4, H/CX blocks)
+        # GpuTranspilerConfig{}: fusion on, width 4, the reference's fusable set (H/X/Y/Z/S/CX and
+        # matrices; RX is not in it, so this circuit comes out unchanged)
+        dp = qlx.transpile_gpu(prog, fusion=True, max_width=4, rotations=False)
         oa = qlx.exec_opts("auto")
         t0 = time.perf_counter()
         st.execute(dp, oa)
