@@ -363,6 +363,15 @@ int qlx_server_submit(qlx_server* s, uint32_t num_qubits, const qlx_gate_instr* 
 int qlx_server_pump(qlx_server* s, uint32_t* queued, uint32_t* active, uint32_t* inflight);
 /* Requests completed since the last call, with submit-to-completion latency in seconds. */
 int qlx_server_take_completed(qlx_server* s, uint64_t* ids, double* latency_s, size_t cap, size_t* n_out);
+/* Where a request is: QLX_REQ_QUEUED (submitted, not yet admitted), QLX_REQ_ACTIVE (admitted:
+ * its state is in the arena and its sweeps are being launched), QLX_REQ_DONE (its last sweep
+ * has completed). times_s (may be NULL) receives {submit, admit, done} in seconds since the
+ * server was created, -1 for a step not reached; admission and completion are the times the
+ * server's pump observed them. */
+#define QLX_REQ_QUEUED 0
+#define QLX_REQ_ACTIVE 1
+#define QLX_REQ_DONE   2
+int qlx_server_request_info(qlx_server* s, uint64_t id, int* status, double* times_s);
 /* As qlx_batch_compare, for a completed request. */
 int qlx_server_compare(qlx_server* s, uint64_t id, qlx_state* ref, double* infidelity, double* max_abs);
 int qlx_server_state_to_host(qlx_server* s, uint64_t id, double* out_c128, size_t dim);

@@ -513,6 +513,22 @@ int qlx_server_pump(qlx_server* s, uint32_t* queued, uint32_t* active, uint32_t*
     });
 }
 
+int qlx_server_request_info(qlx_server* s, uint64_t id, int* status, double* times_s) {
+    return guard([&] {
+        if (!s) fail(QLX_EINVAL, "null server");
+        const Req& r = get_req(s, id);
+        if (status)
+            *status = r.status == qlx_server::QUEUED ? QLX_REQ_QUEUED
+                      : r.status == qlx_server::ACTIVE ? QLX_REQ_ACTIVE
+                                                       : QLX_REQ_DONE;
+        if (times_s) {
+            times_s[0] = r.submit_s;
+            times_s[1] = r.status == qlx_server::QUEUED ? -1.0 : r.admit_s;
+            times_s[2] = r.status == qlx_server::DONE ? r.done_s : -1.0;
+        }
+    });
+}
+
 int qlx_server_take_completed(qlx_server* s, uint64_t* ids, double* latency_s, size_t cap, size_t* n_out) {
     return guard([&] {
         if (!s) fail(QLX_EINVAL, "null server");

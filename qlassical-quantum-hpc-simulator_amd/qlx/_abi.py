@@ -215,6 +215,7 @@ SIGNATURES = [
     ("qlx_server_create", [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
     ("qlx_server_submit", [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint64)]),
     ("qlx_server_pump", [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("qlx_server_request_info", [C.c_void_p, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     ("qlx_server_take_completed", [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     ("qlx_server_compare", [C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("qlx_server_state_to_host", [C.c_void_p, C.c_uint64, C.c_void_p, C.c_size_t]),

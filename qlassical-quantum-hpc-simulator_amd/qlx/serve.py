@@ -30,6 +30,14 @@ class Server:
         _abi.check(_abi.lib().qlx_server_pump(self._h, C.byref(q), C.byref(a), C.byref(f)))
         return q.value, a.value, f.value
 
+    def info(self, rid: int) -> Tuple[str, float, float, float]:
+        """(status, submit_s, admit_s, done_s): status 'queued' | 'active' | 'done'; times in
+        seconds since the server was created, -1 for a step not reached."""
+        st = C.c_int(0)
+        t = (C.c_double * 3)()
+        _abi.check(_abi.lib().qlx_server_request_info(self._h, rid, C.byref(st), t))
+        return ({0: "queued", 1: "active", 2: "done"}[st.value], t[0], t[1], t[2])
+
     def take_completed(self, cap: int = 4096) -> List[Tuple[int, float]]:
         ids = (C.c_uint64 * cap)()
         lat = (C.c_double * cap)()
