@@ -220,6 +220,20 @@ int qlx_execute(qlx_state* st,
                 const double* pool_c128, size_t pool_elems,
                 const qlx_matrix_entry* meta, size_t n_meta,
                 const qlx_exec_opts* opts, size_t* stop_at);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops). qlx_execute in QLX_EXEC_PLAIN mode (opts NULL or
+ * mode QLX_EXEC_PLAIN; any other mode -> QLX_EUNSUPPORTED) that allocates nothing: the matrices
+ * of UNITARY / FUSED_BLOCK gates are staged in a fixed 1 MiB device array of the library (one
+ * per GPU, held until the process ends) instead of the state's program buffer, so pool_elems
+ * may be at most 65536 (larger -> QLX_EINVAL); dense gates run on the vector units; a state
+ * with kernel timing enabled is refused (QLX_EINVAL). Same results as qlx_execute in plain
+ * mode with QLX_DENSE_VALU. Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (states on different streams) race. */
+int qlx_execute_noalloc(qlx_state* st,
+                        const qlx_gate_instr* instrs, size_t n_instr,
+                        const double* pool_c128, size_t pool_elems,
+                        const qlx_matrix_entry* meta, size_t n_meta,
+                        const qlx_exec_opts* opts, size_t* stop_at);
 /* Host-only planning of QLX_EXEC_WINDOWED (or, with opts->mode = QLX_EXEC_DENSE_SWEEPS, of
  * dense-sweep) execution (no GPU work): the order in which the instructions of a
  * MEASURE-free stream would run. order[k] is the source index of the k-th executed

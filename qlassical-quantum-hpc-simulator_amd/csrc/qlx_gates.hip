@@ -11,6 +11,7 @@
 // Index arithmetic is 64-bit: states up to 2^34 amplitudes (256 GiB c128) fit one MI355X.
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 
 #include "qlx_internal.hpp"
@@ -563,6 +564,96 @@ int qlx_execute(qlx_state* st, const qlx_gate_instr* instrs, size_t n_instr, con
             else
                 execute_windowed(st, instrs, run, pool, pool_elems, meta, n_meta, o, stats);
         }
+        st->stats = stats;
+        if (bad < stop) {
+            QLX_HIP_CHECK(hipStreamSynchronize(st->stream));
+            fail(QLX_EUNSUPPORTED, "CPUOpenMPEngine: Unrecognized instruction type");
+        }
+        if (stop_at) *stop_at = stop;
+    });
+}
+
+}  // extern "C"
+
+// ---- execution without device allocation ------------------------------------------------
+// The matrices of UNITARY / FUSED_BLOCK gates are staged in `scratch`, a file-scope device
+// array of kNoAllocPoolBytes per GPU (held until the process ends), instead of the state's
+// growable program buffer. In host code `scratch` names a host-side stand-in, not a device
+// address: host code uses only its device address, taken once per device with
+// hipGetSymbolAddress. Every call on one GPU shares that GPU's array; calls that overlap on
+// one GPU (states on different streams) race on it.
+namespace qlx {
+constexpr size_t kNoAllocPoolBytes = size_t(1) << 20;  // 65536 complex128 matrix elements
+static __device__ __align__(256) unsigned char scratch[kNoAllocPoolBytes];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` on each GPU
+
+void* scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) fail(QLX_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        QLX_HIP_CHECK(hipGetDevice(&cur));
+        if (cur != device) QLX_HIP_CHECK(hipSetDevice(device));
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != device) QLX_HIP_CHECK(hipSetDevice(cur));
+        if (e != hipSuccess || !ptr)
+            fail(QLX_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+}  // namespace qlx
+
+extern "C" {
+
+int qlx_execute_noalloc(qlx_state* st, const qlx_gate_instr* instrs, size_t n_instr, const double* pool,
+                        size_t pool_elems, const qlx_matrix_entry* meta, size_t n_meta, const qlx_exec_opts* opts,
+                        size_t* stop_at) {
+    return guard([&] {
+        if (!st || !st->amp) fail(QLX_EINVAL, "null state handle");
+        if (n_instr && !instrs) fail(QLX_EINVAL, "null instruction stream");
+        qlx_exec_opts o{};
+        if (opts) o = *opts;
+        if (o.mode != QLX_EXEC_PLAIN)
+            fail(QLX_EUNSUPPORTED, "qlx_execute_noalloc runs one kernel per gate (QLX_EXEC_PLAIN) only");
+        if (st->timing) fail(QLX_EINVAL, "qlx_execute_noalloc does not time kernels: disable timing on the state");
+        if (pool_elems > kNoAllocPoolBytes / 16)
+            fail(QLX_EINVAL, "qlx_execute_noalloc: the unitary pool holds " + std::to_string(pool_elems) +
+                                 " complex elements; at most " + std::to_string(kNoAllocPoolBytes / 16) +
+                                 " fit the staging area");
+        if (pool_elems && !pool) fail(QLX_EINVAL, "null unitary pool");
+        size_t stop = n_instr;
+        for (size_t i = 0; i < n_instr; ++i)
+            if (instrs[i].type == QLX_MEASURE) {
+                stop = i;
+                break;
+            }
+        const size_t bad = first_unknown(instrs, stop);
+        const size_t run = std::min(stop, bad);
+        for (size_t i = 0; i < run; ++i) validate_instr(instrs[i], st->nq, meta, n_meta, pool_elems);
+        if (stop < n_instr && stop == run) validate_instr(instrs[stop], st->nq, meta, n_meta, pool_elems);
+        bool need_pool = false;
+        for (size_t i = 0; i < run; ++i)
+            if (instrs[i].type == QLX_UNITARY || instrs[i].type == QLX_FUSED_BLOCK) need_pool = true;
+        const double* dpool = nullptr;
+        if (need_pool && pool_elems) {
+            void* dst = scratch_on(st->device);
+            // same stream as the kernels below; the source is pageable caller memory, so the copy
+            // completes before this returns
+            QLX_HIP_CHECK(hipMemcpyAsync(dst, pool, pool_elems * 16, hipMemcpyHostToDevice, st->stream));
+            QLX_HIP_CHECK(hipStreamSynchronize(st->stream));
+            dpool = (const double*)dst;
+        }
+        // dense gates on the vector units (the matrix-core forms need fragment buffers)
+        st->frag_index.clear();
+        o.dense_kernel = QLX_DENSE_VALU;
+        qlx_exec_stats stats{};
+        for (size_t i = 0; i < run; ++i) launch_plain_instr(st, instrs[i], dpool, meta, o, stats);
         st->stats = stats;
         if (bad < stop) {
             QLX_HIP_CHECK(hipStreamSynchronize(st->stream));

@@ -183,6 +183,8 @@ SIGNATURES = [
     ("qlx_state_copy", [C.c_void_p, C.c_void_p]),
     ("qlx_execute", [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
                      C.POINTER(ExecOpts), C.POINTER(C.c_size_t)]),
+    ("qlx_execute_noalloc", [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
+                             C.POINTER(ExecOpts), C.POINTER(C.c_size_t)]),
     ("qlx_plan_windowed", [C.c_uint32, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,
                            C.c_size_t, C.POINTER(ExecOpts), C.c_void_p, C.c_void_p, C.c_size_t,
                            C.POINTER(C.c_size_t), C.POINTER(PlanInfo)]),
