@@ -55,9 +55,42 @@ struct Gen {
     int logt, rb, w, R, elem;
     std::string out;
     std::vector<int> map;  // register index -> variable
+    // pending conditional X (OP_CXT): xf[r] = mask of work-item bits whose parity, per
+    // work-item, says whether register bit r is flipped (register i holds the amplitude of
+    // register position i ^ f, f = the flipped bits)
+    std::vector<uint32_t> xf;
     int tmp = 0;
     void line(const std::string& s) { out += s; out += '\n'; }
     std::string var(int i) const { return "r" + std::to_string(map[i]); }
+    bool frame_pending() const {
+        for (uint32_t m : xf)
+            if (m) return true;
+        return false;
+    }
+    std::string parity(uint32_t mask) const {
+        return "((__builtin_popcount(" + tv + " & " + std::to_string(mask) + "u) & 1u) != 0u)";
+    }
+    // applies the pending X on register bit r: conditional swaps of the register pairs
+    void materialize(int r) {
+        if (!xf[r]) return;
+        const std::string T2 = f32 ? "float2" : "double2";
+        line("  { const bool fc = " + parity(xf[r]) + ";");
+        for (int i = 0; i < R; ++i) {
+            if (i & (1 << r)) continue;
+            const std::string x = var(i), y = var(i | (1 << r));
+            line("    { const " + T2 + " a0 = " + x + ", a1 = " + y + "; " + x + " = fc ? a1 : a0; " + y +
+                 " = fc ? a0 : a1; }");
+        }
+        line("  }");
+        xf[r] = 0;
+    }
+    // per-work-item XOR of `offset(bit r)` over the flipped register bits, as `name`
+    template <class F>
+    void frame_offset(const std::string& type, const std::string& name, F offset) {
+        line("  " + type + " " + name + " = 0;");
+        for (int r = 0; r < rb; ++r)
+            if (xf[r]) line("  if " + parity(xf[r]) + " " + name + " ^= " + offset(r) + ";");
+    }
 
     std::string tv = "t";  // the variable holding the work-item index
 
@@ -79,7 +112,19 @@ struct Gen {
         const std::string T = f32 ? "float" : "double", T2 = f32 ? "float2" : "double2";
         const double* m = p.coef.data() + o.coef;
         const std::string id = std::to_string(tmp++);
+        // pending X frames: RX, SCALE and the CX/SWAP renames carry them; other ops apply them
         switch (kind) {
+            case OP_RXT: case OP_RXU: case OP_SCALE: case OP_CXT: break;
+            case OP_CX: xf[b1] ^= xf[b0]; break;               // X on the control spreads to the target
+            case OP_SWAP: std::swap(xf[b0], xf[b1]); break;
+            case OP_U4: materialize(b0); materialize(b1); break;
+            default: materialize(b0); break;                     // OP_HB, OP_U2
+        }
+        switch (kind) {
+            case OP_CXT:
+                // control on work-item bit b0, target register bit b1: recorded, applied later
+                xf[b1] ^= 1u << b0;
+                break;
             case OP_RXT: {
                 // RX / c: a0' = a0 - i t a1, a1' = a1 - i t a0
                 const std::string F = f32 ? "__builtin_fmaf" : "__builtin_fma";
@@ -233,6 +278,7 @@ struct Gen {
         emit_base("c", "base", "  ");
         line("  const unsigned long long lo = base | tl;");
         map.resize(R);
+        xf.assign(rb, 0u);
         for (int i = 0; i < R; ++i) {
             map[i] = i;
             line("  " + T2 + " r" + std::to_string(i) + " = " + ld + "[lo | " + hex64(sw.ld_roff[i]) + "];");
@@ -260,8 +306,19 @@ struct Gen {
                 line("    const unsigned l0 = " + thread_bits(cur->tlocal) + ";");
                 line("    const unsigned s0 = l0 ^ (((l0 >> " + std::to_string(sh) + ") ^ (l0 >> " + std::to_string(2 * sh) +
                      ") ^ (l0 >> " + std::to_string(3 * sh) + ")) & " + std::to_string((1 << sh) - 1) + "u);");
+                std::string s0v = "s0";
+                if (frame_pending()) {
+                    // register i holds the amplitude of position i ^ f: write it there (the swizzle
+                    // is linear, so the frame adds a per-work-item XOR to every address)
+                    frame_offset("unsigned", "fx", [&](int r) {
+                        return std::to_string(swz_host(cur->roff[1 << r], elem)) + "u";
+                    });
+                    line("    const unsigned s0f = s0 ^ fx;");
+                    s0v = "s0f";
+                    std::fill(xf.begin(), xf.end(), 0u);
+                }
                 for (int i = 0; i < R; ++i)
-                    line("    lds[s0 ^ " + std::to_string(swz_host(cur->roff[i], elem)) + "u] = " + var(i) + ";");
+                    line("    lds[" + s0v + " ^ " + std::to_string(swz_host(cur->roff[i], elem)) + "u] = " + var(i) + ";");
                 line("    __syncthreads();");
                 line("    const unsigned l1 = " + thread_bits(P.tlocal) + ";");
                 line("    const unsigned s1 = l1 ^ (((l1 >> " + std::to_string(sh) + ") ^ (l1 >> " + std::to_string(2 * sh) +
@@ -282,7 +339,15 @@ struct Gen {
         } else {
             line("  const unsigned long long so = base | ts;");
         }
-        for (int i = 0; i < R; ++i) line("  " + st + "[so | " + hex64(sw.st_roff[i]) + "] = " + var(i) + ";");
+        if (frame_pending()) {
+            // register i holds the amplitude of register position i ^ f: store it there
+            frame_offset("unsigned long long", "fo", [&](int r) { return hex64(sw.st_roff[1 << r]); });
+            line("  const unsigned long long sf = so ^ fo;");
+            for (int i = 0; i < R; ++i) line("  " + st + "[sf ^ " + hex64(sw.st_roff[i]) + "] = " + var(i) + ";");
+            std::fill(xf.begin(), xf.end(), 0u);
+        } else {
+            for (int i = 0; i < R; ++i) line("  " + st + "[so | " + hex64(sw.st_roff[i]) + "] = " + var(i) + ";");
+        }
         if (persistent) {
             line("  if (!more) break;");
             std::string l = " ";

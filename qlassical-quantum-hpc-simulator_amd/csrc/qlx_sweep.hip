@@ -119,6 +119,9 @@ struct PGate {
     int nq;             // operand count
     int q[5];
     bool sweepable;
+    // a CX of a circuit-specialised sweep: its control may sit on a work-item bit (OP_CXT), so a
+    // register phase can run it when its target alone is in the registers
+    bool ctl_free = false;
 };
 
 static bool is_sweepable(const qlx_gate_instr& g, const qlx_matrix_entry* meta) {
@@ -145,9 +148,10 @@ static int operand_count(const qlx_gate_instr& g) {
 
 // Marks (in `run`) the pending gates one sweep over the qubit set `in_set` can apply: a
 // gate runs when it is sweepable, its qubits are in the set and no earlier pending gate on
-// any of its qubits stays behind. Scanning in program order, a gate that stays behind
-// blocks its qubits for every later gate (each later gate on such a qubit depends on it),
-// so the scan ends once every qubit of the set is blocked. Returns the count.
+// any of its qubits stays behind (a ctl_free CX: its target is in the set and neither of its
+// qubits is held back). Scanning in program order, a gate that stays behind blocks its
+// qubits for every later gate (each later gate on such a qubit depends on it), so the scan
+// ends once every qubit of the set is blocked. Returns the count.
 static int count_runnable(const std::vector<PGate>& gates, const std::vector<int>& pending,
                           const std::vector<char>& done, const std::vector<char>& in_set,
                           std::vector<char>& run, const std::vector<std::vector<int>>& preds) {
@@ -163,7 +167,10 @@ static int count_runnable(const std::vector<PGate>& gates, const std::vector<int
         const int gi = pending[k];
         const PGate& g = gates[gi];
         bool ok = g.sweepable;
-        for (int j = 0; ok && j < g.nq; ++j) ok = in_set[g.q[j]] && !blocked[g.q[j]];
+        if (g.ctl_free)
+            ok = ok && in_set[g.q[1]] && !blocked[g.q[1]] && !blocked[g.q[0]];
+        else
+            for (int j = 0; ok && j < g.nq; ++j) ok = in_set[g.q[j]] && !blocked[g.q[j]];
         run[gi] = ok;
         if (ok) {
             ++cnt;
@@ -473,6 +480,10 @@ static void plan_phases(PlanSweep& sw, const std::vector<PGate>& lg, const Geome
                 op.src = lg[gi].src;
                 op.l0 = loc2st[lg[gi].q[0]] - LT;
                 op.l1 = lg[gi].nq > 1 ? loc2st[lg[gi].q[1]] - LT : -1;
+                if (lg[gi].ctl_free && op.l0 < 0) {
+                    op.tctl = loc2st[lg[gi].q[0]];  // the control is a work-item bit
+                    op.l0 = -1;
+                }
                 op.kind = lg[gi].type;  // instruction type; resolved to an OpKind later
                 phase.ops.push_back(op);
                 done[gi] = 1;
@@ -632,6 +643,7 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
     if ((int)nq < geo.w) fail(QLX_EINVAL, "window larger than the state");
     plan->geo = geo;
     const int W = geo.w;
+    const bool specialised = (opts.specialize & QLX_SPECIALIZE) != 0;
 
     // gate list (BARRIER and no-op GLOBAL_SWAP dropped)
     std::vector<PGate> gates;
@@ -717,6 +729,7 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
         for (int gi : taken) {
             PGate l = gates[gi];
             for (int k = 0; k < l.nq; ++k) l.q[k] = loc[l.q[k]];
+            l.ctl_free = specialised && l.type == QLX_CX;
             lg.push_back(l);
         }
         PlanSweep sw;
@@ -782,6 +795,14 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
                 o.coef = (uint32_t)plan->coef.size();
                 uint8_t kind = 0;
                 fill_coef(plan->coef, instrs[op.src], pool, meta, kind);
+                if (op.tctl >= 0) {
+                    // CX with its control on work-item bit tctl (circuit-specialised sweeps only)
+                    if (kind != OP_CX || !factored || op.tctl >= geo.logt || op.l1 < 0 || op.l1 >= geo.rb)
+                        fail(QLX_EHIP, "sweep planner: bad work-item-controlled CX");
+                    o.key = (uint32_t)((OP_CXT << 8) | (op.tctl << 4) | op.l1);
+                    plan->dop.push_back(o);
+                    continue;
+                }
                 const bool two = kind == OP_CX || kind == OP_SWAP || kind == OP_U4;
                 if (two && (op.l1 < 0 || op.l0 == op.l1)) fail(QLX_EHIP, "sweep planner: bad two-qubit operands");
                 if (op.l0 < 0 || op.l0 >= geo.rb || (two && op.l1 >= geo.rb))

@@ -26,9 +26,18 @@ constexpr int kMaxR = 32;
 // the pending factors, at the end of each sweep or as soon as the product falls below 2^-30
 // (each factor is at least 1/sqrt 2 in magnitude, so registers stay within 2^30 of the
 // amplitudes' magnitude, also in FP32).
+//
+// Circuit-specialised sweeps also take OP_CXT: a CX whose control is work-item bit b0 of the
+// phase's layout (a window qubit outside the registers) and whose target is register bit b1.
+// The generator applies no instruction for it: it records a pending X on the target register
+// bit, conditioned per work-item on the parity of selected work-item bits (the control), and
+// carries it through later ops (a CX passes it on from its control to its target, a SWAP moves
+// it, RX commutes with it). The pending X is a per-work-item permutation of the registers, so
+// the next LDS re-layout or the final store takes it up in its addresses; only an op that does
+// not commute with it (H, dense matrices) applies it first, as conditional register swaps.
 enum OpKind : uint8_t {
     OP_U2 = 1, OP_RX = 2, OP_H = 3, OP_CX = 4, OP_SWAP = 5, OP_U4 = 6,
-    OP_RXT = 7, OP_RXU = 8, OP_HB = 9, OP_SCALE = 10
+    OP_RXT = 7, OP_RXU = 8, OP_HB = 9, OP_SCALE = 10, OP_CXT = 11
 };
 
 // 32-bit fields only: the descriptor is read with scalar loads (a 16-bit field would be
@@ -402,7 +411,8 @@ struct Geometry {
 struct PlanOp {
     int src;
     uint8_t kind;
-    int l0, l1;  // register bits (l1 = -1 for 1q)
+    int l0, l1;     // register bits (l1 = -1 for 1q)
+    int tctl = -1;  // a CX whose control is this work-item bit (OP_CXT; l0 unused)
 };
 
 struct PlanPhase {

@@ -42,11 +42,12 @@ def test_source_structure(dtype, regs):
         src = qlx.sweep_source(prog, s, dtype)
         assert re.search(rf"qlx_sweep_spec_[0-9a-f]{{6}}_{s}\(", src)
         assert len(re.findall(r"= amp\[lo \| 0x[0-9a-f]+ull\];", src)) == regs
-        assert len(re.findall(r"amp\[so \| 0x[0-9a-f]+ull\] = r\d+;", src)) == regs
+        # stores: through `so`, or `sf` = so ^ (the pending work-item-controlled X of the registers)
+        assert len(re.findall(r"amp\[(?:so \||sf \^) 0x[0-9a-f]+ull\] = r\d+;", src)) == regs
         # gate coefficients are literals; no descriptor arrays are read
         assert "coef[" not in src and "ops[" not in src
         # every remap writes and reads all registers through LDS
-        assert src.count("lds[s0 ^") == regs * src.count("__syncthreads();") // 2
+        assert src.count("lds[s0 ^") + src.count("lds[s0f ^") == regs * src.count("__syncthreads();") // 2
 
 
 def test_cx_and_swap_are_register_renaming():
@@ -75,6 +76,42 @@ def test_sources_compile_for_gfx950():
         for s in range(min(2, info["sweeps"])):
             rc, log = compile_ok(qlx.sweep_source(prog, s, dtype))
             assert rc == 0, log[-2000:]
+
+
+def test_cx_controlled_by_a_work_item_bit_costs_no_instructions():
+    """A CX whose control is outside the registers (a work-item bit of the phase) is recorded as
+    a pending X on its target register bit and taken up by the next LDS re-layout's or the final
+    store's addresses: no arithmetic, no register swaps; the kernel compiles."""
+    n = 12
+    c = qlx.Circuit(n)
+    for q in range(n - 1):
+        c.cx(q, q + 1)                  # a chain: no 4-qubit register set holds every control
+    c.rx(7, 0.3).cx(6, 7).rx(7, 1.1)    # RX on a target with a pending X: applied as it is
+    prog = c.release()
+    order, sweep_of, info = qlx.plan_windowed(prog, "c128", qlx.exec_opts("windowed", specialize=True))
+    assert info["sweeps"] == 1 and sorted(order) == list(range(len(prog)))
+    src = qlx.sweep_source(prog, 0, "c128")
+    assert "__builtin_popcount(t &" in src              # a parity of work-item bits
+    assert "fc ?" not in src                            # never applied as register swaps
+    assert src.count("__builtin_fma(") == 2 * 8 * 4     # the two RX (four FMAs per pair), nothing else
+    if os.path.exists(HIPRTC):
+        rc, log = compile_ok(src)
+        assert rc == 0, log[-2000:]
+
+
+def test_pending_x_is_applied_before_an_op_it_does_not_commute_with():
+    """H after a work-item-controlled CX on the same target: the pending X is applied first, as
+    conditional swaps of the register pairs."""
+    n = 12
+    c = qlx.Circuit(n)
+    for q in range(n - 1):
+        c.cx(q, q + 1)
+    for q in range(1, n):
+        c.h(q)
+    prog = c.release()
+    srcs = [qlx.sweep_source(prog, s, "c128") for s in range(qlx.plan_windowed(
+        prog, "c128", qlx.exec_opts("windowed", specialize=True))[2]["sweeps"])]
+    assert any("fc ? a1 : a0" in s for s in srcs)
 
 
 def test_h_and_rx_take_their_factor_out():
@@ -138,9 +175,14 @@ def test_relabelled_sweeps_address_every_amplitude_once(prog_name):
         for base, thr, ptr, idx in (("base", "tl", "src", "lo"), ("sbase", "ts", "dst", "so")):
             chunk = _bits(re.search(rf"const unsigned long long {base} = ([^;]+);", src).group(1))
             work = _bits(re.search(rf"const unsigned long long {thr} = ([^;]+);", src).group(1))
-            offs = [int(h, 16) for h in re.findall(rf"{ptr}\[{idx} \| (0x[0-9a-f]+)ull\]", src)]
+            pat = rf"{ptr}\[{idx} \| (0x[0-9a-f]+)ull\]" if idx == "lo" else rf"{ptr}\[(?:so \||sf \^) (0x[0-9a-f]+)ull\]"
+            offs = [int(h, 16) for h in re.findall(pat, src)]
             assert len(offs) == 16 and len(set(offs)) == 16
             regs = [b for b in range(n) if any((o_ >> b) & 1 for o_ in offs)]
+            if idx == "so":
+                # the per-work-item XOR of a pending X moves a store only among the register offsets
+                for h in re.findall(r"fo \^= (0x[0-9a-f]+)ull;", src):
+                    assert int(h, 16) in offs and int(h, 16) != 0
             # register offsets are all combinations of 4 address bits
             assert len(regs) == 4 and sorted(offs) == sorted(
                 sum(1 << regs[k] for k in range(4) if (i >> k) & 1) for i in range(16))
