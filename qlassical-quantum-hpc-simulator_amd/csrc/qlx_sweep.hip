@@ -99,6 +99,7 @@ static bool geometry_built(int dtype, int w, int rb, bool specialised) {
 
 static Geometry geometry_for(int dtype, uint32_t nq, const qlx_exec_opts& o) {
     Geometry g{};
+    g.elem = (int)amp_size(dtype);
     g.w = o.window_qubits ? (int)o.window_qubits : (dtype == QLX_C128 ? 12 : 13);
     g.rb = o.register_qubits ? (int)o.register_qubits : (dtype == QLX_C128 ? 4 : 5);
     if (!geometry_built(dtype, g.w, g.rb, (o.specialize & QLX_SPECIALIZE) != 0))
@@ -284,6 +285,15 @@ static double plain_passes(uint8_t type) {
     }
 }
 
+// Vector-unit work of a gate inside a sweep, in units of one one-qubit rotation or H over the
+// chunk (CX and SWAP: a quarter, their cost being the register phases they add).
+static double gate_work(const PGate& g) {
+    switch (g.type) {
+        case QLX_CX: case QLX_SWAP: case QLX_GLOBAL_SWAP: return 0.25;
+        default: return g.nq >= 2 ? 4.0 : 1.0;
+    }
+}
+
 // The planner's walk over the gate DAG: plain-only gates run as soon as they are ready;
 // otherwise the next sweep takes every pending gate its window can run.
 struct Walk {
@@ -333,17 +343,23 @@ struct Walk {
         return -1;
     }
     // the gates one sweep over window S runs, in program order; marks them done
+    // with a work budget (> 0): the runnable gates in program order until their work reaches
+    // it (a program-order prefix of runnable gates holds every predecessor it needs)
+    double budget = 0.0;
     std::vector<int> take(const std::vector<int>& S, int nq) {
         std::vector<char> sset(nq, 0);
         for (int q : S) sset[q] = 1;
         std::vector<char> run(gates->size(), 0);
         count_runnable(*gates, pending, done, sset, run, *preds);
         std::vector<int> out;
+        double work = 0.0;
         for (int gi : pending)
             if (run[gi]) {
+                if (budget > 0.0 && work >= budget) break;
                 out.push_back(gi);
                 done[gi] = 1;
                 ++n_done;
+                work += gate_work((*gates)[gi]);
             }
         prune();
         return out;
@@ -592,13 +608,14 @@ static std::vector<std::vector<int>> window_candidates(const Walk& w, int nq, co
 // Returns the windows of the first walk to finish.
 static std::vector<std::vector<int>> beam_windows(const std::vector<PGate>& gates,
                                                   const std::vector<std::vector<int>>& preds, int nq,
-                                                  const Geometry& geo, int width) {
+                                                  const Geometry& geo, int width, double budget = 0.0) {
     struct Node {
         Walk w;
         std::vector<std::vector<int>> wins;
     };
     std::vector<Node> beam;
     beam.push_back(Node{Walk(gates, preds), {}});
+    beam[0].w.budget = budget;
     beam[0].w.run_plain();
     for (size_t step = 0; step <= gates.size(); ++step) {
         for (const Node& nd : beam)
@@ -627,6 +644,78 @@ static std::vector<std::vector<int>> beam_windows(const std::vector<PGate>& gate
         if (beam.empty()) fail(QLX_EHIP, "sweep planner: beam search made no progress");
     }
     fail(QLX_EHIP, "sweep planner: beam search did not finish");
+}
+
+// Arithmetic a complex128 sweep may carry before it outlasts its HBM pass, in vector-unit work
+// units (one unit: a one-qubit rotation or H on every amplitude of the chunk; a CX counts a
+// quarter, its cost being the register phases it adds). Measured on the 30-qubit layered
+// circuit (12-qubit window, 16 registers): sweeps of up to ~85 units run at the streaming rate.
+// A complex64 pass moves half the bytes in the same instructions: half the budget.
+constexpr double kBalanceWork = 85.0;
+// The walk's cap on one sweep's arithmetic (same units): a sweep stops taking runnable gates,
+// in program order, once it holds this much, leaving later layers to later sweeps. Caps of
+// 90-130 units measured slower on the 30-qubit layered circuit (102.4-103.2 ms against
+// 102.2 ms: the window planner then moves the heavy work to other sweeps), so no cap is set.
+constexpr double kTakeWork = 1e9;
+
+// Evens out the arithmetic of the circuit-specialised sweeps. The window planner gives each
+// sweep every gate its window can run, so sweeps on the edge of a circuit (whose light cone is
+// open on one side) take far more gates than the rest and outlast their HBM pass, while
+// others are light. A gate moves from an over-budget sweep to the earliest later sweep whose
+// window holds its qubits and that precedes none of its successors (each successor runs in a
+// later step, or in that sweep after it, in program order), as long as the receiving sweep
+// stays within the budget or ends up with less work than the giving one. Gates are tried from the end of the sweep backwards, so a gate's
+// successors in the same sweep have moved before it is considered. Plain steps in between
+// keep their places; the moved gate still runs after all its predecessors.
+template <class IsSweep, class Window, class Taken>
+static void balance_sweeps(const std::vector<PGate>& gates, const std::vector<std::vector<int>>& preds, size_t nsteps,
+                           IsSweep is_sweep, Window window, Taken taken, std::vector<int>& step_of_gate, int nq,
+                           double budget) {
+    const int G = (int)gates.size();
+    std::vector<std::vector<int>> succs(G);
+    for (int i = 0; i < G; ++i)
+        for (int p : preds[i]) succs[p].push_back(i);
+    std::vector<double> work(nsteps, 0.0);
+    std::vector<std::vector<char>> inwin(nsteps);
+    for (size_t k = 0; k < nsteps; ++k) {
+        if (!is_sweep(k)) continue;
+        for (int gi : taken(k)) work[k] += gate_work(gates[gi]);
+        inwin[k].assign(nq, 0);
+        for (int q : window(k)) inwin[k][q] = 1;
+    }
+    for (size_t k = 0; k < nsteps; ++k) {
+        if (!is_sweep(k) || work[k] <= budget) continue;
+        std::vector<int>& src = taken(k);
+        for (int idx = (int)src.size() - 1; idx >= 0 && work[k] > budget; --idx) {
+            const int gi = src[idx];
+            // the earliest step any successor runs in
+            size_t limit = nsteps;
+            bool held = false;
+            for (int sc : succs[gi]) {
+                const int ss = step_of_gate[sc];
+                if (ss == (int)k) held = true;  // a successor stayed in this sweep
+                limit = std::min(limit, (size_t)ss);
+            }
+            if (held) continue;
+            for (size_t k2 = k + 1; k2 <= limit && k2 < nsteps; ++k2) {
+                if (!is_sweep(k2)) continue;
+                bool fits = true;
+                for (int j = 0; j < gates[gi].nq && fits; ++j) fits = inwin[k2][gates[gi].q[j]] != 0;
+                if (!fits) continue;
+                // the receiving sweep stays within the budget, or at least below the giving one
+                const double w2 = work[k2] + gate_work(gates[gi]);
+                if (w2 > budget && w2 > work[k] - gate_work(gates[gi])) continue;
+                // insert in program order (gate indices follow program order)
+                std::vector<int>& dst = taken(k2);
+                dst.insert(std::lower_bound(dst.begin(), dst.end(), gi), gi);
+                work[k2] += gate_work(gates[gi]);
+                work[k] -= gate_work(gates[gi]);
+                step_of_gate[gi] = (int)k2;
+                src.erase(src.begin() + idx);
+                break;
+            }
+        }
+    }
 }
 
 enum class Chooser { Greedy, Wide, Given };
@@ -670,9 +759,11 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
                 last[q] = i;
             }
     }
+    // circuit-specialised sweeps: each takes at most about kTakeWork units of arithmetic
+    const double take_budget = specialised && thorough ? kTakeWork * geo.elem / 16.0 : 0.0;
     std::vector<std::vector<int>> beam;
     if (chooser == Chooser::Given && !given) {
-        beam = beam_windows(gates, preds, (int)nq, geo, 16);
+        beam = beam_windows(gates, preds, (int)nq, geo, 16, take_budget);
         given = &beam;
     }
     auto plain_step = [&](int gi) {
@@ -682,14 +773,23 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
         plan->passes += plain_passes(gates[gi].type);
     };
 
+    // 1. the walk: plain steps and sweeps (window + the gates it takes, in program order)
+    struct RawStep {
+        bool sweep;
+        int gate;                // plain step: the gate
+        std::vector<int> S;      // sweep: window qubits
+        std::vector<int> taken;  // sweep: gates, program order
+    };
+    std::vector<RawStep> raw;
     Walk walk(gates, preds);
+    walk.budget = take_budget;
     size_t next_window = 0;
     while (true) {
-        // 1. plain-only gates whose predecessors are done run as their own kernels
-        for (int gi : walk.run_plain()) plain_step(gi);
+        // plain-only gates whose predecessors are done run as their own kernels
+        for (int gi : walk.run_plain()) raw.push_back(RawStep{false, gi, {}, {}});
         if (walk.pending.empty()) break;
 
-        // 2. the window
+        // the window
         const int seed_gate = walk.seed_gate();
         if (seed_gate < 0) fail(QLX_EHIP, "sweep planner: no runnable gate");
         std::vector<int> S;
@@ -712,17 +812,45 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
         }
         if ((int)S.size() != W) fail(QLX_EHIP, "sweep planner: window smaller than W");
 
-        // 3. the sweep's gates, in program order
+        // the sweep's gates, in program order
         const std::vector<int> taken = walk.take(S, (int)nq);
         if (taken.empty()) fail(QLX_EHIP, "sweep planner: empty sweep");
         if (thorough) {
             double cost = 0.0;
             for (int gi : taken) cost += plain_passes(gates[gi].type);
             if (cost < 1.0) {
-                for (int gi : taken) plain_step(gi);
+                for (int gi : taken) raw.push_back(RawStep{false, gi, {}, {}});
                 continue;
             }
         }
+        raw.push_back(RawStep{true, -1, S, taken});
+    }
+
+    // 2. circuit-specialised sweeps: even out the arithmetic (see balance_sweeps)
+    if (specialised && thorough) {
+        std::vector<std::vector<int>> sets;
+        std::vector<std::vector<int>*> lists;
+        std::vector<int> step_of_gate(G, -1);
+        for (size_t k = 0; k < raw.size(); ++k) {
+            if (raw[k].sweep)
+                for (int gi : raw[k].taken) step_of_gate[gi] = (int)k;
+            else
+                step_of_gate[raw[k].gate] = (int)k;
+        }
+        balance_sweeps(gates, preds, raw.size(), [&](size_t k) { return raw[k].sweep; },
+                       [&](size_t k) -> const std::vector<int>& { return raw[k].S; },
+                       [&](size_t k) -> std::vector<int>& { return raw[k].taken; }, step_of_gate, (int)nq,
+                       kBalanceWork * geo.elem / 16.0);
+    }
+
+    // 3. the steps: plain kernels, and sweeps with their register phases
+    for (RawStep& st : raw) {
+        if (!st.sweep) {
+            plain_step(st.gate);
+            continue;
+        }
+        const std::vector<int>& S = st.S;
+        const std::vector<int>& taken = st.taken;
         std::vector<int> loc(nq, -1);
         for (int j = 0; j < W; ++j) loc[S[j]] = j;
         std::vector<PGate> lg;

@@ -406,6 +406,7 @@ __device__ __forceinline__ void run_chunk(A* __restrict__ amp, const DevSweep* _
 // ---------------------------------------------------------------------------------
 struct Geometry {
     int logt, rb, w, c;
+    int elem = 16;  // bytes per amplitude
 };
 
 struct PlanOp {
