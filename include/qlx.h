@@ -300,6 +300,18 @@ int qlx_timing_read(qlx_state* st, qlx_kernel_timing* out, size_t cap, size_t* n
  *      GPU branch :474-476 calls the empty stub :866) ----------------------------
  * Same pass pipeline as transpile_cpu_openmp (middleware.hpp:488-524): DAG build,
  * pass_gate_fusion (:526-753), pass_qubit_windowing (:759-860), DAG release. */
+
+/* Windowing cost model (TranspilerConfig::cost_model_type, reference middleware.hpp:352-356). */
+#define QLX_COST_THRESHOLD 0  /* ThresholdCostModel(k_safe, numa_penalty, swap_penalty) (:63-119) */
+#define QLX_COST_GREEDY    1  /* GreedyCostModel(k_safe): penalties 1 and 0 (:121-131) */
+#define QLX_COST_CUSTOM    2  /* the caller's model, through custom_evaluate */
+/* WindowingCostModel::evaluate_swap (reference middleware.hpp:44-61). The three arrays hold
+ * num_qubits entries each. Returns 1 with the (active, victim) logical qubits of the swap to
+ * inject, 0 for none, a negative value to abort the transpilation (QLX_EINVAL). */
+typedef int (*qlx_swap_eval_fn)(void* ctx, const uint32_t* window_frequencies, const uint16_t* log_to_phys,
+                                const uint16_t* phys_to_log, uint32_t num_qubits, uint16_t* active,
+                                uint16_t* victim);
+
 typedef struct qlx_transpile_opts {
     uint32_t enable_gate_fusion;  /* TranspilerConfig::enable_gate_fusion */
     uint32_t max_fusion_width;    /* 1..5; 0 = 4 (the reference's MAX_GATE_ARITY, middleware.hpp:137) */
@@ -312,7 +324,12 @@ typedef struct qlx_transpile_opts {
     int32_t  dtype;               /* precision the window is sized for (QLX_C128 / QLX_C64) */
     uint32_t build_on_device;     /* form the fused-block matrices in FP64 on the current GPU
                                      (the host decides the blocks); 0 = on the host */
-    uint32_t reserved[6];
+    uint32_t cost_model;          /* QLX_COST_*; with QLX_COST_CUSTOM, k_safe is the model's
+                                     get_k_safe() (0 = qlx_gpu_safe_qubit_limit) */
+    uint32_t reserved;
+    qlx_swap_eval_fn custom_evaluate;  /* QLX_COST_CUSTOM: the model (NULL -> QLX_EINVAL, as the
+                                          reference's transpile throws invalid_argument, :510-511) */
+    void*    custom_ctx;          /* passed to custom_evaluate */
 } qlx_transpile_opts;
 typedef struct qlx_program qlx_program;
 int qlx_transpile(uint32_t num_qubits, const qlx_gate_instr* instrs, size_t n,

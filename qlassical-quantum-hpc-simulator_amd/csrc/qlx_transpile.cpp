@@ -372,9 +372,29 @@ void gate_fusion(Dag& dag, Pool& pool, int max_width, bool rotations, DeviceFusi
 }
 
 // ---- pass_qubit_windowing (middleware.hpp:759-860) + ThresholdCostModel (:63-119) ---------
+// (GreedyCostModel, :121-131, is the threshold model with penalties 1 and 0; a custom model
+// is the caller's evaluate_swap through a C callback.)
 struct Threshold {
     uint32_t k_safe, numa_penalty, swap_penalty;
-    bool evaluate(const std::vector<uint32_t>& freq, const std::vector<uint16_t>& l2p, int& cand, int& victim) const {
+    qlx_swap_eval_fn custom = nullptr;
+    void* ctx = nullptr;
+    bool evaluate(const std::vector<uint32_t>& freq, const std::vector<uint16_t>& l2p,
+                  const std::vector<uint16_t>& p2l, int& cand, int& victim) const {
+        if (custom) {
+            uint16_t a = 0, v = 0;
+            const int rc = custom(ctx, freq.data(), l2p.data(), p2l.data(), (uint32_t)freq.size(), &a, &v);
+            if (rc < 0) fail(QLX_EINVAL, "custom windowing cost model failed");
+            if (rc == 0) return false;
+            if (a >= freq.size() || v >= freq.size() || a == v)
+                fail(QLX_EINVAL, "custom windowing cost model proposed an invalid swap (" + std::to_string(a) + ", " +
+                                     std::to_string(v) + ")");
+            cand = a;
+            victim = v;
+            return true;
+        }
+        return threshold(freq, l2p, cand, victim);
+    }
+    bool threshold(const std::vector<uint32_t>& freq, const std::vector<uint16_t>& l2p, int& cand, int& victim) const {
         cand = -1;
         uint32_t best = 0;
         for (size_t q = 0; q < freq.size(); ++q) {
@@ -441,7 +461,12 @@ void qubit_windowing(Dag& dag, const Threshold& cm, uint32_t nq, bool restore) {
             ++j;
         }
         int cand, victim;
-        while (cm.evaluate(freq, l2p, cand, victim)) {
+        uint32_t swaps = 0;
+        while (cm.evaluate(freq, l2p, p2l, cand, victim)) {
+            // a model that keeps proposing swaps for one window does not converge (every useful
+            // swap moves one more active qubit into the safe zone)
+            if (cm.custom && ++swaps > nq)
+                fail(QLX_EINVAL, "custom windowing cost model proposed more than num_qubits swaps for one window");
             const uint16_t pc = l2p[cand], pv = l2p[victim];
             gswap(pc, pv);
             l2p[cand] = pv;
@@ -548,6 +573,17 @@ int qlx_transpile(uint32_t nq, const qlx_gate_instr* instrs, size_t n, const dou
                 if (qlx_gpu_safe_qubit_limit(-1, o.dtype == QLX_C64 ? QLX_C64 : QLX_C128, &ks) != QLX_OK) ks = 12;
             }
             Threshold cm{ks, o.numa_penalty ? o.numa_penalty : 10u, o.swap_penalty};
+            if (o.cost_model == QLX_COST_GREEDY) {
+                cm.numa_penalty = 1;
+                cm.swap_penalty = 0;
+            } else if (o.cost_model == QLX_COST_CUSTOM) {
+                if (!o.custom_evaluate)
+                    fail(QLX_EINVAL, "TranspilerConfig: CostModelType::CUSTOM specified but custom_cost_model is null.");
+                cm.custom = o.custom_evaluate;
+                cm.ctx = o.custom_ctx;
+            } else if (o.cost_model != QLX_COST_THRESHOLD) {
+                fail(QLX_EINVAL, "unknown windowing cost model");
+            }
             qubit_windowing(dag, cm, nq, o.restore_layout != 0);
         }
         prog->instrs = release_dag(std::move(dag));

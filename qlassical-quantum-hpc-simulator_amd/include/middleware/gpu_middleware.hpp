@@ -13,9 +13,12 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <memory>
 #include <optional>
+#include <span>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include <backend/gpu_state_vector.hpp>
@@ -24,6 +27,45 @@
 #include "qlx.h"
 
 namespace qlassical::gpu {
+
+// The windowing pass's cost model: the interface of the reference's WindowingCostModel
+// (include/middleware/middleware.hpp:44-61), evaluate_swap returning the (active, victim)
+// logical qubits of the next GLOBAL_SWAP for a window, or nullopt; get_k_safe the window's
+// qubit limit. In the reference tree a reference model is passed through
+// cost_model_adapter() below.
+class WindowingCostModel {
+public:
+    virtual ~WindowingCostModel() = default;
+    virtual std::optional<std::pair<uint16_t, uint16_t>> evaluate_swap(
+        std::span<const uint32_t> window_frequencies, std::span<const uint16_t> log_to_phys,
+        std::span<const uint16_t> phys_to_log) const = 0;
+    virtual uint16_t get_k_safe() const = 0;
+};
+
+// Wraps any object with the reference's evaluate_swap / get_k_safe members (for example the
+// reference's own ThresholdCostModel, GreedyCostModel or a user subclass of its
+// WindowingCostModel) as a gpu::WindowingCostModel.
+template <class Model>
+class CostModelAdapter final : public WindowingCostModel {
+public:
+    explicit CostModelAdapter(std::shared_ptr<const Model> m) : m_(std::move(m)) {}
+    std::optional<std::pair<uint16_t, uint16_t>> evaluate_swap(std::span<const uint32_t> f,
+                                                               std::span<const uint16_t> l2p,
+                                                               std::span<const uint16_t> p2l) const override {
+        return m_->evaluate_swap(f, l2p, p2l);
+    }
+    uint16_t get_k_safe() const override { return m_->get_k_safe(); }
+
+private:
+    std::shared_ptr<const Model> m_;
+};
+template <class Model>
+std::shared_ptr<WindowingCostModel> cost_model_adapter(std::shared_ptr<const Model> m) {
+    return std::make_shared<CostModelAdapter<Model>>(std::move(m));
+}
+
+// TranspilerConfig::cost_model_type (reference middleware.hpp:352-356).
+enum class CostModelType : uint8_t { THRESHOLD = 0, GREEDY = 1, CUSTOM = 2 };
 
 struct GpuTranspilerConfig {
     bool enable_gate_fusion = true;     // TranspilerConfig::enable_gate_fusion
@@ -35,7 +77,33 @@ struct GpuTranspilerConfig {
     uint32_t swap_penalty = 1;          // TranspilerConfig::swap_penalty
     bool restore_layout = true;         // append GLOBAL_SWAPs that restore logical order
     backend::Precision precision = backend::Precision::C128;
+    // windowing cost model (TranspilerConfig::cost_model_type / custom_cost_model,
+    // middleware.hpp:366-382): a custom model is used whenever it is set
+    CostModelType cost_model_type = CostModelType::THRESHOLD;
+    std::shared_ptr<WindowingCostModel> custom_cost_model = nullptr;
+    void set_cost_model(std::shared_ptr<WindowingCostModel> m) {
+        custom_cost_model = std::move(m);
+        cost_model_type = CostModelType::CUSTOM;
+    }
 };
+
+namespace detail {
+// qlx_swap_eval_fn over a gpu::WindowingCostModel; an exception aborts the pass (-1)
+inline int evaluate_swap_cb(void* ctx, const uint32_t* f, const uint16_t* l2p, const uint16_t* p2l, uint32_t n,
+                            uint16_t* active, uint16_t* victim) {
+    try {
+        const auto* m = static_cast<const WindowingCostModel*>(ctx);
+        const auto r = m->evaluate_swap(std::span<const uint32_t>(f, n), std::span<const uint16_t>(l2p, n),
+                                        std::span<const uint16_t>(p2l, n));
+        if (!r) return 0;
+        *active = r->first;
+        *victim = r->second;
+        return 1;
+    } catch (...) {
+        return -1;
+    }
+}
+}  // namespace detail
 
 class GPUTopology {
 public:
@@ -75,6 +143,14 @@ inline void transpile_gpu(IRModule& module, const GpuTranspilerConfig& cfg = {})
     o.swap_penalty = cfg.swap_penalty;
     o.restore_layout = cfg.restore_layout;
     o.dtype = static_cast<int>(cfg.precision);
+    if (cfg.custom_cost_model) {
+        o.cost_model = QLX_COST_CUSTOM;
+        o.custom_evaluate = &detail::evaluate_swap_cb;
+        o.custom_ctx = const_cast<WindowingCostModel*>(cfg.custom_cost_model.get());
+        o.k_safe = cfg.custom_cost_model->get_k_safe();  // the pass's window limit is the model's (:771)
+    } else {
+        o.cost_model = static_cast<uint32_t>(cfg.cost_model_type);
+    }
     qlx_program* p = nullptr;
     const auto& pool = module.unitary_pool;
     backend::qlx_throw_on_error(qlx_transpile(

@@ -129,8 +129,19 @@ class TranspileOpts(C.Structure):
         ("restore_layout", C.c_uint32),
         ("dtype", C.c_int32),
         ("build_on_device", C.c_uint32),
-        ("reserved", C.c_uint32 * 6),
+        ("cost_model", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("custom_evaluate", C.c_void_p),
+        ("custom_ctx", C.c_void_p),
     ]
+
+
+QLX_COST_THRESHOLD = 0
+QLX_COST_GREEDY = 1
+QLX_COST_CUSTOM = 2
+# WindowingCostModel::evaluate_swap as a C callback (qlx_swap_eval_fn)
+SWAP_EVAL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint16),
+                           C.POINTER(C.c_uint16), C.c_uint32, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16))
 
 
 class BatchInfo(C.Structure):

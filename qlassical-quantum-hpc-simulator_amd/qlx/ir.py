@@ -256,12 +256,36 @@ def clustered(num_qubits: int, cluster_size: int = 6, depth: int = 15) -> Progra
 
 def transpile_gpu(prog: Program, fusion: bool = True, max_width: int = 4, rotations: bool = True,
                   windowing: bool = False, k_safe: int = 0, numa_penalty: int = 10, swap_penalty: int = 1,
-                  restore: bool = True, dtype: int = 0, on_device: bool = False) -> Program:
+                  restore: bool = True, dtype: int = 0, on_device: bool = False, cost_model="threshold") -> Program:
     """Middleware::transpile_gpu through the library (qlx_transpile). on_device: form the
-    fused-block matrices in FP64 on the current GPU (the blocks are chosen on the host)."""
+    fused-block matrices in FP64 on the current GPU (the blocks are chosen on the host).
+    cost_model: "threshold" (ThresholdCostModel), "greedy" (GreedyCostModel), "custom" (no model:
+    rejected, as the reference rejects CUSTOM without an instance), or a callable
+    evaluate_swap(window_frequencies, log_to_phys, phys_to_log) -> (active, victim) | None,
+    WindowingCostModel::evaluate_swap (middleware.hpp:44-61); its k_safe is `k_safe`."""
     L = _abi.lib()
     o = _abi.TranspileOpts()
     o.build_on_device = int(on_device)
+    cb = None
+    if callable(cost_model):
+        model = cost_model
+
+        def _eval(_ctx, freq, l2p, p2l, nq, active, victim):
+            try:
+                got = model([freq[i] for i in range(nq)], [l2p[i] for i in range(nq)],
+                             [p2l[i] for i in range(nq)])
+            except Exception:  # an exception in the model aborts the pass
+                return -1
+            if got is None:
+                return 0
+            active[0], victim[0] = int(got[0]), int(got[1])
+            return 1
+        cb = _abi.SWAP_EVAL_FN(_eval)
+        o.cost_model = _abi.QLX_COST_CUSTOM
+        o.custom_evaluate = C.cast(cb, C.c_void_p)
+    else:
+        o.cost_model = {"threshold": _abi.QLX_COST_THRESHOLD, "greedy": _abi.QLX_COST_GREEDY,
+                        "custom": _abi.QLX_COST_CUSTOM}[cost_model]
     o.enable_gate_fusion, o.max_fusion_width, o.fuse_rotations = int(fusion), max_width, int(rotations)
     o.enable_windowing, o.k_safe, o.numa_penalty, o.swap_penalty = int(windowing), k_safe, numa_penalty, swap_penalty
     o.restore_layout, o.dtype = int(restore), dtype

@@ -3,6 +3,11 @@
 // tests/middleware/test_middleware.cpp). Links libqlx.so but never touches a GPU.
 #include <complex>
 #include <cstddef>
+#include <memory>
+#include <optional>
+#include <span>
+#include <stdexcept>
+#include <utility>
 #include <type_traits>
 #include <vector>
 
@@ -147,6 +152,95 @@ TEST_CASE("transpile_gpu windowing (test_middleware.cpp:94-140, 164-235, 296-323
     gpu::transpile_gpu(m4, cfg);
     CHECK(m4.gate_count() > 5);
     CHECK(types(m4).back() == GateType::GLOBAL_SWAP);
+}
+
+namespace {
+// CustomSpyCostModel of test_middleware.cpp:237-256: evaluates, never swaps
+class SpyCostModel final : public gpu::WindowingCostModel {
+public:
+    mutable bool evaluated = false;
+    uint16_t limit;
+    explicit SpyCostModel(uint16_t l) : limit(l) {}
+    uint16_t get_k_safe() const override { return limit; }
+    std::optional<std::pair<uint16_t, uint16_t>> evaluate_swap(std::span<const uint32_t>, std::span<const uint16_t>,
+                                                               std::span<const uint16_t>) const override {
+        evaluated = true;
+        return std::nullopt;
+    }
+};
+// a model with the reference's member functions but no base class (as the reference's own
+// ThresholdCostModel is to this package): always swaps the first active unsafe qubit with the
+// first idle safe one (GreedyCostModel's choice)
+struct PlainGreedy {
+    uint16_t k;
+    uint16_t get_k_safe() const { return k; }
+    std::optional<std::pair<uint16_t, uint16_t>> evaluate_swap(std::span<const uint32_t> f, std::span<const uint16_t> l2p,
+                                                               std::span<const uint16_t>) const {
+        int cand = -1;
+        uint32_t best = 0;
+        for (std::size_t q = 0; q < f.size(); ++q)
+            if (f[q] > 0 && l2p[q] > k && f[q] > best) {
+                best = f[q];
+                cand = (int)q;
+            }
+        if (cand < 0) return std::nullopt;
+        for (std::size_t v = 0; v < f.size(); ++v)
+            if (l2p[v] <= k && f[v] == 0) return std::make_pair((uint16_t)cand, (uint16_t)v);
+        return std::nullopt;
+    }
+};
+}  // namespace
+
+TEST_CASE("custom windowing cost model (test_middleware.cpp:258-294)") {
+    QuantumCircuit qc(6);
+    qc.h(0).cx(4, 5);
+    IRModule m = std::move(qc).release();
+    auto spy = std::make_shared<SpyCostModel>(3);
+    gpu::GpuTranspilerConfig cfg;
+    cfg.enable_gate_fusion = false;
+    cfg.enable_windowing = true;
+    cfg.set_cost_model(spy);
+    CHECK(cfg.cost_model_type == gpu::CostModelType::CUSTOM && cfg.custom_cost_model == spy);
+    gpu::transpile_gpu(m, cfg);
+    CHECK(spy->evaluated);
+    CHECK(m.gate_count() == 2);
+
+    // CUSTOM without a model: invalid_argument, as the reference's transpile throws
+    QuantumCircuit qc2(4);
+    qc2.h(0);
+    IRModule m2 = std::move(qc2).release();
+    gpu::GpuTranspilerConfig bad;
+    bad.enable_windowing = true;
+    bad.cost_model_type = gpu::CostModelType::CUSTOM;
+    CHECK_THROWS_AS(gpu::transpile_gpu(m2, bad), std::invalid_argument);
+
+    // an adapted model object and CostModelType::GREEDY give the greedy stream of the
+    // windowing test above
+    auto make = [] {
+        QuantumCircuit q(10);
+        q.x(0).x(1).cx(8, 9);
+        return std::move(q).release();
+    };
+    gpu::GpuTranspilerConfig g1;
+    g1.enable_gate_fusion = false;
+    g1.enable_windowing = true;
+    g1.restore_layout = false;
+    g1.set_cost_model(gpu::cost_model_adapter(std::make_shared<const PlainGreedy>(PlainGreedy{2})));
+    IRModule a = make();
+    gpu::transpile_gpu(a, g1);
+    gpu::GpuTranspilerConfig g2;
+    g2.enable_gate_fusion = false;
+    g2.enable_windowing = true;
+    g2.restore_layout = false;
+    g2.k_safe = 2;
+    g2.cost_model_type = gpu::CostModelType::GREEDY;
+    IRModule b = make();
+    gpu::transpile_gpu(b, g2);
+    CHECK(a.gate_count() == 5 && b.gate_count() == 5);
+    for (std::size_t i = 0; i < 5; ++i) {
+        CHECK(a.program()[i].type == b.program()[i].type);
+        CHECK(a.program()[i].qubits[0] == b.program()[i].qubits[0] && a.program()[i].qubits[1] == b.program()[i].qubits[1]);
+    }
 }
 
 TEST_CASE("clustered benchmark circuit: gate counts of the strategy matrix (bench.txt:9, 15, 21, 27)") {

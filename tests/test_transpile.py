@@ -75,6 +75,94 @@ def test_windowing_restore_appends_inverse_layout():
     assert abs(s[1] - 1) < 1e-12
 
 
+# ---- test_middleware.cpp:237-294 (custom cost model) and the greedy model ----------------
+def threshold_model(k_safe, numa, swap):
+    """ThresholdCostModel::evaluate_swap (middleware.hpp:76-112), restated for the checks."""
+    def evaluate(freq, l2p, p2l):
+        cand, best = None, 0
+        for q, f in enumerate(freq):
+            if f > 0 and l2p[q] > k_safe:
+                r = f * numa
+                if r >= swap and r > best:
+                    best, cand = r, q
+        if cand is None:
+            return None
+        for v, f in enumerate(freq):
+            if l2p[v] <= k_safe and f == 0:
+                return cand, v
+        return None
+    return evaluate
+
+
+def test_custom_cost_model_is_invoked():
+    calls = []
+
+    def spy(freq, l2p, p2l):          # CustomSpyCostModel: evaluates, never swaps
+        calls.append((list(freq), list(l2p), list(p2l)))
+        return None
+    p = qlx.Circuit(6).h(0).cx(4, 5).release()
+    out = qlx.transpile_gpu(p, fusion=False, windowing=True, k_safe=3, cost_model=spy)
+    assert calls and len(out) == 2
+    freq, l2p, p2l = calls[0]
+    assert len(freq) == 6 and l2p == list(range(6)) and p2l == list(range(6))
+    assert freq[0] == 1 and freq[4] == 1 and freq[5] == 1
+
+
+def test_custom_cost_model_type_without_a_model_is_rejected():
+    with pytest.raises(ValueError):
+        qlx.transpile_gpu(qlx.Circuit(4).h(0).release(), fusion=False, windowing=True, cost_model="custom")
+
+
+def test_custom_threshold_model_equals_the_built_in_one():
+    rng = np.random.default_rng(21)
+    for trial in range(4):
+        ins, pool, meta = random_program(12, 60, rng)
+        prog = qlx.Program(12, ins, pool if pool is not None else np.zeros(0, complex),
+                           meta if meta is not None else np.zeros(0, qlx.META_DTYPE))
+        for k_safe, numa, swap in ((4, 10, 1), (6, 1, 3), (3, 1, 0)):
+            built = qlx.transpile_gpu(prog, fusion=False, windowing=True, k_safe=k_safe, numa_penalty=numa,
+                                      swap_penalty=swap)
+            custom = qlx.transpile_gpu(prog, fusion=False, windowing=True, k_safe=k_safe,
+                                       cost_model=threshold_model(k_safe, numa, swap))
+            assert built.instrs.tobytes() == custom.instrs.tobytes(), (trial, k_safe)
+
+
+def test_greedy_cost_model_is_threshold_with_penalties_one_and_zero():
+    p = qlx.Circuit(10).x(0).x(1).cx(8, 9).release()
+    greedy = qlx.transpile_gpu(p, fusion=False, windowing=True, k_safe=2, numa_penalty=100, swap_penalty=100,
+                               restore=False, cost_model="greedy")
+    assert names(greedy) == ["X", "X", "GLOBAL_SWAP", "GLOBAL_SWAP", "CX"]
+    assert greedy.instrs.tobytes() == windowed(2, 1, 0).instrs.tobytes()
+
+
+def test_custom_cost_model_errors_abort_the_pass():
+    p = qlx.Circuit(10).x(0).x(1).cx(8, 9).release()
+
+    def broken(freq, l2p, p2l):
+        raise RuntimeError("model failure")
+    with pytest.raises(ValueError):
+        qlx.transpile_gpu(p, fusion=False, windowing=True, k_safe=2, cost_model=broken)
+    with pytest.raises(ValueError):         # never converges: always proposes the same swap
+        qlx.transpile_gpu(p, fusion=False, windowing=True, k_safe=2, cost_model=lambda f, a, b: (8, 0))
+    with pytest.raises(ValueError):         # out-of-range qubit
+        qlx.transpile_gpu(p, fusion=False, windowing=True, k_safe=2, cost_model=lambda f, a, b: (8, 40))
+
+
+def test_custom_windowed_program_keeps_the_state():
+    """A custom model's swaps are real permutations with the layout restored: same state."""
+    rng = np.random.default_rng(5)
+    ins, pool, meta = random_program(10, 50, rng)
+    prog = qlx.Program(10, ins, pool if pool is not None else np.zeros(0, complex),
+                       meta if meta is not None else np.zeros(0, qlx.META_DTYPE))
+    out = qlx.transpile_gpu(prog, fusion=False, windowing=True, k_safe=4, restore=True,
+                            cost_model=threshold_model(4, 1, 0))
+    assert any(n == "GLOBAL_SWAP" for n in names(out))
+    init = bf.random_state(10, rng)
+    want, _ = oracle.execute(10, prog.instrs, prog.pool, prog.meta, amps=init)
+    got, _ = oracle.execute(10, out.instrs, out.pool, out.meta, amps=init, gswap="permute")
+    assert np.abs(got - want).max() <= 1e-12
+
+
 def test_gpu_safe_qubit_limit():
     assert qlx.gpu_safe_qubit_limit(0) == 12
     assert qlx.gpu_safe_qubit_limit(1) == 13
