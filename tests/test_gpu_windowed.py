@@ -141,6 +141,7 @@ def test_c64_phases_on_matrix_cores_vs_oracle(gpu, seed):
                                  p.meta if len(p.meta) else None, amps=init)
         inf = _infidelity(got, want)
         print(f"c64 {n}q {len(p)} gates: {info['mfma_phases']} of {info['phases']} phases fused, infidelity {inf:.2e}")
+        # both programs run some phases on the matrix cores, so the comparison covers them
+        assert info["mfma_phases"] > 0, (len(p), info)
         assert inf <= 1e-10
         assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
-    assert info["mfma_phases"] > 0
