@@ -763,10 +763,10 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
     const double take_budget = specialised && thorough ? kTakeWork * geo.elem / 16.0 : 0.0;
     std::vector<std::vector<int>> beam;
     if (chooser == Chooser::Given && !given) {
-        // circuit-specialised plans are made once per program (and compiled for seconds), so
-        // they search a wider beam: the 30-qubit depth-20 layered circuit takes 13 sweeps at
-        // width 64 against 14 at width 16 (planning 0.6 s against 0.2 s)
-        beam = beam_windows(gates, preds, (int)nq, geo, specialised ? 64 : 16, take_budget);
+        // (a width of 64 finds 13 sweeps instead of 14 for the 30-qubit depth-20 layered
+        // circuit, but one of them is a lone CX, run as a half-pass kernel, and the others are
+        // heavier: 102.4 ms against 101-102 ms, for 3x the planning time)
+        beam = beam_windows(gates, preds, (int)nq, geo, 16, take_budget);
         given = &beam;
     }
     auto plain_step = [&](int gi) {
