@@ -131,11 +131,14 @@ typedef struct qlx_exec_opts {
 #define QLX_SPECIALIZE_RELABEL 2u
 #define QLX_SPECIALIZE_ASYNC 4u
 
-/* Dense-gate arithmetic (qlx_exec_opts::dense_kernel). The MFMA forms apply to 4- and
- * 5-qubit gates on complex64 states; every other dense gate runs on the vector units
- * (FP64 for complex128, FP32 for complex64). The matrices stay FP64 in the pool and are
- * converted on the device. */
-#define QLX_DENSE_AUTO        0  /* complex64 4/5-qubit gates on states of >= k+5 qubits: MFMA FP16 */
+/* Dense-gate arithmetic (qlx_exec_opts::dense_kernel). The split and FP32 MFMA forms apply
+ * to 4- and 5-qubit gates on complex64 states. On complex128 states every value except
+ * QLX_DENSE_VALU runs 4- and 5-qubit gates on the FP64 matrix cores (v_mfma_f64_16x16x4f64,
+ * FP64 products and sums) on states of >= k+4 qubits. Every other dense gate runs on the
+ * vector units (FP64 for complex128, FP32 for complex64). The matrices stay FP64 in the pool
+ * and are converted on the device. */
+#define QLX_DENSE_AUTO        0  /* complex64 4/5-qubit gates on states of >= k+5 qubits: MFMA FP16;
+                                    complex128: FP64 MFMA */
 #define QLX_DENSE_VALU        1  /* vector units only */
 #define QLX_DENSE_MFMA_BF16   2  /* BF16 split operands (matrix hi+mid+lo, state hi+lo), four
                                     products accumulated in FP32 */

@@ -4,7 +4,8 @@
 //   single_target_loop  :260-266  -> k_1q      (one work-item per amplitude pair)
 //   double_hole_loop    :270-297  -> k_2hole   (one work-item per quad; CX/SWAP touch 1/2 of the state)
 //   triple_hole_loop    :301-334  -> k_3hole   (one work-item per octet; CCX/CSWAP touch 1/4)
-//   apply_unitary       :539-606  -> k_dense   (LDS-staged 2^k-amplitude groups, k <= 5)
+//   apply_unitary       :539-606  -> k_dense   (LDS-staged 2^k-amplitude groups, k <= 5);
+//                                        k_dense_f64mfma (complex128, k = 4, 5: FP64 matrix cores)
 //   apply_measure       :617-671  -> k_prob1 / k_collapse
 // The per-gate arithmetic follows the reference expressions term by term so that
 // permutation gates (X, Y, Z, S, CX, SWAP, CCX, CSWAP) and H are bit-exact with it.
@@ -192,6 +193,90 @@ __global__ void __launch_bounds__(kThreads) k_dense(A* __restrict__ amp, uint64_
     }
 }
 
+// ---- dense 4/5-qubit unitary on complex128, FP64 matrix cores -----------------------
+// v_mfma_f64_16x16x4f64 computes D[16x16] += A[16x4] B[4x16] in FP64; work-item l holds
+// A[l%16][l/16], B[l/16][l%16] and D[l/16 + 4v][l%16] in element v (rows interleaved; a first
+// version assumed D[4(l/16) + v], and a CPU emulation with the interleaved rows reproduced its
+// results on the GPU exactly). The gate runs as the real embedding E (2D x 2D, D = 2^K) of U,
+// E = [[Re U, -Im U], [Im U, Re U]], with rows and columns ordered so that work-items load
+// and store whole amplitudes:
+//   column c = 4s + h (k-step s, lane quarter h) <-> input amplitude 4(s/2) + h, part s%2;
+//   row 16r + i (block r, i = h + 4v) <-> output amplitude 8r + 2h + v/2, part v%2, so lane
+//   quarter h holds outputs 8r + 2h (elements 0, 1) and 8r + 2h + 1 (elements 2, 3).
+// The 16 columns of B are 16 amplitude groups (a tile). E stays in VGPRs for the whole launch
+// (K = 5: 64 doubles per work-item); wave w walks tiles w, w + W, ... (W = waves in the grid,
+// all resident) and loads the next tile's amplitudes before it computes the current one.
+// Every group of a tile is read and written by one wave, so the update is in place.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <int K>
+__global__ void __launch_bounds__(kThreads) k_dense_f64mfma(double2* __restrict__ amp, uint64_t ntiles,
+                                                           const double* __restrict__ U, DenseArgs da) {
+    constexpr int D = 1 << K;
+    constexpr int RB = 2 * D / 16;  // row blocks of E
+    constexpr int KS = 2 * D / 4;   // k-steps
+    constexpr int NP = D / 4;       // amplitudes per work-item per tile
+    const int lane = threadIdx.x & 63;
+    const int j = lane & 15, h = lane >> 4;
+    double e[RB][KS];
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int mo = 8 * r + 2 * (j & 3) + (j >> 3), po = (j >> 2) & 1;
+            const int mi = 4 * (s >> 1) + h, pi = s & 1;
+            const double ur = U[2 * (mo * D + mi)], ui = U[2 * (mo * D + mi) + 1];
+            e[r][s] = po == pi ? ur : (po == 0 ? -ui : ui);
+        }
+    // matrix index bits 0-1 of an input amplitude are h; bits 1-2 of an output amplitude are h
+    const uint64_t in_h = h == 0 ? da.off[0] : h == 1 ? da.off[1] : h == 2 ? da.off[2] : da.off[3];
+    const uint64_t out_h = h == 0 ? da.off[0] : h == 1 ? da.off[2] : h == 2 ? da.off[4] : da.off[6];
+    const uint64_t nwaves = (uint64_t)gridDim.x * (kThreads / 64);
+    uint64_t tile = (uint64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    if (tile >= ntiles) return;
+    auto group_base = [&](uint64_t t) {
+        uint64_t b = t * 16 + (uint64_t)j;
+#pragma unroll
+        for (int q = 0; q < K; ++q) b = insert_zero(b, da.sorted[q]);
+        return b;
+    };
+    uint64_t base = group_base(tile);
+    double2 x[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) x[p] = amp[base | in_h | da.off[4 * p]];
+    for (;;) {
+        const uint64_t next = tile + nwaves;
+        const bool more = next < ntiles;
+        uint64_t nbase = base;
+        double2 xn[NP];
+        if (more) {
+            nbase = group_base(next);
+#pragma unroll
+            for (int p = 0; p < NP; ++p) xn[p] = amp[nbase | in_h | da.off[4 * p]];
+        }
+        f64x4 acc[RB];
+#pragma unroll
+        for (int r = 0; r < RB; ++r) acc[r] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const double b = (s & 1) ? x[s >> 1].y : x[s >> 1].x;
+#pragma unroll
+            for (int r = 0; r < RB; ++r) acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(e[r][s], b, acc[r], 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const uint64_t o = base | out_h | da.off[8 * r];
+            amp[o] = double2{acc[r][0], acc[r][1]};
+            amp[o | da.off[1]] = double2{acc[r][2], acc[r][3]};
+        }
+        if (!more) break;
+        tile = next;
+        base = nbase;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) x[p] = xn[p];
+    }
+}
+
 // ---- measurement ----------------------------------------------------------------
 constexpr int kRedBlocks = 2048;
 
@@ -290,6 +375,24 @@ static void launch_3hole(qlx_state* st, uint32_t a, uint32_t b, uint32_t c, uint
     QLX_HIP_CHECK(hipGetLastError());
 }
 
+// complex128, 4/5-qubit gates on at least 16 groups: the FP64 matrix-core kernel, one grid of
+// resident work-groups (its waves split the tiles statically)
+template <int K>
+static void launch_dense_f64mfma(qlx_state* st, const double* dU, const DenseArgs& da) {
+    const uint64_t ntiles = (1ull << (st->nq - K)) / 16;
+    static thread_local int per_cu = 0;
+    if (!per_cu) {
+        QLX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dense_f64mfma<K>, kThreads, 0));
+        per_cu = std::max(per_cu, 1);
+    }
+    int dev = 0, cus = 0;
+    QLX_HIP_CHECK(hipGetDevice(&dev));
+    QLX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint64_t grid = std::min<uint64_t>((ntiles + 3) / 4, (uint64_t)per_cu * (uint64_t)cus);
+    k_dense_f64mfma<K><<<(unsigned)grid, kThreads, 0, st->stream>>>((double2*)st->amp, ntiles, dU, da);
+    QLX_HIP_CHECK(hipGetLastError());
+}
+
 template <class A>
 static void launch_dense_t(qlx_state* st, const double* dU, const DenseArgs& da) {
     const uint64_t ngroups = 1ull << (st->nq - da.k);
@@ -306,7 +409,7 @@ static void launch_dense_t(qlx_state* st, const double* dU, const DenseArgs& da)
     QLX_HIP_CHECK(hipGetLastError());
 }
 
-void launch_dense(qlx_state* st, const double* dU, const uint8_t* qubits, int k) {
+void launch_dense(qlx_state* st, const double* dU, const uint8_t* qubits, int k, bool matrix_cores) {
     if (k < 1 || k > 5) fail(QLX_EUNSUPPORTED, "dense gate arity must be 1..5");
     if ((uint32_t)k > st->nq) fail(QLX_ERANGE, "gate arity exceeds the qubit count");
     DenseArgs da{};
@@ -321,8 +424,14 @@ void launch_dense(qlx_state* st, const double* dU, const uint8_t* qubits, int k)
     for (int i = 0; i < k; ++i) s[i] = qubits[i];
     std::sort(s, s + k);
     for (int i = 0; i < k; ++i) da.sorted[i] = s[i];
-    if (st->dtype == QLX_C128) launch_dense_t<double2>(st, dU, da);
-    else launch_dense_t<float2>(st, dU, da);
+    if (st->dtype == QLX_C128 && matrix_cores && k >= 4 && st->nq >= (uint32_t)k + 4) {
+        if (k == 5) launch_dense_f64mfma<5>(st, dU, da);
+        else launch_dense_f64mfma<4>(st, dU, da);
+    } else if (st->dtype == QLX_C128) {
+        launch_dense_t<double2>(st, dU, da);
+    } else {
+        launch_dense_t<float2>(st, dU, da);
+    }
 }
 
 // ---- validation ---------------------------------------------------------------------
@@ -494,8 +603,10 @@ void launch_plain_instr(qlx_state* st, const qlx_gate_instr& g, const double* dp
                 launch_dense_mfma(st, (const unsigned char*)st->frags.ptr + st->frag_index[mi], qs, k,
                                   st->frag_variant);
             } else {
-                TimedLaunch tl(st, "k_dense", pass);
-                launch_dense(st, dpool + 2 * (size_t)e.offset, qs, k);
+                const bool mc = st->dtype == QLX_C128 && opts.dense_kernel != QLX_DENSE_VALU && k >= 4 &&
+                                st->nq >= (uint32_t)k + 4;
+                TimedLaunch tl(st, mc ? "k_dense_f64mfma" : "k_dense", pass);
+                launch_dense(st, dpool + 2 * (size_t)e.offset, qs, k, mc);
             }
             account(1.0);
             break;

@@ -163,7 +163,9 @@ void launch_plain_instr(qlx_state* st, const qlx_gate_instr& g, const double* dp
 
 // Dense k-qubit unitary on the device state (k <= 5), matrix already on device
 // as row-major complex128. Used by both executors.
-void launch_dense(qlx_state* st, const double* d_matrix, const uint8_t* qubits, int k);
+// matrix_cores: complex128 4/5-qubit gates on states of >= k + 4 qubits use the FP64
+// matrix-core kernel; otherwise (and with QLX_DENSE_VALU) the vector-unit kernel
+void launch_dense(qlx_state* st, const double* d_matrix, const uint8_t* qubits, int k, bool matrix_cores);
 
 // Matrix-core path for 4/5-qubit dense gates on complex64 states (qlx_mfma.hip).
 struct MfmaSplitJob {

@@ -100,9 +100,20 @@ __global__ void k_fill_basis0(uint4* __restrict__ p, uint64_t n16) {
         p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 // Plain streaming copy (16-byte vectors); also the reference point for HBM counters.
+// Whole 64 KiB chunks: one work-group per chunk, each work-item issues its 16 loads before
+// its 16 stores (the access shape of the window sweeps). Other sizes: grid-stride.
 __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+constexpr uint64_t kCopyChunk16 = 4096;  // 16-byte vectors per chunk
+__global__ void __launch_bounds__(256) k_copy_chunks(const uint4* __restrict__ src, uint4* __restrict__ dst) {
+    const uint64_t base = (uint64_t)blockIdx.x * kCopyChunk16 + threadIdx.x;
+    uint4 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = src[base + 256u * r];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) dst[base + 256u * r] = v[r];
 }
 
 __global__ void k_set_first_c128(double2* p) { p[0] = make_double2(1.0, 0.0); }
@@ -563,9 +574,14 @@ int qlx_state_copy(qlx_state* dst, qlx_state* src) {
             QLX_HIP_CHECK(hipGetDevice(&dev));
             int cus = 0;
             QLX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-            const uint64_t blocks = std::min<uint64_t>((n16 + 255) / 256, (uint64_t)cus * 16);
-            k_copy<<<(unsigned)std::max<uint64_t>(blocks, 1), 256, 0, dst->stream>>>((const uint4*)src->amp,
-                                                                                  (uint4*)dst->amp, n16);
+            if (n16 % kCopyChunk16 == 0 && n16 / kCopyChunk16 <= 0x7fffffffull) {
+                k_copy_chunks<<<(unsigned)(n16 / kCopyChunk16), 256, 0, dst->stream>>>((const uint4*)src->amp,
+                                                                                      (uint4*)dst->amp);
+            } else {
+                const uint64_t blocks = std::min<uint64_t>((n16 + 255) / 256, (uint64_t)cus * 16);
+                k_copy<<<(unsigned)std::max<uint64_t>(blocks, 1), 256, 0, dst->stream>>>((const uint4*)src->amp,
+                                                                                      (uint4*)dst->amp, n16);
+            }
         } else if (dst->dtype == src->dtype) {
             QLX_HIP_CHECK(hipMemcpyAsync(dst->amp, src->amp, dst->bytes, hipMemcpyDeviceToDevice, dst->stream));
         } else if (src->dtype == QLX_C128) {

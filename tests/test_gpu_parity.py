@@ -117,6 +117,29 @@ def test_dense_unitary_vs_oracle(gpu, k):
             assert np.abs(got - want).max() <= TOL, (n, qs)
 
 
+@pytest.mark.parametrize("k", [4, 5])
+def test_dense_unitary_fp64_matrix_cores_vs_oracle(gpu, k):
+    """complex128 4/5-qubit gates on >= k + 4 qubits run on the FP64 matrix cores
+    (k_dense_f64mfma): the smallest state (one 16-group tile), low, high, spread and unsorted
+    target sets, several gates in a row; against the oracle and the vector-unit kernel."""
+    rng = np.random.default_rng(4400 + k)
+    cases = [(k + 4, list(range(k))), (k + 4, list(range(4, 4 + k))[::-1]), (k + 5, None),
+             (17, [0, 3, 7, 12, 16][:k]), (17, None), (20, [19, 1, 10, 5, 14][:k]), (20, None)]
+    for n, qs in cases:
+        c = gpu.Circuit(n)
+        for _ in range(3):
+            qq = qs if qs is not None else [int(x) for x in rng.choice(n, k, replace=False)]
+            c.unitary(qq, bf.random_unitary(k, rng), max_arity=5)
+            qs = None
+        p = c.release()
+        init = bf.random_state(n, rng)
+        got, _ = gpu_run(gpu, p, init)
+        valu, _ = gpu_run(gpu, p, init, opts=gpu.exec_opts("plain", dense="valu"))
+        want = oracle_run(p, init)
+        assert np.abs(got - want).max() <= TOL, (n, k)
+        assert np.abs(got - valu).max() <= TOL, (n, k)
+
+
 def test_custom_swap_unitary_gather_scatter(gpu):
     sw = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], complex)
     p = gpu.Circuit(3).x(0).unitary([0, 2], sw).release()        # test_engine_kernels.cpp:173-202
