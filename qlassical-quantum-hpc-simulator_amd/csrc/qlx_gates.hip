@@ -203,10 +203,15 @@ __global__ void __launch_bounds__(kThreads) k_dense(A* __restrict__ amp, uint64_
 //   column c = 4s + h (k-step s, lane quarter h) <-> input amplitude 4(s/2) + h, part s%2;
 //   row 16r + i (block r, i = h + 4v) <-> output amplitude 8r + 2h + v/2, part v%2, so lane
 //   quarter h holds outputs 8r + 2h (elements 0, 1) and 8r + 2h + 1 (elements 2, 3).
-// The 16 columns of B are 16 amplitude groups (a tile). E stays in VGPRs for the whole launch
-// (K = 5: 64 doubles per work-item); wave w walks tiles w, w + W, ... (W = waves in the grid,
-// all resident) and loads the next tile's amplitudes before it computes the current one.
-// Every group of a tile is read and written by one wave, so the update is in place.
+// The 16 columns of B are 16 amplitude groups (a tile). E is built once per work-group in LDS
+// (K = 5: 32 KiB), fragment (r, s) as 64 consecutive doubles in lane order, so every MFMA
+// reads its A operand with one conflict-free ds_read_b64. Wave w walks tiles w, w + W, ...
+// (W = waves in the grid, all resident). Every group of a tile is read and written by one
+// wave, so the update is in place. Measured on a 30-qubit state (16 GiB, copy 6.47 ms), K = 5:
+// E in LDS without prefetch (4 waves per SIMD) 6.91 / 6.91 / 7.49 ms for targets 0-4, 25-29
+// and 3,9,15,21,27; E in VGPRs (2 waves per SIMD) 6.85 / 6.89 / 8.05 ms. K = 4 (E: 8 KiB)
+// also loads the next tile before computing the current one (spread targets: 7.19 ms; E in
+// VGPRs 7.37 ms).
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 template <int K>
@@ -216,18 +221,19 @@ __global__ void __launch_bounds__(kThreads) k_dense_f64mfma(double2* __restrict_
     constexpr int RB = 2 * D / 16;  // row blocks of E
     constexpr int KS = 2 * D / 4;   // k-steps
     constexpr int NP = D / 4;       // amplitudes per work-item per tile
+    constexpr bool kPrefetch = K == 4;
+    __shared__ double e[RB * KS * 64];
+    for (int idx = threadIdx.x; idx < RB * KS * 64; idx += kThreads) {
+        const int l = idx & 63, f = idx >> 6, r = f / KS, s = f % KS;
+        const int jj = l & 15, hh = l >> 4;
+        const int mo = 8 * r + 2 * (jj & 3) + (jj >> 3), po = (jj >> 2) & 1;
+        const int mi = 4 * (s >> 1) + hh, pi = s & 1;
+        const double ur = U[2 * (mo * D + mi)], ui = U[2 * (mo * D + mi) + 1];
+        e[idx] = po == pi ? ur : (po == 0 ? -ui : ui);
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int j = lane & 15, h = lane >> 4;
-    double e[RB][KS];
-#pragma unroll
-    for (int r = 0; r < RB; ++r)
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const int mo = 8 * r + 2 * (j & 3) + (j >> 3), po = (j >> 2) & 1;
-            const int mi = 4 * (s >> 1) + h, pi = s & 1;
-            const double ur = U[2 * (mo * D + mi)], ui = U[2 * (mo * D + mi) + 1];
-            e[r][s] = po == pi ? ur : (po == 0 ? -ui : ui);
-        }
     // matrix index bits 0-1 of an input amplitude are h; bits 1-2 of an output amplitude are h
     const uint64_t in_h = h == 0 ? da.off[0] : h == 1 ? da.off[1] : h == 2 ? da.off[2] : da.off[3];
     const uint64_t out_h = h == 0 ? da.off[0] : h == 1 ? da.off[2] : h == 2 ? da.off[4] : da.off[6];
@@ -240,20 +246,8 @@ __global__ void __launch_bounds__(kThreads) k_dense_f64mfma(double2* __restrict_
         for (int q = 0; q < K; ++q) b = insert_zero(b, da.sorted[q]);
         return b;
     };
-    uint64_t base = group_base(tile);
-    double2 x[NP];
-#pragma unroll
-    for (int p = 0; p < NP; ++p) x[p] = amp[base | in_h | da.off[4 * p]];
-    for (;;) {
-        const uint64_t next = tile + nwaves;
-        const bool more = next < ntiles;
-        uint64_t nbase = base;
-        double2 xn[NP];
-        if (more) {
-            nbase = group_base(next);
-#pragma unroll
-            for (int p = 0; p < NP; ++p) xn[p] = amp[nbase | in_h | da.off[4 * p]];
-        }
+    // one tile: 2D/4 k-steps x RB row blocks of MFMA, then the outputs in place
+    auto apply = [&](uint64_t base, const double2 (&x)[NP]) {
         f64x4 acc[RB];
 #pragma unroll
         for (int r = 0; r < RB; ++r) acc[r] = f64x4{0.0, 0.0, 0.0, 0.0};
@@ -261,7 +255,8 @@ __global__ void __launch_bounds__(kThreads) k_dense_f64mfma(double2* __restrict_
         for (int s = 0; s < KS; ++s) {
             const double b = (s & 1) ? x[s >> 1].y : x[s >> 1].x;
 #pragma unroll
-            for (int r = 0; r < RB; ++r) acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(e[r][s], b, acc[r], 0, 0, 0);
+            for (int r = 0; r < RB; ++r)
+                acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(e[(r * KS + s) * 64 + lane], b, acc[r], 0, 0, 0);
         }
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
@@ -269,11 +264,39 @@ __global__ void __launch_bounds__(kThreads) k_dense_f64mfma(double2* __restrict_
             amp[o] = double2{acc[r][0], acc[r][1]};
             amp[o | da.off[1]] = double2{acc[r][2], acc[r][3]};
         }
-        if (!more) break;
-        tile = next;
-        base = nbase;
+    };
+    if constexpr (kPrefetch) {
+        uint64_t base = group_base(tile);
+        double2 x[NP];
 #pragma unroll
-        for (int p = 0; p < NP; ++p) x[p] = xn[p];
+        for (int p = 0; p < NP; ++p) x[p] = amp[base | in_h | da.off[4 * p]];
+        for (;;) {
+            const uint64_t next = tile + nwaves;
+            const bool more = next < ntiles;
+            uint64_t nbase = base;
+            double2 xn[NP];
+            if (more) {
+                nbase = group_base(next);
+#pragma unroll
+                for (int p = 0; p < NP; ++p) xn[p] = amp[nbase | in_h | da.off[4 * p]];
+            }
+            apply(base, x);
+            if (!more) break;
+            tile = next;
+            base = nbase;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) x[p] = xn[p];
+        }
+    } else {
+        for (; tile < ntiles; tile += nwaves) {
+            // E's reads stay inside the loop (hoisted, they would take 128 VGPRs at K = 5)
+            asm volatile("" ::: "memory");
+            const uint64_t base = group_base(tile);
+            double2 x[NP];
+#pragma unroll
+            for (int p = 0; p < NP; ++p) x[p] = amp[base | in_h | da.off[4 * p]];
+            apply(base, x);
+        }
     }
 }
 
