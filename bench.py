@@ -16,6 +16,10 @@ evolving it, so every step does identical work). Prints one JSON line.
              the process exits with status 3 if it is not)
   serve      the serving workload (Poisson requests of 12-28 qubits, complex64): circuits/s,
              p50/p99 latency, infidelity of spot checks against complex128
+  precision  the 32-qubit depth-30 circuit fused to width 5 on a complex64 state (dense sweeps,
+             FP16 split operands on the matrix cores): time per circuit, accuracy against complex128
+  batch      4096 independent circuits (complex64): circuits/s of one persistent launch and of
+             per-level launches against per-circuit plain launches, spot checks against complex128
   config.dropin_default  the same circuit through the drop-in default options (QLX_EXEC_AUTO)
 """
 import argparse
@@ -140,6 +144,110 @@ def serve_summary(qlx, requests, budget_gb):
         "saturated_circuits_per_s": round(cap["circuits_per_s"], 2), "saturated_p99_ms": round(cap["p99_ms"], 3),
         "spot_checks": len(res["checks"]), "max_infidelity_vs_c128": res["max_infidelity"],
         "max_abs_vs_c128": res["max_abs"], "infidelity_ok": bool(res["max_infidelity"] <= 1e-6),
+    }
+
+
+def precision_summary(qlx, steps=3, warmup=1):
+    """The precision workload in the default run: the 32-qubit depth-30 layered circuit (seed 1003)
+    fused to width 5 (FP64 matrices built on the GPU), on a complex64 state (32 GiB) in relabelled
+    dense sweeps with FP16 split operands on the matrix cores. Accuracy against the complex128 state
+    (64 GiB) of the unfused circuit in FP64 window sweeps, from a common entangled start (RX/RY
+    product state and two CX layers, prepared in complex128 and narrowed), since from |0...0> this
+    circuit only multiplies the state by a global phase."""
+    import numpy as np
+    nq, depth, seed = 32, 30, 1003
+    prog = qlx.layered(nq, depth, seed)
+    t0 = time.perf_counter()
+    fused = qlx.transpile_gpu(prog, max_width=5, rotations=True, on_device=True)
+    fuse_s = time.perf_counter() - t0
+    o = qlx.exec_opts("dense", dense="f16", relabel=True)
+    st = qlx.State(nq, "c64")
+    for _ in range(warmup):
+        st.execute(fused, o)
+    st.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.execute(fused, o)
+    st.sync()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    bytes_step = st.last_stats()["algorithmic_bytes"]
+    rng = np.random.default_rng(seed)
+    prep = qlx.Circuit(nq)
+    for q in range(nq):
+        prep.rx(q, float(rng.uniform(0, 2 * np.pi))).ry(q, float(rng.uniform(0, 2 * np.pi)))
+    for off in (0, 1):
+        for q in range(off, nq - 1, 2):
+            prep.cx(q, q + 1)
+    prep = prep.release()
+    ref = qlx.State(nq, "c128")
+    ref.execute(prep, qlx.exec_opts("windowed"))
+    ref.sync()
+    st.copy_from(ref)                      # the same start, narrowed to complex64
+    st.sync()                              # (the copy reads ref on st's stream)
+    ref.execute(prog, qlx.exec_opts("windowed"))
+    st.execute(fused, o)
+    ref.sync()
+    st.sync()
+    ip = st.inner(ref)
+    n64 = st.norm_sq()
+    acc = {"max_abs_err": st.max_abs_diff(ref), "infidelity": 1.0 - abs(ip) ** 2 / n64, "norm_sq_minus_1": n64 - 1.0}
+    ref.close()
+    st.close()
+    log(f"precision: 32q fused width 5, c64 dense sweeps (FP16 split): {ms:.1f} ms/step; vs complex128 {acc}")
+    return {
+        "workload": f"layered h/rx/cx circuit, {nq} qubits, depth {depth}, seed {seed}, {len(prog)} gates fused "
+                    f"(width <= 5, FP64 products on the GPU) into {len(fused)} instructions; complex64 (32 GiB) "
+                    f"in relabelled dense sweeps, FP16 hi/lo split operands on the matrix cores",
+        "ms_per_step": round(ms, 3), "steps": steps, "gbps_algorithmic": round(bytes_step / (ms / 1e3) / 1e9, 1),
+        "fusion_s": round(fuse_s, 3), "accuracy_vs_complex128": acc,
+        "infidelity_ok": bool(acc["infidelity"] <= 1e-6),
+    }
+
+
+def batch_summary(qlx, n=4096, steps=3, warmup=1, check=64):
+    """The batching workload in the default run: 4096 layered circuits (N ~ U[10,20], D ~ U[5,40],
+    seeds 400000+i), complex64, fusion width and window autotuned on 256 of them; one persistent
+    launch and per-level launches timed, per-circuit plain launches as the baseline; every
+    (4096 / check)-th state compared with the complex128 plain path of its unfused circuit."""
+    from qlx import batch as qb
+    work = qlx.batch_workload(n)
+    progs = [qlx.layered(nn, d, s) for nn, d, s in work]
+    t0 = time.perf_counter()
+    tune = qb.autotune(progs, "c64", sample=min(256, len(progs)))
+    tune_s = time.perf_counter() - t0
+    b = qlx.Batch(qb.prepare_programs(progs, tune["fusion_width"]), "c64", window_qubits=tune["window_qubits"])
+    res = {}
+    for mode, k in (("persistent", steps), ("levels", steps), ("per_circuit", 1)):
+        for _ in range(warmup):
+            b.run(mode)
+            b.sync()
+        times = []
+        for _ in range(k):
+            b.run(mode)
+            times.append(b.sync())
+        ms = sorted(times)[len(times) // 2]
+        res[mode] = {"ms": round(ms, 3), "circuits_per_s": round(len(progs) / (ms / 1e3), 1)}
+    best = max(("persistent", "levels"), key=lambda m: res[m]["circuits_per_s"])
+    b.run(best)
+    b.sync()
+    worst = 0.0
+    idx = list(range(0, len(progs), max(1, len(progs) // check)))
+    for i in idx:
+        ref = qlx.State(progs[i].num_qubits, "c128")
+        ref.execute(progs[i])
+        ref.sync()
+        inf, _ = b.compare(i, ref)
+        ref.close()
+        worst = max(worst, inf)
+    b.close()
+    log(f"batch: {res}; best {best}; max infidelity vs complex128 over {len(idx)} circuits {worst:.2e}")
+    return {
+        "workload": f"{len(progs)} layered h/rx/cx circuits, N~U[10,20], D~U[5,40] (seeds 400000+i), complex64",
+        "circuits_per_s": res[best]["circuits_per_s"], "mode": best, "modes": res,
+        "speedup_vs_per_circuit": round(res[best]["circuits_per_s"] / res["per_circuit"]["circuits_per_s"], 1),
+        "autotune": {"fusion_width": tune["fusion_width"], "window_qubits": tune["window_qubits"],
+                     "seconds": round(tune_s, 1)},
+        "checked": len(idx), "max_infidelity_vs_c128": worst, "infidelity_ok": bool(worst <= 1e-6),
     }
 
 
@@ -501,6 +609,8 @@ def main():
                          "or on the first layer only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="circuit30: skip the serving leg")
+    ap.add_argument("--no-precision", action="store_true", help="circuit30: skip the 32-qubit complex64 leg")
+    ap.add_argument("--no-batch", action="store_true", help="circuit30: skip the 4096-circuit batch leg")
     ap.add_argument("--config", choices=["circuit30", "precision", "batch", "serve", "clustered"],
                     default="circuit30")
     ap.add_argument("--dense-variants", default="dense:f16,dense:bf16,windowed:f16,dense:f32,windowed:valu,unfused:windowed",
@@ -723,6 +833,10 @@ def main():
             out["cpu_baseline"], out["parity"] = cpu_baseline_and_parity(qlx, prog, nq, opts, 300000 + args.seed)
     if not args.no_serve:
         out["serve"] = serve_summary(qlx, args.serve_requests, args.serve_budget_gb)
+    if not args.no_precision:
+        out["precision"] = precision_summary(qlx)
+    if not args.no_batch:
+        out["batch"] = batch_summary(qlx)
     print(json.dumps(out), flush=True)
     if "parity" in out and not out["parity"]["ok"]:
         sys.exit(3)
