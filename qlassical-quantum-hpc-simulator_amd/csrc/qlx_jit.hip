@@ -55,10 +55,15 @@ struct Gen {
     int logt, rb, w, R, elem;
     std::string out;
     std::vector<int> map;  // register index -> variable
-    // pending conditional X (OP_CXT): xf[r] = mask of work-item bits whose parity, per
-    // work-item, says whether register bit r is flipped (register i holds the amplitude of
-    // register position i ^ f, f = the flipped bits)
-    std::vector<uint32_t> xf;
+    // Pauli frame of the registers: per register bit r, xf[r] and zf[r] are masks of work-item
+    // bits whose parities, per work-item, give the exponents of X and Z on that bit. The chunk
+    // is (prod_r X_r^x Z_r^z) applied to the registers: register i holds the amplitude of
+    // register position i ^ f (f = the bits with x = 1), negated for each bit r with z = 1 that
+    // is set in i ^ f. A CX whose control is a work-item bit (OP_CXT) adds an X; Clifford ops
+    // on register bits move the frame (H swaps a bit's X and Z, CX spreads X from control to
+    // target and Z from target to control), RX carries X and runs with a per-work-item sign on
+    // its angle under Z; other ops, LDS re-layouts and the stores apply it.
+    std::vector<uint32_t> xf, zf;
     int tmp = 0;
     void line(const std::string& s) { out += s; out += '\n'; }
     std::string var(int i) const { return "r" + std::to_string(map[i]); }
@@ -66,6 +71,25 @@ struct Gen {
         for (uint32_t m : xf)
             if (m) return true;
         return false;
+    }
+    // applies the pending Z on register bit r: negates, per work-item, the registers with bit r set
+    void materialize_z(int r) {
+        if (!zf[r]) return;
+        const std::string U = f32 ? "unsigned" : "unsigned long long";
+        const std::string F = f32 ? "float" : "double";
+        const std::string sign = f32 ? "0x80000000u" : "0x8000000000000000ull";
+        line("  { const " + U + " zs = " + parity(zf[r]) + " ? " + sign + " : 0;");
+        for (int i = 0; i < R; ++i) {
+            if (!(i & (1 << r))) continue;
+            const std::string x = var(i);
+            line("    " + x + ".x = __builtin_bit_cast(" + F + ", __builtin_bit_cast(" + U + ", " + x + ".x) ^ zs); " +
+                 x + ".y = __builtin_bit_cast(" + F + ", __builtin_bit_cast(" + U + ", " + x + ".y) ^ zs);");
+        }
+        line("  }");
+        zf[r] = 0;
+    }
+    void materialize_all_z() {
+        for (int r = 0; r < rb; ++r) materialize_z(r);
     }
     std::string parity(uint32_t mask) const {
         return "((__builtin_popcount(" + tv + " & " + std::to_string(mask) + "u) & 1u) != 0u)";
@@ -112,13 +136,35 @@ struct Gen {
         const std::string T = f32 ? "float" : "double", T2 = f32 ? "float2" : "double2";
         const double* m = p.coef.data() + o.coef;
         const std::string id = std::to_string(tmp++);
-        // pending X frames: RX, SCALE and the CX/SWAP renames carry them; other ops apply them
+        // the pending frame through the op (Z before X where both are applied: the registers
+        // hold X^x Z^z of the chunk)
+        const bool rx_sign = kind == OP_RXT && zf[b0] != 0u;  // RX(theta) Z = Z RX(-theta)
         switch (kind) {
-            case OP_RXT: case OP_RXU: case OP_SCALE: case OP_CXT: break;
-            case OP_CX: xf[b1] ^= xf[b0]; break;               // X on the control spreads to the target
-            case OP_SWAP: std::swap(xf[b0], xf[b1]); break;
-            case OP_U4: materialize(b0); materialize(b1); break;
-            default: materialize(b0); break;                     // OP_HB, OP_U2
+            case OP_RXT: case OP_SCALE: break;
+            case OP_RXU: materialize_z(b0); break;
+            case OP_CXT: materialize_z(b1); break;               // Z on the target would reach the control
+            case OP_CX:                                          // control b0, target b1
+                xf[b1] ^= xf[b0];
+                zf[b0] ^= zf[b1];
+                break;
+            case OP_SWAP:
+                std::swap(xf[b0], xf[b1]);
+                std::swap(zf[b0], zf[b1]);
+                break;
+            case OP_HB:                                          // H X = Z H, H Z = X H
+                if (xf[b0] && zf[b0]) materialize_z(b0);
+                std::swap(xf[b0], zf[b0]);
+                break;
+            case OP_U4:
+                materialize_z(b0);
+                materialize(b0);
+                materialize_z(b1);
+                materialize(b1);
+                break;
+            default:                                             // OP_U2
+                materialize_z(b0);
+                materialize(b0);
+                break;
         }
         switch (kind) {
             case OP_CXT:
@@ -128,13 +174,18 @@ struct Gen {
             case OP_RXT: {
                 // RX / c: a0' = a0 - i t a1, a1' = a1 - i t a0
                 const std::string F = f32 ? "__builtin_fmaf" : "__builtin_fma";
-                line("  { const " + T + " t = " + lit(o.c0, f32) + ";");
+                // (the coefficient is `tc`: `t` is the work-item index the frame parities read)
+                if (rx_sign)
+                    line("  { const " + T + " tc = " + parity(zf[b0]) + " ? -(" + lit(o.c0, f32) + ") : (" +
+                         lit(o.c0, f32) + ");");
+                else
+                    line("  { const " + T + " tc = " + lit(o.c0, f32) + ";");
                 for (int i = 0; i < R; ++i) {
                     if (i & (1 << b0)) continue;
                     const std::string x = var(i), y = var(i | (1 << b0));
                     line("    { const " + T2 + " a0 = " + x + ", a1 = " + y + "; " + x + " = " + T2 + "{" + F +
-                         "(t, a1.y, a0.x), " + F + "(-t, a1.x, a0.y)}; " + y + " = " + T2 + "{" + F +
-                         "(t, a0.y, a1.x), " + F + "(-t, a0.x, a1.y)}; }");
+                         "(tc, a1.y, a0.x), " + F + "(-tc, a1.x, a0.y)}; " + y + " = " + T2 + "{" + F +
+                         "(tc, a0.y, a1.x), " + F + "(-tc, a0.x, a1.y)}; }");
                 }
                 line("  }");
                 break;
@@ -279,6 +330,7 @@ struct Gen {
         line("  const unsigned long long lo = base | tl;");
         map.resize(R);
         xf.assign(rb, 0u);
+        zf.assign(rb, 0u);
         for (int i = 0; i < R; ++i) {
             map[i] = i;
             line("  " + T2 + " r" + std::to_string(i) + " = " + ld + "[lo | " + hex64(sw.ld_roff[i]) + "];");
@@ -302,6 +354,7 @@ struct Gen {
         for (uint32_t ph = 0; ph < sw.n_phases; ++ph) {
             const DevPhase& P = p.dph[sw.phase_begin + ph];
             if (P.remap) {
+                materialize_all_z();  // Z: sign flips in the registers; X: folded into the addresses
                 line("  {");
                 line("    const unsigned l0 = " + thread_bits(cur->tlocal) + ";");
                 line("    const unsigned s0 = l0 ^ (((l0 >> " + std::to_string(sh) + ") ^ (l0 >> " + std::to_string(2 * sh) +
@@ -333,6 +386,7 @@ struct Gen {
             cur = &P;
             for (uint32_t k = 0; k < P.n_ops; ++k) op(p.dop[P.op_begin + k]);
         }
+        materialize_all_z();
         if (p.relabel) {
             emit_base("c", "sbase", "  ", true);
             line("  const unsigned long long so = sbase | ts;");

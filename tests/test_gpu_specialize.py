@@ -86,6 +86,52 @@ def test_specialized_window_geometries(gpu, dtype, w, rb):
             assert inf <= 1e-6, (w, rb, n, inf)
 
 
+@pytest.mark.parametrize("dtype,relabel", [("c128", False), ("c128", True), ("c64", True)])
+def test_pauli_frames_vs_oracle(gpu, dtype, relabel):
+    """CX gates whose control is outside the registers leave pending X frames; H turns them into
+    Z frames, CX and SWAP move them, RX carries X and runs with a per-work-item sign on its angle
+    under Z, other gates, re-layouts and stores apply them. Random H/RX/CX/SWAP/S/Z/RY circuits
+    with long-range CX, and the clustered H/CX circuit, against the oracle."""
+    rng = np.random.default_rng(7100 + relabel)
+    progs = []
+    for n in (13, 16):
+        c = gpu.Circuit(n)
+        for _ in range(300):
+            r = rng.random()
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            if r < 0.35:
+                c.cx(a, b)
+            elif r < 0.6:
+                c.h(a)
+            elif r < 0.82:
+                c.rx(a, float(rng.uniform(0, 2 * np.pi)))
+            elif r < 0.87:
+                c.swap(a, b)
+            elif r < 0.92:
+                c.s(a)
+            elif r < 0.96:
+                c.z(a)
+            else:
+                c.ry(a, float(rng.uniform(0, 2 * np.pi)))
+        progs.append(c.release())
+    progs.append(gpu.clustered(18))
+    o = gpu.exec_opts("windowed", specialize=True, relabel=relabel)
+    for prog in progs:
+        n = prog.num_qubits
+        init = bf.random_state(n, rng)
+        want, _ = oracle.execute(n, prog.instrs, amps=init)
+        st = gpu.State(amps=init, dtype=dtype)
+        st.timing(True)
+        st.execute(prog, o)
+        assert "k_sweep_spec" in st.timing_read()
+        got = st.to_host()
+        st.close()
+        if dtype == "c128":
+            assert np.abs(got - want).max() <= 1e-12, n
+        else:
+            assert 1 - abs(np.vdot(want, got)) ** 2 / np.vdot(got, got).real <= 1e-6, n
+
+
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 def test_long_runs_of_factored_gates(gpu, dtype):
     """Hundreds of H and RX gates in one sweep: their factors are applied in several steps

@@ -99,18 +99,61 @@ def test_cx_controlled_by_a_work_item_bit_costs_no_instructions():
         assert rc == 0, log[-2000:]
 
 
-def test_pending_x_is_applied_before_an_op_it_does_not_commute_with():
-    """H after a work-item-controlled CX on the same target: the pending X is applied first, as
-    conditional swaps of the register pairs."""
-    n = 12
+def sources(prog):
+    o = qlx.exec_opts("windowed", specialize=True)
+    return [qlx.sweep_source(prog, s, "c128", o) for s in range(qlx.plan_windowed(prog, "c128", o)[2]["sweeps"])]
+
+
+def chain(n):
     c = qlx.Circuit(n)
     for q in range(n - 1):
-        c.cx(q, q + 1)
+        c.cx(q, q + 1)                  # work-item-controlled CX: pending X frames on the targets
+    return c
+
+
+def test_h_turns_a_pending_x_into_a_pending_z():
+    """H after a work-item-controlled CX on the same target: H X = Z H, so the frame becomes a
+    pending Z (no register swaps), applied as sign flips before the re-layout or store."""
+    n = 12
+    c = chain(n)
     for q in range(1, n):
         c.h(q)
-    prog = c.release()
-    srcs = [qlx.sweep_source(prog, s, "c128") for s in range(qlx.plan_windowed(
-        prog, "c128", qlx.exec_opts("windowed", specialize=True))[2]["sweeps"])]
+    srcs = sources(c.release())
+    assert not any("fc ?" in s for s in srcs)
+    assert any("zs = " in s and "^ zs)" in s for s in srcs)
+    for s in srcs:
+        if os.path.exists(HIPRTC):
+            rc, log = compile_ok(s)
+            assert rc == 0, log[-2000:]
+
+
+def test_rx_under_a_pending_z_flips_its_angle_per_work_item():
+    """RX(theta) Z = Z RX(-theta): under a pending Z the RX coefficient t = tan(theta/2) takes a
+    per-work-item sign; the u = cot(theta/2) form (|sin| > |cos|) applies the Z first."""
+    n = 12
+    c = chain(n)
+    for q in range(1, n):
+        c.h(q)
+    c.rx(5, 0.4)                        # t form
+    c.rx(6, 3.0)                        # u form
+    srcs = sources(c.release())
+    assert any(re.search(r"const double tc = \(\(__builtin_popcount\(t & \d+u\) & 1u\) != 0u\) \? -\(", s)
+               for s in srcs)
+    if os.path.exists(HIPRTC):
+        for s in srcs:
+            rc, log = compile_ok(s)
+            assert rc == 0, log[-2000:]
+    assert not any("fc ?" in s for s in srcs)
+
+
+def test_pending_frame_is_applied_before_a_dense_gate():
+    """A dense 1-qubit gate (RY runs as a 2x2 matrix) commutes with neither X nor Z: the pending
+    X is applied first, as conditional swaps of the register pairs."""
+    n = 12
+    c = chain(n)
+    for q in range(1, n):
+        c.ry(q, 0.3)
+    srcs = sources(c.release())
     assert any("fc ? a1 : a0" in s for s in srcs)
 
 
