@@ -109,7 +109,17 @@ typedef struct qlx_exec_opts {
     uint32_t specialize;      /* QLX_SPECIALIZE_* flags. windowed: 0 = the generic sweep kernel;
                                  dense sweeps: QLX_SPECIALIZE_RELABEL alone selects out-of-place
                                  relabelled sweeps */
+    uint32_t plan_flags;      /* QLX_PLAN_* flags of the window-sweep planner; 0 = defaults */
 } qlx_exec_opts;
+
+/* qlx_exec_opts::plan_flags.
+ * By default the window-sweep planner (windowed and AUTO modes) merges a rotation RX/RY/RZ(b)
+ * into an earlier rotation of the same axis on the same qubit, RX(a) -> RX(a + b), when every
+ * gate between the two on that qubit commutes with it: for RX an X or the target of a CX/CCX,
+ * for RY a Y, for RZ a Z, S or a control of a CX/CCX/CSWAP. The state is the same up to
+ * rounding (the angles are summed in double precision, as the cosines and sines are formed).
+ * QLX_PLAN_KEEP_ROTATIONS applies every rotation as written. Plain execution never merges. */
+#define QLX_PLAN_KEEP_ROTATIONS 1u
 
 /* qlx_exec_opts::specialize flags.
  * QLX_SPECIALIZE: compile the program's sweeps into straight-line kernels (hipRTC, once per
@@ -241,7 +251,9 @@ int qlx_execute_noalloc(qlx_state* st,
  * dense-sweep) execution (no GPU work): the order in which the instructions of a
  * MEASURE-free stream would run. order[k] is the source index of the k-th executed
  * instruction and sweep_of[k] the window sweep that runs it (-1: its own kernel). Gates
- * are only reordered past gates on disjoint qubits. */
+ * are only reordered past gates on disjoint qubits, except merged rotations (see
+ * qlx_exec_opts::plan_flags): a rotation merged into an earlier one is listed right after it,
+ * having moved back past gates on its qubit that commute with it. */
 typedef struct qlx_plan_info {
     uint64_t sweeps;           /* window-sweep launches (one full HBM pass each) */
     uint64_t phases;           /* register phases over all sweeps (LDS re-layouts = phases - sweeps) */
@@ -253,6 +265,7 @@ typedef struct qlx_plan_info {
     uint32_t coalesce_qubits;  /* low qubits always in the window */
     uint64_t mfma_phases;      /* complex64 generic sweeps: phases fused into one 32 x 32 matrix and
                                   applied on the matrix cores */
+    uint64_t rotations_merged; /* rotations merged into an earlier one (applied as one op) */
 } qlx_plan_info;
 int qlx_plan_windowed(uint32_t num_qubits, int dtype, const qlx_gate_instr* instrs, size_t n,
                       const double* pool_c128, size_t pool_elems,

@@ -123,7 +123,70 @@ struct PGate {
     // a CX of a circuit-specialised sweep: its control may sit on a work-item bit (OP_CXT), so a
     // register phase can run it when its target alone is in the registers
     bool ctl_free = false;
+    int merged = -1;    // a rotation that absorbed later ones: its index in the MergedRot table
 };
+
+// A rotation RX/RY/RZ that absorbed later rotations of the same axis on its qubit: the summed
+// angle and the source instructions in program order (the first is the PGate's own src).
+struct MergedRot {
+    double theta;
+    std::vector<int> srcs;
+};
+
+// Whether operand k of a gate of type `t` commutes with a rotation about axis a (0 X, 1 Y,
+// 2 Z) on that operand's qubit: X with X and with the target of CX/CCX (X on the target,
+// conditioned on the controls); Y with Y; Z with the diagonal Z and S and with the controls
+// of CX/CCX/CSWAP (the gate is block-diagonal in the control's basis states).
+static bool commutes_with_rotation(uint8_t t, int k, int a) {
+    switch (a) {
+        case 0: return t == QLX_X || (t == QLX_CX && k == 1) || (t == QLX_CCX && k == 2);
+        case 1: return t == QLX_Y;
+        default:
+            return t == QLX_Z || t == QLX_S || (t == QLX_CX && k == 0) || (t == QLX_CCX && k < 2) ||
+                   (t == QLX_CSWAP && k == 0);
+    }
+}
+
+// Merges every sweepable rotation into the latest earlier rotation of the same axis on its
+// qubit when each gate between the two on that qubit commutes with it: the later rotation
+// moves back past those gates (and past gates on other qubits, trivially), and
+// R(b)·R(a) = R(a + b) about one axis. The angles are the instructions' floats, summed in
+// double precision (exact for angles within a few turns of each other). Removes the merged
+// gates from `gates`; their sources stay listed in the table entry of the gate they joined.
+static void merge_rotations(std::vector<PGate>& gates, const qlx_gate_instr* instrs, uint32_t nq,
+                            std::vector<MergedRot>& merged) {
+    std::vector<int> open[3];
+    for (auto& o : open) o.assign(nq, -1);
+    std::vector<char> drop(gates.size(), 0);
+    for (size_t i = 0; i < gates.size(); ++i) {
+        PGate& g = gates[i];
+        const int axis = g.type == QLX_RX ? 0 : g.type == QLX_RY ? 1 : g.type == QLX_RZ ? 2 : -1;
+        if (axis >= 0 && g.sweepable) {
+            const int q = g.q[0];
+            const int j = open[axis][q];
+            if (j >= 0) {
+                PGate& h = gates[j];
+                if (h.merged < 0) {
+                    h.merged = (int)merged.size();
+                    merged.push_back(MergedRot{static_cast<double>(instrs[h.src].payload.param), {h.src}});
+                }
+                merged[h.merged].theta += static_cast<double>(instrs[g.src].payload.param);
+                merged[h.merged].srcs.push_back(g.src);
+                drop[i] = 1;
+                continue;
+            }
+            for (int a = 0; a < 3; ++a) open[a][q] = a == axis ? (int)i : -1;
+            continue;
+        }
+        for (int k = 0; k < g.nq; ++k)
+            for (int a = 0; a < 3; ++a)
+                if (!commutes_with_rotation(g.type, k, a)) open[a][g.q[k]] = -1;
+    }
+    size_t w = 0;
+    for (size_t i = 0; i < gates.size(); ++i)
+        if (!drop[i]) gates[w++] = gates[i];
+    gates.resize(w);
+}
 
 static bool is_sweepable(const qlx_gate_instr& g, const qlx_matrix_entry* meta) {
     switch (g.type) {
@@ -189,8 +252,10 @@ static int count_runnable(const std::vector<PGate>& gates, const std::vector<int
     return cnt;
 }
 
+// `theta`, when given, replaces a rotation's angle (a merged rotation).
 static void fill_coef(std::vector<double>& coef, const qlx_gate_instr& g, const double* pool,
-                      const qlx_matrix_entry* meta, uint8_t& kind) {
+                      const qlx_matrix_entry* meta, uint8_t& kind, const double* theta = nullptr) {
+    const double angle = theta ? *theta : static_cast<double>(g.payload.param);
     auto push_u2 = [&](double m00r, double m00i, double m01r, double m01i, double m10r, double m10i, double m11r,
                        double m11i) {
         kind = OP_U2;
@@ -204,19 +269,19 @@ static void fill_coef(std::vector<double>& coef, const qlx_gate_instr& g, const 
             break;
         case QLX_RX: {
             kind = OP_RX;
-            const double half = static_cast<double>(g.payload.param) * 0.5;
+            const double half = angle * 0.5;
             coef.push_back(std::cos(half));
             coef.push_back(std::sin(half));
             break;
         }
         case QLX_RY: {
-            const double half = static_cast<double>(g.payload.param) * 0.5;
+            const double half = angle * 0.5;
             const double c = std::cos(half), s = std::sin(half);
             push_u2(c, 0, -s, 0, s, 0, c, 0);
             break;
         }
         case QLX_RZ: {
-            const double half = static_cast<double>(g.payload.param) * 0.5;
+            const double half = angle * 0.5;
             push_u2(std::cos(-half), std::sin(-half), 0, 0, 0, 0, std::cos(half), std::sin(half));
             break;
         }
@@ -501,6 +566,7 @@ static void plan_phases(PlanSweep& sw, const std::vector<PGate>& lg, const Geome
                     op.l0 = -1;
                 }
                 op.kind = lg[gi].type;  // instruction type; resolved to an OpKind later
+                op.merged = lg[gi].merged;
                 phase.ops.push_back(op);
                 done[gi] = 1;
             } else {
@@ -748,6 +814,9 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
         p.sweepable = is_sweepable(g, meta) && (int)nq >= W;
         gates.push_back(p);
     }
+    std::vector<MergedRot> merged;
+    if (!(opts.plan_flags & QLX_PLAN_KEEP_ROTATIONS)) merge_rotations(gates, instrs, nq, merged);
+    for (const MergedRot& m : merged) plan->rotations_merged += m.srcs.size() - 1;
     const int G = (int)gates.size();
     std::vector<std::vector<int>> preds(G);
     {
@@ -770,10 +839,15 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
         given = &beam;
     }
     auto plain_step = [&](int gi) {
-        plan->steps.push_back(PlanStep{false, gates[gi].src, -1});
-        plan->order.push_back(gates[gi].src);
-        plan->order_sweep.push_back(-1);
-        plan->passes += plain_passes(gates[gi].type);
+        // a merged rotation runs as the instructions it holds (consecutive: they commute to here)
+        std::vector<int> srcs{gates[gi].src};
+        if (gates[gi].merged >= 0) srcs = merged[gates[gi].merged].srcs;
+        for (int src : srcs) {
+            plan->steps.push_back(PlanStep{false, src, -1});
+            plan->order.push_back(src);
+            plan->order_sweep.push_back(-1);
+            plan->passes += plain_passes(gates[gi].type);
+        }
     };
 
     // 1. the walk: plain steps and sweeps (window + the gates it takes, in program order)
@@ -869,6 +943,14 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
         const int si = (int)plan->sweeps.size();
         for (const PlanPhase& ph : sw.phases)
             for (const PlanOp& op : ph.ops) {
+                if (op.merged >= 0) {
+                    for (int src : merged[op.merged].srcs) {
+                        plan->order.push_back(src);
+                        plan->order_sweep.push_back(si);
+                    }
+                    plan->gates_in_sweeps += merged[op.merged].srcs.size() - 1;
+                    continue;
+                }
                 plan->order.push_back(op.src);
                 plan->order_sweep.push_back(si);
             }
@@ -925,7 +1007,8 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
                 DevOp o{};
                 o.coef = (uint32_t)plan->coef.size();
                 uint8_t kind = 0;
-                fill_coef(plan->coef, instrs[op.src], pool, meta, kind);
+                fill_coef(plan->coef, instrs[op.src], pool, meta, kind,
+                          op.merged >= 0 ? &merged[op.merged].theta : nullptr);
                 if (op.tctl >= 0) {
                     // CX with its control on work-item bit tctl (circuit-specialised sweeps only)
                     if (kind != OP_CX || !factored || op.tctl >= geo.logt || op.l1 < 0 || op.l1 >= geo.rb)
@@ -1646,6 +1729,7 @@ extern "C" int qlx_plan_windowed(uint32_t num_qubits, int dtype, const qlx_gate_
                 info->plain_steps = p->steps.size() - p->sweeps.size();
                 info->gates_in_sweeps = p->gates_in_sweeps;
                 for (const DevPhase& ph : p->dph) info->mfma_phases += ph.mfma ? 1 : 0;
+                info->rotations_merged = p->rotations_merged;
             }
         }
     });

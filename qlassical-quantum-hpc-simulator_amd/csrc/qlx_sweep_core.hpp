@@ -414,6 +414,7 @@ struct PlanOp {
     uint8_t kind;
     int l0, l1;     // register bits (l1 = -1 for 1q)
     int tctl = -1;  // a CX whose control is this work-item bit (OP_CXT; l0 unused)
+    int merged = -1;  // a merged rotation: its index in the planner's table of merged rotations
 };
 
 struct PlanPhase {
@@ -445,7 +446,7 @@ struct SweepPlan {
     std::vector<DevPhase> dph;
     std::vector<DevOp> dop;
     std::vector<double> coef;
-    uint64_t phases = 0, gates_in_sweeps = 0;
+    uint64_t phases = 0, gates_in_sweeps = 0, rotations_merged = 0;
     double passes = 0.0;  // full-state passes: one per sweep, plus the plain steps' fractions
     uint64_t tag = 0;     // hash of the planned program (names its generated kernels)
     // out-of-place sweeps with per-sweep qubit layouts (assign_layouts): step_pos[s][q] is the

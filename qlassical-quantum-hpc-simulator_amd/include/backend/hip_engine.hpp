@@ -60,6 +60,10 @@ struct HIPEngineOptions {
     // (once per process; the ROCm compiler's disk cache also serves later processes) every
     // execution of it, from any engine, runs them. Off: the generic sweep kernel only.
     bool auto_specialize = true;
+    // window sweeps: merge a rotation into an earlier one of the same axis on the same qubit
+    // when the gates between them on that qubit commute with it (QLX_PLAN_KEEP_ROTATIONS off);
+    // the state is the same up to rounding
+    bool merge_rotations = true;
 };
 
 class HIPEngine final : public ExecutionEngine {
@@ -107,6 +111,7 @@ public:
                  : (opts_.auto_specialize && !opts_.specialize) ? QLX_EXEC_AUTO
                                                                  : QLX_EXEC_WINDOWED;
         o.gswap_mode = opts_.global_swap_noop ? QLX_GSWAP_NOOP : QLX_GSWAP_PERMUTE;
+        o.plan_flags = opts_.merge_rotations ? 0u : QLX_PLAN_KEEP_ROTATIONS;
         if (opts_.dense_sweeps)
             o.specialize = opts_.relabel ? QLX_SPECIALIZE_RELABEL : 0u;
         else

@@ -31,6 +31,7 @@ QLX_AUTO_SPECIALIZE_QUBITS = 20
 QLX_SPECIALIZE = 1
 QLX_SPECIALIZE_RELABEL = 2
 QLX_SPECIALIZE_ASYNC = 4
+QLX_PLAN_KEEP_ROTATIONS = 1
 QLX_DENSE_AUTO = 0
 QLX_DENSE_VALU = 1
 QLX_DENSE_MFMA_BF16 = 2
@@ -79,6 +80,7 @@ class ExecOpts(C.Structure):
         ("dense_kernel", C.c_uint32),
         ("sweep_workgroups", C.c_uint32),
         ("specialize", C.c_uint32),
+        ("plan_flags", C.c_uint32),
     ]
 
 
@@ -114,6 +116,7 @@ class PlanInfo(C.Structure):
         ("threads_per_group", C.c_uint32),
         ("coalesce_qubits", C.c_uint32),
         ("mfma_phases", C.c_uint64),
+        ("rotations_merged", C.c_uint64),
     ]
 
 

@@ -40,7 +40,7 @@ def test_version_and_error_string():
 
 
 def test_struct_sizes_match_header():
-    assert C.sizeof(qlx._abi.ExecOpts) == 32
+    assert C.sizeof(qlx._abi.ExecOpts) == 36
     assert C.sizeof(qlx._abi.ExecStats) == 48
     assert ir.GATE_DTYPE.itemsize == 16 and ir.META_DTYPE.itemsize == 12
 

@@ -141,3 +141,83 @@ def test_c64_generic_plans_fuse_heavy_register_phases():
         c.cx(q, q + 1)
     _, _, cx = qlx.plan_windowed(c.release(), "c64", qlx.exec_opts("windowed"))
     assert cx["mfma_phases"] == 0
+
+
+def rotation_rich(n, count, rng):
+    """Random program on n qubits, mostly rotations, with every gate kind that commutes with
+    some axis on some operand (X, Y, Z, S, CX, CCX, CSWAP) and kinds that commute with none
+    (H, SWAP), so that merges both happen and are blocked."""
+    c = qlx.Circuit(n)
+    for _ in range(count):
+        k = int(rng.integers(12))
+        q = [int(x) for x in rng.permutation(n)[:3]]
+        th = float(rng.uniform(-7.0, 7.0))
+        if k < 4:
+            [c.rx, c.ry, c.rz, c.rx][k](q[0], th)
+        elif k == 4:
+            [c.x, c.y, c.z, c.s][int(rng.integers(4))](q[0])
+        elif k < 7:
+            c.cx(q[0], q[1])
+        elif k == 7:
+            c.ccx(q[0], q[1], q[2])
+        elif k == 8:
+            c.cswap(q[0], q[1], q[2])
+        elif k == 9:
+            c.h(q[0])
+        elif k == 10:
+            c.swap(q[0], q[1])
+        else:
+            c.rz(q[0], th)
+    return c.release()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_merged_rotations_are_a_valid_reordering(seed):
+    """Rotations merged through the gates they commute with: the order lists each merged
+    rotation right after the one it joined, and that reordered stream gives the original
+    state on the oracle (a wrong commutation rule would not)."""
+    rng = np.random.default_rng(4200 + seed)
+    n = 13
+    prog = rotation_rich(n, 160, rng)
+    order, sweep, info = qlx.plan_windowed(prog, "c128", qlx.exec_opts("windowed", specialize=True))
+    kept = qlx.plan_windowed(prog, "c128", qlx.exec_opts("windowed", specialize=True, keep_rotations=True))[2]
+    assert info["rotations_merged"] > 0 and kept["rotations_merged"] == 0
+    assert sorted(order.tolist()) == list(range(len(prog)))
+    init = bf.random_state(n, rng)
+    a, _ = oracle.execute(n, prog.instrs, amps=init)
+    b, _ = oracle.execute(n, prog.instrs[order], amps=init)
+    np.testing.assert_allclose(b, a, atol=1e-12)
+
+
+@pytest.mark.parametrize("build,merges", [
+    (lambda c: c.rx(3, 0.3).x(3).cx(1, 3).ccx(0, 2, 3).rx(3, 0.4), 1),   # X, CX/CCX targets
+    (lambda c: c.rx(3, 0.3).cx(3, 1).rx(3, 0.4), 0),                     # CX control
+    (lambda c: c.rx(3, 0.3).h(3).rx(3, 0.4), 0),
+    (lambda c: c.rx(3, 0.3).rz(3, 0.2).rx(3, 0.4), 0),
+    (lambda c: c.rz(3, 0.3).z(3).s(3).cx(3, 1).ccx(3, 0, 2).cswap(3, 0, 1).rz(3, 0.4), 1),
+    (lambda c: c.rz(3, 0.3).cx(1, 3).rz(3, 0.4), 0),                     # CX target
+    (lambda c: c.rz(3, 0.3).cswap(0, 3, 1).rz(3, 0.4), 0),               # CSWAP swapped operand
+    (lambda c: c.ry(3, 0.3).y(3).ry(3, 0.4).ry(3, 0.5), 2),
+    (lambda c: c.ry(3, 0.3).x(3).ry(3, 0.4), 0),
+    (lambda c: c.rx(3, 0.3).swap(3, 4).rx(4, 0.4).rx(3, 0.5), 0),
+])
+def test_rotation_merge_rules(build, merges):
+    """Which gates a rotation merges through, and which stop it."""
+    n = 12
+    prog = build(qlx.Circuit(n)).release()
+    order, _, info = qlx.plan_windowed(prog, "c128", qlx.exec_opts("windowed", specialize=True))
+    assert info["rotations_merged"] == merges
+    init = bf.random_state(n, np.random.default_rng(7))
+    a, _ = oracle.execute(n, prog.instrs, amps=init)
+    b, _ = oracle.execute(n, prog.instrs[order], amps=init)
+    np.testing.assert_allclose(b, a, atol=1e-12)
+
+
+def test_config2_merges_half_of_its_rotations():
+    """The 30-qubit depth-20 layered circuit: an RX on the target of a CX joins the RX before
+    that CX (294 of its 600 RX)."""
+    p = qlx.layered(30, 20, 1002)
+    o = qlx.exec_opts("windowed", specialize=True, relabel=True)
+    order, sweep, info = qlx.plan_windowed(p, "c128", o)
+    assert info["rotations_merged"] == 294 and info["gates_in_sweeps"] == 920
+    assert sorted(order.tolist()) == list(range(len(p)))

@@ -88,9 +88,11 @@ def test_cx_controlled_by_a_work_item_bit_costs_no_instructions():
         c.cx(q, q + 1)                  # a chain: no 4-qubit register set holds every control
     c.rx(7, 0.3).cx(6, 7).rx(7, 1.1)    # RX on a target with a pending X: applied as it is
     prog = c.release()
-    order, sweep_of, info = qlx.plan_windowed(prog, "c128", qlx.exec_opts("windowed", specialize=True))
+    # rotations kept as written (merged, the two RX would be one)
+    o = qlx.exec_opts("windowed", specialize=True, keep_rotations=True)
+    order, sweep_of, info = qlx.plan_windowed(prog, "c128", o)
     assert info["sweeps"] == 1 and sorted(order) == list(range(len(prog)))
-    src = qlx.sweep_source(prog, 0, "c128")
+    src = qlx.sweep_source(prog, 0, "c128", o)
     assert "__builtin_popcount(t &" in src              # a parity of work-item bits
     assert "fc ?" not in src                            # never applied as register swaps
     assert src.count("__builtin_fma(") == 2 * 8 * 4     # the two RX (four FMAs per pair), nothing else
