@@ -118,8 +118,13 @@ typedef struct qlx_exec_opts {
  * gate between the two on that qubit commutes with it: for RX an X or the target of a CX/CCX,
  * for RY a Y, for RZ a Z, S or a control of a CX/CCX/CSWAP. The state is the same up to
  * rounding (the angles are summed in double precision, as the cosines and sines are formed).
- * QLX_PLAN_KEEP_ROTATIONS applies every rotation as written. Plain execution never merges. */
+ * QLX_PLAN_KEEP_ROTATIONS applies every rotation as written. Plain execution never merges.
+ * Out-of-place specialised sweeps (QLX_SPECIALIZE | QLX_SPECIALIZE_RELABEL) may take windows
+ * that leave out the low qubits: a sweep's loads and stores stay coalesced on qubits it shares
+ * with the next sweep, which the store moves to the low address bits. The planner keeps such a
+ * plan when it needs fewer passes. QLX_PLAN_FIXED_WINDOWS keeps the low qubits in every window. */
 #define QLX_PLAN_KEEP_ROTATIONS 1u
+#define QLX_PLAN_FIXED_WINDOWS  2u
 
 /* qlx_exec_opts::specialize flags.
  * QLX_SPECIALIZE: compile the program's sweeps into straight-line kernels (hipRTC, once per

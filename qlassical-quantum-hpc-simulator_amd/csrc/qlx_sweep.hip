@@ -110,6 +110,8 @@ static Geometry geometry_for(int dtype, uint32_t nq, const qlx_exec_opts& o) {
     // low qubits kept in the window: one 128-byte cache line of amplitudes (8 x 16 B, 16 x 8 B)
     g.c = o.coalesce_qubits ? (int)o.coalesce_qubits : (dtype == QLX_C64 ? 4 : 3);
     if (g.c < 0 || g.c > g.logt) fail(QLX_EINVAL, "coalesce_qubits out of range");
+    g.free = (o.specialize & QLX_SPECIALIZE) && (o.specialize & QLX_SPECIALIZE_RELABEL) &&
+             !(o.plan_flags & QLX_PLAN_FIXED_WINDOWS);
     (void)nq;
     return g;
 }
@@ -440,7 +442,8 @@ struct Walk {
 // the walks that finish first, one whose last set leaves the low C qubits out of the
 // registers is preferred (otherwise the store needs one more re-layout).
 static std::vector<std::vector<int>> beam_regsets(const std::vector<PGate>& lg, const std::vector<std::vector<int>>& preds,
-                                                  const Geometry& geo, int width, bool runs) {
+                                                  const Geometry& geo, int width, bool runs,
+                                                  const std::vector<int>& st_low) {
     const int W = geo.w, RB = geo.rb, C = geo.c;
     struct Node {
         Walk w;
@@ -469,9 +472,10 @@ static std::vector<std::vector<int>> beam_regsets(const std::vector<PGate>& lg, 
         }
         return out;
     };
+    // the store's coalesced local bits (st_low) must be work-item bits of the last layout
     auto low_in = [&](const std::vector<int>& S) {
         for (int l : S)
-            if (l < C) return true;
+            if (std::find(st_low.begin(), st_low.end(), l) != st_low.end()) return true;
         return false;
     };
     std::vector<Node> beam;
@@ -508,7 +512,8 @@ static std::vector<std::vector<int>> beam_regsets(const std::vector<PGate>& lg, 
 // Register phases of one sweep (gates given in local bits). thorough: the beam search over
 // register sets (width 8, with the runs of consecutive qubits); otherwise the greedy set
 // alone at each step (width 1), which plans several times faster.
-static void plan_phases(PlanSweep& sw, const std::vector<PGate>& lg, const Geometry& geo, bool thorough) {
+static void plan_phases(PlanSweep& sw, const std::vector<PGate>& lg, const Geometry& geo, bool thorough,
+                        const std::vector<int>& st_low) {
     const int W = geo.w, RB = geo.rb, LT = geo.logt;
     const int n = (int)lg.size();
     std::vector<std::vector<int>> preds(n);
@@ -543,7 +548,7 @@ static void plan_phases(PlanSweep& sw, const std::vector<PGate>& lg, const Geome
         return r;
     };
 
-    const std::vector<std::vector<int>> sets = beam_regsets(lg, preds, geo, thorough ? 8 : 1, thorough);
+    const std::vector<std::vector<int>> sets = beam_regsets(lg, preds, geo, thorough ? 8 : 1, thorough, st_low);
     bool first = true;
     for (const std::vector<int>& regset : sets) {
         PlanPhase phase;
@@ -578,19 +583,42 @@ static void plan_phases(PlanSweep& sw, const std::vector<PGate>& lg, const Geome
         first = false;
     }
     if (!pending.empty()) fail(QLX_EHIP, "sweep phase planner left gates behind");
-    // last layout must keep the low C local bits as the low work-item bits (coalesced store)
+    // the last layout holds the store's coalesced local bits (st_low) as work-item bits
+    // 0..C-1: one more re-layout when its registers hold one of them, or when the sweep has a
+    // single phase (the load's layout) and the store's coalesced bits differ from the load's
+    auto is_st_low = [&](int l) { return std::find(st_low.begin(), st_low.end(), l) != st_low.end(); };
     std::vector<int> last = regset_of(sw.phases.back().st2loc);
     bool low_in_regs = false;
-    for (int l : last)
-        if (l < geo.c) low_in_regs = true;
-    if (low_in_regs) {
+    for (int l : last) low_in_regs = low_in_regs || is_st_low(l);
+    bool same_low = true;
+    for (size_t j = 0; j < st_low.size(); ++j) same_low = same_low && st_low[j] == (int)j;
+    if (low_in_regs || (sw.phases.size() == 1 && !same_low)) {
         std::vector<int> rs;
-        for (int l = W - 1; l >= geo.c && (int)rs.size() < RB; --l) rs.push_back(l);
-        std::sort(rs.begin(), rs.end());
+        if (low_in_regs) {
+            for (int l = W - 1; l >= 0 && (int)rs.size() < RB; --l)
+                if (!is_st_low(l)) rs.push_back(l);
+            std::sort(rs.begin(), rs.end());
+        } else {
+            rs = last;
+        }
         PlanPhase phase;
         phase.st2loc = make_layout(rs);
         sw.phases.push_back(std::move(phase));
     }
+    // order the last layout's work-item bits: st_low first, then the others ascending (a
+    // work-item bit named by an op of that phase is renumbered)
+    PlanPhase& lp = sw.phases.back();
+    std::vector<int> st2loc;
+    for (int l : st_low) st2loc.push_back(l);
+    for (int b = 0; b < LT; ++b)
+        if (!is_st_low(lp.st2loc[b])) st2loc.push_back(lp.st2loc[b]);
+    for (int b = LT; b < W; ++b) st2loc.push_back(lp.st2loc[b]);
+    if ((int)st2loc.size() != W) fail(QLX_EHIP, "sweep phase planner: store layout");
+    std::vector<int> loc2st(W);
+    for (int b = 0; b < W; ++b) loc2st[st2loc[b]] = b;
+    for (PlanOp& op : lp.ops)
+        if (op.tctl >= 0) op.tctl = loc2st[lp.st2loc[op.tctl]];
+    lp.st2loc = std::move(st2loc);
 }
 
 // Candidate windows for the next sweep, each the low C qubits plus W - C more: (a) the
@@ -643,10 +671,53 @@ static std::vector<int> choose_window_wide(const std::vector<PGate>& gates, cons
     return S;
 }
 
-// Candidate windows for the next sweep (see choose_window_wide), as qubit lists.
-static std::vector<std::vector<int>> window_candidates(const Walk& w, int nq, const Geometry& geo, int seed_gate) {
+static int overlap(const std::vector<int>& a, const std::vector<int>& b) {
+    int n = 0;
+    for (int x : a)
+        n += std::find(b.begin(), b.end(), x) != b.end();
+    return n;
+}
+
+// Candidate windows for the next sweep (see choose_window_wide), as qubit lists. Free
+// windows (Geometry::free; `prev` the previous window, if any): also every run of W
+// consecutive qubits and the greedy extension of the first ready gate's operands alone,
+// each kept only when it shares at least C qubits with `prev` (the previous sweep's store
+// is coalesced on shared qubits). The first window holds qubits 0..C-1 (the logical layout's
+// load is coalesced on them): the fixed candidates.
+static std::vector<std::vector<int>> window_candidates(const Walk& w, int nq, const Geometry& geo, int seed_gate,
+                                                       const std::vector<int>* prev = nullptr) {
     const int W = geo.w, C = std::min(geo.c, nq);
     std::vector<std::vector<int>> out;
+    if (geo.free && prev && !prev->empty()) {
+        std::vector<std::vector<int>> cands;
+        {
+            std::vector<char> s(nq, 0);
+            int have = 0;
+            const PGate& g = (*w.gates)[seed_gate];
+            for (int k = 0; k < g.nq; ++k)
+                if (!s[g.q[k]]) {
+                    s[g.q[k]] = 1;
+                    ++have;
+                }
+            const std::vector<char> allowed(nq, 1);
+            cands.push_back(choose_set(*w.gates, w.pending, w.done, *w.preds, nq, s, have, W, allowed));
+        }
+        for (int a = 0; a + W <= nq; ++a) {
+            std::vector<int> S;
+            for (int q = a; q < a + W; ++q) S.push_back(q);
+            cands.push_back(std::move(S));
+        }
+        Geometry fixed = geo;
+        fixed.free = false;
+        for (std::vector<int>& S : window_candidates(w, nq, fixed, seed_gate)) cands.push_back(std::move(S));
+        for (std::vector<int>& S : cands) {
+            if (overlap(S, *prev) < C) continue;
+            bool dup = false;
+            for (const std::vector<int>& o : out) dup = dup || o == S;
+            if (!dup) out.push_back(std::move(S));
+        }
+        return out;
+    }
     {
         std::vector<char> s(nq, 0);
         int have = 0;
@@ -666,6 +737,33 @@ static std::vector<std::vector<int>> window_candidates(const Walk& w, int nq, co
         for (int q = a; q < a + (W - C); ++q) S.push_back(q);
         if (S != out.front()) out.push_back(std::move(S));
     }
+    return out;
+}
+
+// The last window of a free-window walk (Geometry::free): qubits 0..C-1, the qubits its
+// gates `took` use, then qubits of S that the previous window holds, of S, and of the
+// previous window, up to W; sorted. Empty when 0..C-1 and the gates' qubits exceed W, or
+// when the result shares fewer than C qubits with the previous window.
+static std::vector<int> close_window(const std::vector<int>& S, const std::vector<int>& took,
+                                     const std::vector<PGate>& gates, const std::vector<int>& prev, int C, int W) {
+    std::vector<int> out;
+    auto add = [&](int q) {
+        if ((int)out.size() < W && std::find(out.begin(), out.end(), q) == out.end()) out.push_back(q);
+    };
+    for (int q = 0; q < C; ++q) add(q);
+    for (int gi : took)
+        for (int k = 0; k < gates[gi].nq; ++k) {
+            const int q = gates[gi].q[k];
+            if (std::find(out.begin(), out.end(), q) != out.end()) continue;
+            if ((int)out.size() == W) return {};
+            out.push_back(q);
+        }
+    for (int q : S)
+        if (std::find(prev.begin(), prev.end(), q) != prev.end()) add(q);
+    for (int q : S) add(q);
+    for (int q : prev) add(q);
+    if ((int)out.size() != W || overlap(out, prev) < C) return {};
+    std::sort(out.begin(), out.end());
     return out;
 }
 
@@ -690,10 +788,17 @@ static std::vector<std::vector<int>> beam_windows(const std::vector<PGate>& gate
         for (const Node& nd : beam) {
             const int seed = nd.w.seed_gate();
             if (seed < 0) fail(QLX_EHIP, "sweep planner: no runnable gate");
-            for (const std::vector<int>& S : window_candidates(nd.w, nq, geo, seed)) {
+            const std::vector<int>* prev = nd.wins.empty() ? nullptr : &nd.wins.back();
+            for (std::vector<int> S : window_candidates(nd.w, nq, geo, seed, prev)) {
                 Node c = nd;
-                if (c.w.take(S, nq).empty()) continue;
+                const std::vector<int> took = c.w.take(S, nq);
+                if (took.empty()) continue;
                 c.w.run_plain();
+                if (geo.free && c.w.pending.empty() && prev) {
+                    // the last window stores the logical layout: it must hold qubits 0..C-1
+                    S = close_window(S, took, gates, *prev, std::min(geo.c, nq), geo.w);
+                    if (S.empty()) continue;
+                }
                 c.wins.push_back(S);
                 next.push_back(std::move(c));
             }
@@ -892,15 +997,47 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
         // the sweep's gates, in program order
         const std::vector<int> taken = walk.take(S, (int)nq);
         if (taken.empty()) fail(QLX_EHIP, "sweep planner: empty sweep");
-        if (thorough) {
+        raw.push_back(RawStep{true, -1, S, taken});
+    }
+    // thorough: a sweep whose gates cost less than a pass as plain kernels runs them as plain
+    // kernels. With free windows only where the sweeps around it still chain: the first
+    // and the last sweep hold qubits 0..C-1, and consecutive sweeps share C qubits.
+    if (thorough) {
+        const int Cq = std::min(geo.c, (int)nq);
+        auto has_low = [&](const std::vector<int>& S) {
+            for (int q = 0; q < Cq; ++q)
+                if (std::find(S.begin(), S.end(), q) == S.end()) return false;
+            return true;
+        };
+        std::vector<char> plain(raw.size(), 0);
+        for (size_t k = 0; k < raw.size(); ++k) {
+            if (!raw[k].sweep) continue;
             double cost = 0.0;
-            for (int gi : taken) cost += plain_passes(gates[gi].type);
-            if (cost < 1.0) {
-                for (int gi : taken) raw.push_back(RawStep{false, gi, {}, {}});
+            for (int gi : raw[k].taken) cost += plain_passes(gates[gi].type);
+            if (cost >= 1.0) continue;
+            bool ok = true;
+            if (geo.free) {
+                int kp = (int)k - 1, kn = (int)k + 1;
+                while (kp >= 0 && (!raw[kp].sweep || plain[kp])) --kp;
+                while (kn < (int)raw.size() && !raw[kn].sweep) ++kn;
+                if (kp < 0)
+                    ok = kn == (int)raw.size() || has_low(raw[kn].S);
+                else if (kn == (int)raw.size())
+                    ok = has_low(raw[kp].S);
+                else
+                    ok = overlap(raw[kp].S, raw[kn].S) >= Cq;
+            }
+            plain[k] = ok;
+        }
+        std::vector<RawStep> out;
+        for (size_t k = 0; k < raw.size(); ++k) {
+            if (!plain[k]) {
+                out.push_back(std::move(raw[k]));
                 continue;
             }
+            for (int gi : raw[k].taken) out.push_back(RawStep{false, gi, {}, {}});
         }
-        raw.push_back(RawStep{true, -1, S, taken});
+        raw.swap(out);
     }
 
     // 2. circuit-specialised sweeps: even out the arithmetic (see balance_sweeps)
@@ -920,16 +1057,66 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
                        kBalanceWork * geo.elem / 16.0);
     }
 
-    // 3. the steps: plain kernels, and sweeps with their register phases
-    for (RawStep& st : raw) {
+    // 3. the coalesced qubits of each sweep's load and store: qubits 0..C-1 at the first load
+    // and the last store (the logical layout); between two sweeps, C qubits both windows hold,
+    // kept from one sweep to the next while they can be. With windows that all hold 0..C-1 these
+    // are 0..C-1 throughout.
+    const int C = std::min(geo.c, (int)nq);
+    std::vector<std::vector<int>> ld_low(raw.size()), st_low(raw.size());
+    {
+        std::vector<int> L;
+        for (int q = 0; q < C; ++q) L.push_back(q);
+        auto holds = [](const std::vector<int>& S, int q) { return std::find(S.begin(), S.end(), q) != S.end(); };
+        for (size_t k = 0; k < raw.size(); ++k) {
+            if (!raw[k].sweep) continue;
+            size_t k2 = k + 1;
+            while (k2 < raw.size() && !raw[k2].sweep) ++k2;
+            std::vector<int> st;
+            if (k2 == raw.size()) {
+                for (int q = 0; q < C; ++q) st.push_back(q);
+            } else {
+                const std::vector<int>& N = raw[k2].S;
+                for (int q : L)
+                    if (holds(N, q)) st.push_back(q);
+                std::vector<int> rest;
+                for (int q : raw[k].S)
+                    if (holds(N, q) && !holds(st, q)) rest.push_back(q);
+                std::sort(rest.begin(), rest.end());
+                for (size_t i = 0; (int)st.size() < C && i < rest.size(); ++i) st.push_back(rest[i]);
+            }
+            std::sort(st.begin(), st.end());
+            bool ok = (int)st.size() == C;
+            for (int q : st) ok = ok && holds(raw[k].S, q);
+            for (int q : L) ok = ok && holds(raw[k].S, q);
+            if (!ok) fail(QLX_EHIP, "sweep planner: consecutive windows share fewer than the coalesced qubits");
+            ld_low[k] = L;
+            st_low[k] = st;
+            L = st;
+        }
+    }
+
+    // 4. the steps: plain kernels, and sweeps with their register phases
+    for (size_t k = 0; k < raw.size(); ++k) {
+        RawStep& st = raw[k];
         if (!st.sweep) {
             plain_step(st.gate);
             continue;
         }
-        const std::vector<int>& S = st.S;
+        // local bits: the load's coalesced qubits first, then the rest ascending
+        std::vector<int> S = ld_low[k];
+        {
+            std::vector<int> rest;
+            for (int q : st.S)
+                if (std::find(S.begin(), S.end(), q) == S.end()) rest.push_back(q);
+            std::sort(rest.begin(), rest.end());
+            S.insert(S.end(), rest.begin(), rest.end());
+        }
         const std::vector<int>& taken = st.taken;
         std::vector<int> loc(nq, -1);
         for (int j = 0; j < W; ++j) loc[S[j]] = j;
+        std::vector<int> st_local;
+        for (int q : st_low[k]) st_local.push_back(loc[q]);
+        std::sort(st_local.begin(), st_local.end());
         std::vector<PGate> lg;
         for (int gi : taken) {
             PGate l = gates[gi];
@@ -939,7 +1126,8 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
         }
         PlanSweep sw;
         sw.S = S;
-        plan_phases(sw, lg, geo, thorough);
+        sw.st_low = st_low[k];
+        plan_phases(sw, lg, geo, thorough, st_local);
         const int si = (int)plan->sweeps.size();
         for (const PlanPhase& ph : sw.phases)
             for (const PlanOp& op : ph.ops) {
@@ -1110,13 +1298,24 @@ std::vector<DenseWindow> plan_dense_windows(uint32_t nq, const qlx_gate_instr* i
 std::unique_ptr<SweepPlan> build_plan_geo(uint32_t nq, const Geometry& geo, const qlx_gate_instr* instrs, size_t n,
                                           const double* pool, const qlx_matrix_entry* meta,
                                           const qlx_exec_opts& opts, bool thorough) {
-    if (!thorough) return build_plan_with(nq, geo, instrs, n, pool, meta, opts, Chooser::Wide, nullptr, false);
-    auto best = build_plan_with(nq, geo, instrs, n, pool, meta, opts, Chooser::Greedy, nullptr, true);
+    Geometry fixed = geo;
+    fixed.free = false;
+    if (!thorough) return build_plan_with(nq, fixed, instrs, n, pool, meta, opts, Chooser::Wide, nullptr, false);
+    auto best = build_plan_with(nq, fixed, instrs, n, pool, meta, opts, Chooser::Greedy, nullptr, true);
     if (best->sweeps.size() <= 1) return best;
-    for (Chooser c : {Chooser::Wide, Chooser::Given}) {
-        auto p = build_plan_with(nq, geo, instrs, n, pool, meta, opts, c, nullptr, true);
+    auto consider = [&](std::unique_ptr<SweepPlan> p) {
         if (p->passes < best->passes - 1e-9 || (p->passes < best->passes + 1e-9 && p->phases < best->phases))
             best = std::move(p);
+    };
+    for (Chooser c : {Chooser::Wide, Chooser::Given})
+        consider(build_plan_with(nq, fixed, instrs, n, pool, meta, opts, c, nullptr, true));
+    if (geo.free) {
+        // free windows (out-of-place sweeps): kept when they need fewer passes; a window
+        // sequence whose consecutive windows share too few qubits is not a plan
+        try {
+            consider(build_plan_with(nq, geo, instrs, n, pool, meta, opts, Chooser::Given, nullptr, true));
+        } catch (const Error&) {
+        }
     }
     return best;
 }
@@ -1239,7 +1438,8 @@ void assign_layouts(SweepPlan& p, uint32_t nq) {
         if (si + 1 == (int)p.sweeps.size()) {
             for (int q = 0; q < n; ++q) next[q] = q;
         } else {
-            // low C qubits stay; then the next window, this window, the rest in current order
+            // the store's coalesced qubits on address bits 0..C-1; then the next window, this
+            // window, the rest in current order
             std::vector<char> placed(n, 0);
             int at = 0;
             auto place = [&](int q) {
@@ -1247,7 +1447,7 @@ void assign_layouts(SweepPlan& p, uint32_t nq) {
                 placed[q] = 1;
                 next[q] = at++;
             };
-            for (int q = 0; q < g.c && q < n; ++q) place(q);
+            for (int q : sw.st_low) place(q);
             for (int q : p.sweeps[si + 1].S) place(q);
             for (int q : sw.S) place(q);
             std::vector<int> rest;
@@ -1256,8 +1456,12 @@ void assign_layouts(SweepPlan& p, uint32_t nq) {
             std::sort(rest.begin(), rest.end(), [&](int a, int b) { return pos[a] < pos[b]; });
             for (int q : rest) place(q);
         }
-        for (int q = 0; q < g.c && q < n; ++q)
-            if (pos[q] != q || next[q] != q) fail(QLX_EHIP, "qubit layout moved a coalesced qubit");
+        // loads and stores coalesced: the load's coalesced qubits (the window's first local
+        // bits) and the store's sit on address bits 0..C-1
+        const int C = std::min(g.c, n);
+        if ((int)sw.st_low.size() != C) fail(QLX_EHIP, "qubit layout: no coalesced store qubits");
+        for (int j = 0; j < C; ++j)
+            if (pos[sw.S[j]] >= C || next[sw.st_low[j]] >= C) fail(QLX_EHIP, "qubit layout: uncoalesced load or store");
         // chunk-index bits: the qubits outside the window, in the order of their load position
         std::vector<char> inw(n, 0);
         for (int q : sw.S) inw[q] = 1;

@@ -65,7 +65,8 @@ struct DevPhase {
 
 struct DevSweep {
     uint32_t n_phases, phase_begin;
-    uint32_t S[kMaxW];           // global qubit of local bit j (ascending)
+    uint32_t S[kMaxW];           // global qubit of local bit j (ascending, except in free-window
+                                 // plans, which run out of place only)
     uint32_t ld_tglob[kMaxLogT]; // global qubit at work-item bit b in the first layout
     uint32_t st_tglob[kMaxLogT]; // ... in the last layout
     uint64_t ld_roff[kMaxR];     // global offset of register i in the first layout
@@ -407,6 +408,10 @@ __device__ __forceinline__ void run_chunk(A* __restrict__ amp, const DevSweep* _
 struct Geometry {
     int logt, rb, w, c;
     int elem = 16;  // bytes per amplitude
+    // out-of-place sweeps: a window need not hold qubits 0..c-1. Each sweep's loads and stores
+    // are coalesced on c qubits it holds that sit on address bits 0..c-1 (the first sweep loads
+    // and the last stores the logical layout, so their windows hold qubits 0..c-1)
+    bool free = false;
 };
 
 struct PlanOp {
@@ -423,7 +428,12 @@ struct PlanPhase {
 };
 
 struct PlanSweep {
-    std::vector<int> S;  // global qubits, ascending, size W
+    // global qubits, size W: local bit j is S[j]. The first c are the qubits on address bits
+    // 0..c-1 at the load (ascending), then the rest ascending
+    std::vector<int> S;
+    // the c qubits on address bits 0..c-1 at the store (the next sweep's coalesced qubits),
+    // ascending; they are the low work-item bits of the last phase
+    std::vector<int> st_low;
     std::vector<PlanPhase> phases;
     // out-of-place sweeps (SweepPlan::relabel): address bit of chunk-index bit j at the load
     // and at the store, and of local bit l at the load and at the store

@@ -32,6 +32,7 @@ QLX_SPECIALIZE = 1
 QLX_SPECIALIZE_RELABEL = 2
 QLX_SPECIALIZE_ASYNC = 4
 QLX_PLAN_KEEP_ROTATIONS = 1
+QLX_PLAN_FIXED_WINDOWS = 2
 QLX_DENSE_AUTO = 0
 QLX_DENSE_VALU = 1
 QLX_DENSE_MFMA_BF16 = 2

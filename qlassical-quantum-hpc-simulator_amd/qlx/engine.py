@@ -35,7 +35,8 @@ DENSE = {"auto": _abi.QLX_DENSE_AUTO, "valu": _abi.QLX_DENSE_VALU, "bf16": _abi.
 def exec_opts(mode: str = "plain", window_qubits: int = 0, coalesce_qubits: int = 0,
               gswap: str = "permute", register_qubits: int = 0, dense: str = "auto",
               sweep_workgroups: int = 0, specialize: bool = False, relabel: bool = False,
-              background: bool = False, keep_rotations: bool = False) -> _abi.ExecOpts:
+              background: bool = False, keep_rotations: bool = False,
+              fixed_windows: bool = False) -> _abi.ExecOpts:
     """qlx_exec_opts. mode: 'plain', 'windowed', 'dense' or 'auto' (the C++ HIPEngine's default:
     window sweeps, with specialised relabelled sweeps compiled in the background for states of
     >= 20 qubits). dense: arithmetic of 4/5-qubit dense gates on complex64 states
@@ -44,7 +45,9 @@ def exec_opts(mode: str = "plain", window_qubits: int = 0, coalesce_qubits: int 
     out-of-place sweeps that re-lay out the qubits between two buffers. background (windowed,
     with specialize): compile on a host thread and run the generic kernel until ready.
     keep_rotations (window sweeps): apply every rotation as written instead of merging
-    same-axis rotations through the gates they commute with (QLX_PLAN_KEEP_ROTATIONS)."""
+    same-axis rotations through the gates they commute with (QLX_PLAN_KEEP_ROTATIONS).
+    fixed_windows (relabelled sweeps): every window holds the low qubits
+    (QLX_PLAN_FIXED_WINDOWS)."""
     if relabel and not specialize and mode != "dense":
         raise ValueError("relabel needs specialize")
     if background and not specialize:
@@ -60,7 +63,8 @@ def exec_opts(mode: str = "plain", window_qubits: int = 0, coalesce_qubits: int 
     o.register_qubits = register_qubits
     o.coalesce_qubits = coalesce_qubits
     o.gswap_mode = {"permute": _abi.QLX_GSWAP_PERMUTE, "noop": _abi.QLX_GSWAP_NOOP}[gswap]
-    o.plan_flags = _abi.QLX_PLAN_KEEP_ROTATIONS if keep_rotations else 0
+    o.plan_flags = ((_abi.QLX_PLAN_KEEP_ROTATIONS if keep_rotations else 0) |
+                    (_abi.QLX_PLAN_FIXED_WINDOWS if fixed_windows else 0))
     return o
 
 

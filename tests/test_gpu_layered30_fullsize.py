@@ -32,7 +32,7 @@ def test_layered30_specialised_relabelled_matches_oracle(gpu):
     oracle.execute_inplace(N, prog.instrs, host)
     a.sync()
     stats = a.last_stats()
-    assert stats["sweeps_full"] >= 10 and stats["gates_applied"] == len(prog), stats
+    assert stats["sweeps_full"] >= 5 and stats["gates_applied"] == len(prog), stats
     want = gpu.State(dtype="c128", amps=host)
     del host
     err = a.max_abs_diff(want)

@@ -221,3 +221,40 @@ def test_config2_merges_half_of_its_rotations():
     order, sweep, info = qlx.plan_windowed(p, "c128", o)
     assert info["rotations_merged"] == 294 and info["gates_in_sweeps"] == 920
     assert sorted(order.tolist()) == list(range(len(p)))
+
+
+@pytest.mark.parametrize("case", ["layered20", "layered24", "random0", "random1", "random2"])
+def test_free_window_plans_are_valid_reorderings(case):
+    """Out-of-place specialised sweeps take windows that leave out qubits 0-2 when that needs
+    fewer passes: the execution order is still a valid reordering (oracle), every sweep's gates
+    touch only W qubits, and the pass count is no higher than with fixed windows."""
+    from test_oracle import random_program
+    rng = np.random.default_rng(hash(case) % 1000)
+    if case.startswith("layered"):
+        n = int(case[7:])
+        prog = qlx.layered(n, 20, 8)
+    else:
+        n = 16
+        ins, pool, meta = random_program(n, 200, np.random.default_rng(int(case[6:]) + 50))
+        prog = qlx.Program(n, ins, pool if pool is not None else np.zeros(0, complex),
+                           meta if meta is not None else np.zeros(0, qlx.META_DTYPE))
+    o = qlx.exec_opts("windowed", specialize=True, relabel=True)
+    order, sweep, info = qlx.plan_windowed(prog, "c128", o)
+    fixed = qlx.plan_windowed(prog, "c128", qlx.exec_opts("windowed", specialize=True, relabel=True,
+                                                          fixed_windows=True))[2]
+    assert info["sweeps"] + info["plain_steps"] <= fixed["sweeps"] + fixed["plain_steps"]
+    if case.startswith("layered"):
+        assert info["sweeps"] < fixed["sweeps"]
+    assert sorted(order.tolist()) == list(range(len(prog)))
+    init = bf.random_state(n, np.random.default_rng(n))
+    pl = prog.pool if prog.pool.size else None
+    me = prog.meta if len(prog.meta) else None
+    a, _ = oracle.execute(n, prog.instrs, pl, me, amps=init)
+    b, _ = oracle.execute(n, prog.instrs[order], pl, me, amps=init)
+    np.testing.assert_allclose(b, a, atol=1e-12)
+    for s in set(sweep.tolist()) - {-1}:
+        qs = set()
+        for g in prog.instrs[order[sweep == s]]:
+            k = g["flags_arity"] & 15
+            qs |= set(int(x) for x in g["qubits"][:k])
+        assert len(qs) <= info["window_qubits"]

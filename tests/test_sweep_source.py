@@ -194,15 +194,27 @@ def _bits(expr):
     return [int(k) for k in re.findall(r"<< (\d+)\)", expr)]
 
 
-@pytest.mark.parametrize("prog_name", ["layered", "hea", "random"])
+def _work_bits(expr):
+    """(work-item bit, address bit) pairs of a generated `tl` / `ts` expression."""
+    return [(int(b), int(k)) for b, k in re.findall(r"\(t >> (\d+)\) & 1u\) << (\d+)\)", expr)]
+
+
+@pytest.mark.parametrize("prog_name", ["layered", "hea", "random", "layered24"])
 def test_relabelled_sweeps_address_every_amplitude_once(prog_name):
     """Out-of-place sweeps (relabel): in every generated kernel the load address bits (chunk
     index bits, work-item bits, register bits) and the store address bits each form a
     permutation of the state's qubits, so loads and stores are in bounds and bijective; the
-    kernel reads `src` and writes `dst`."""
+    kernel reads `src` and writes `dst`; work-item bits 0-2 sit on address bits 0-2 at the
+    load and at the store (128-byte runs), also when windows leave out qubits 0-2 (the 24-qubit
+    layered circuit, whose plan takes such windows)."""
     from test_oracle import random_program
-    n = 17
-    if prog_name == "layered":
+    n = 24 if prog_name == "layered24" else 17
+    if prog_name == "layered24":
+        prog = qlx.layered(n, 20, 8)
+        fixed = qlx.exec_opts("windowed", specialize=True, relabel=True, fixed_windows=True)
+        assert qlx.plan_windowed(prog, "c128", fixed)[2]["sweeps"] > qlx.plan_windowed(
+            prog, "c128", qlx.exec_opts("windowed", specialize=True, relabel=True))[2]["sweeps"]
+    elif prog_name == "layered":
         prog = qlx.layered(n, 8, 5)
     elif prog_name == "hea":
         prog = qlx.hea(n, 4)
@@ -232,8 +244,10 @@ def test_relabelled_sweeps_address_every_amplitude_once(prog_name):
             assert len(regs) == 4 and sorted(offs) == sorted(
                 sum(1 << regs[k] for k in range(4) if (i >> k) & 1) for i in range(16))
             assert sorted(chunk + work + regs) == list(range(n)), (s, base)
-        # the low 3 qubits are the low work-item bits at both ends (128-byte runs)
-        assert re.search(r"tl = 0ull \| \(\(unsigned long long\)\(\(t >> 0\) & 1u\) << 0\) \| \(\(unsigned long long\)\(\(t >> 1\) & 1u\) << 1\) \| \(\(unsigned long long\)\(\(t >> 2\) & 1u\) << 2\)", src)
+        # work-item bits 0-2 are address bits 0-2 at both ends (128-byte runs)
+        for thr in ("tl", "ts"):
+            pairs = _work_bits(re.search(rf"const unsigned long long {thr} = ([^;]+);", src).group(1))
+            assert sorted(k for b, k in pairs if b < 3) == [0, 1, 2], (s, thr, pairs)
 
 
 @pytest.mark.skipif(not os.path.exists(HIPRTC), reason="hipRTC not installed")
