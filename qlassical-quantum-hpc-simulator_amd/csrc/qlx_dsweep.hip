@@ -42,18 +42,22 @@ constexpr int kDChunk = 1 << kDW;
 
 constexpr int kMaxChunkBits = 32;    // chunk-index bits of one launch (2^32 work-groups at most)
 
-// A sweep's addressing: local bit l of the chunk (local bit l <-> window qubit S[l], S
-// ascending) and chunk-index bit j (the qubits outside the window, in the order of their load
-// address bits) sit at address bit ld_*[.] when the chunk is read and st_*[.] when it is
-// written. In place both layouts are the identity; with relabelling (out of place, between two
-// buffers) the store layout puts the next window on the low address bits.
+// A sweep's addressing: local bit l of the chunk (local bit l <-> window qubit S[l]; the
+// first kDC are the qubits on address bits 0..kDC-1 at the load) and chunk-index bit j (the
+// qubits outside the window, in the order of their load address bits) sit at address bit
+// ld_l[l] / ld_c[j] when the chunk is read. The store walks the chunk in its own order: bit j
+// of the store index is local bit st_loc[j] (the first kDC: the qubits on address bits
+// 0..kDC-1 at the store) and sits at address bit st_l[j]; chunk-index bit j at st_c[j]. In place
+// both layouts are the identity; with relabelling (out of place, between two buffers) the store
+// layout puts the next window on the low address bits.
 struct DenseSweepDesc {
     uint32_t op_begin, n_ops;
     uint32_t nc;                    // chunk-index bits (state qubits - kDW)
     uint32_t pad;
     uint32_t ld_l[kDW], st_l[kDW];
     uint32_t ld_c[kMaxChunkBits], st_c[kMaxChunkBits];
-    uint32_t pad2[2];
+    uint32_t st_loc[kDW];
+    uint32_t pad2[1];
 };
 static_assert(sizeof(DenseSweepDesc) % 16 == 0, "descriptor layout");
 static_assert(offsetof(DenseSweepDesc, st_c) == offsetof(DenseSweepDesc, ld_c) + 4 * kMaxChunkBits,
@@ -374,19 +378,34 @@ __global__ void __launch_bounds__(kDThreads) __attribute__((amdgpu_waves_per_eu(
         __syncthreads();
     }
     // the store's register-bit positions are read here (volatile): read with the others, the 32
-    // register offsets would hold scalar registers through every gate
+    // register offsets would hold scalar registers through every gate. Store index t + 256 r:
+    // bit j is local bit st_loc[j] (the identity unless the store's coalesced qubits differ
+    // from the load's); swz is linear over GF(2), so its work-item and register parts are
+    // swizzled apart
     const volatile uint32_t* vd = reinterpret_cast<const volatile uint32_t*>(sw);
     constexpr int kStL = offsetof(DenseSweepDesc, st_l) / 4;
-    uint32_t sp[5];
+    constexpr int kStLoc = offsetof(DenseSweepDesc, st_loc) / 4;
+    uint32_t sp[5], sl[5];
 #pragma unroll
-    for (int l = 0; l < 5; ++l) sp[l] = vd[kStL + 8 + l];
+    for (int l = 0; l < 5; ++l) {
+        sp[l] = vd[kStL + 8 + l];
+        sl[l] = vd[kStLoc + 8 + l];
+    }
+    uint32_t it = 0;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) it |= ((t >> l) & 1u) << vd[kStLoc + l];
+    const uint32_t st_t = swz(it);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         uint64_t ro = 0;
+        uint32_t ir = 0;
 #pragma unroll
         for (int l = 0; l < 5; ++l)
-            if ((r >> l) & 1) ro |= 1ull << sp[l];
-        float2 v = lds[swz(t + kDThreads * r)];
+            if ((r >> l) & 1) {
+                ro |= 1ull << sp[l];
+                ir |= 1u << sl[l];
+            }
+        float2 v = lds[st_t ^ swz(ir)];
         if constexpr (FORM == FORM_F16) v = float2{v.x * kF16Unscale, v.y * kF16Unscale};
         dst[sbase | stl | ro] = v;
     }
@@ -601,13 +620,16 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
         for (int q = 0; q < nq; ++q) pos[q] = q;
         for (size_t si = 0; si < wins.size(); ++si) {
             const DenseWindow& w = wins[si];
-            if (w.S.size() != (size_t)kDW) fail(QLX_EHIP, "dense sweep window of the wrong size");
-            for (int l = 0; l < kDC; ++l)
-                if (w.S[l] != l) fail(QLX_EHIP, "dense sweep window without the low qubits");
+            if (w.S.size() != (size_t)kDW || w.st_low.size() != (size_t)kDC)
+                fail(QLX_EHIP, "dense sweep window of the wrong size");
+            if (!relabel)
+                for (int l = 0; l < kDC; ++l)
+                    if (w.S[l] != l || w.st_low[l] != l) fail(QLX_EHIP, "dense sweep window without the low qubits");
             std::vector<int> next(nq);
             for (int q = 0; q < nq; ++q) next[q] = q;
             if (relabel && si + 1 < wins.size()) {
-                // the low qubits stay; then the next window, this window, the rest in load order
+                // the store's coalesced qubits on address bits 0..kDC-1; then the next window,
+                // this window, the rest in load order
                 std::vector<char> placed(nq, 0);
                 int at = 0;
                 auto place = [&](int q) {
@@ -616,7 +638,7 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
                         next[q] = at++;
                     }
                 };
-                for (int q = 0; q < kDC; ++q) place(q);
+                for (int q : w.st_low) place(q);
                 for (int q : wins[si + 1].S) place(q);
                 for (int q : w.S) place(q);
                 std::vector<int> rest;
@@ -638,13 +660,23 @@ void execute_dense_sweeps(qlx_state* st, const qlx_gate_instr* instrs, size_t n,
                 d.ld_c[j] = (uint32_t)pos[outside[j]];
                 d.st_c[j] = (uint32_t)next[outside[j]];
             }
+            // store order: the store's coalesced local bits first, then the others ascending
+            std::vector<int> sorder;
+            for (int q : w.st_low)
+                sorder.push_back((int)(std::find(w.S.begin(), w.S.end(), q) - w.S.begin()));
+            for (int l = 0; l < kDW; ++l)
+                if (std::find(w.st_low.begin(), w.st_low.end(), w.S[l]) == w.st_low.end()) sorder.push_back(l);
+            if (sorder.size() != (size_t)kDW) fail(QLX_EHIP, "dense sweep store qubits outside the window");
             for (int l = 0; l < kDW; ++l) {
                 d.ld_l[l] = (uint32_t)pos[w.S[l]];
-                d.st_l[l] = (uint32_t)next[w.S[l]];
+                d.st_loc[l] = (uint32_t)sorder[l];
+                d.st_l[l] = (uint32_t)next[w.S[sorder[l]]];
             }
+            // coalesced: local bits 0..kDC-1 at the load and store-index bits 0..kDC-1 at the
+            // store sit on address bits 0..kDC-1
             for (int l = 0; l < kDC; ++l)
-                if (d.ld_l[l] != (uint32_t)l || d.st_l[l] != (uint32_t)l)
-                    fail(QLX_EHIP, "dense sweep layout moved a low qubit");
+                if (d.ld_l[l] >= (uint32_t)kDC || d.st_l[l] >= (uint32_t)kDC)
+                    fail(QLX_EHIP, "dense sweep layout leaves a load or store uncoalesced");
             pos = next;
             d.op_begin = (uint32_t)P.ops.size();
             std::vector<int> loc(st->nq, -1);

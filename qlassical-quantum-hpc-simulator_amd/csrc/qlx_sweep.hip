@@ -889,6 +889,43 @@ static void balance_sweeps(const std::vector<PGate>& gates, const std::vector<st
     }
 }
 
+// The coalesced qubits of a sequence of out-of-place sweeps: ld[k], on address bits 0..C-1
+// when sweep k loads, and st[k], when it stores (both ascending). The first load and the last
+// store are the logical layout's (qubits 0..C-1); between two sweeps, C qubits both windows
+// hold, the previous ones kept while the next window holds them. Fails when the first or last
+// window lacks qubits 0..C-1 or two consecutive windows share fewer than C qubits.
+static void coalesce_chain(const std::vector<std::vector<int>>& wins, int C, std::vector<std::vector<int>>& ld,
+                    std::vector<std::vector<int>>& st) {
+    auto holds = [](const std::vector<int>& S, int q) { return std::find(S.begin(), S.end(), q) != S.end(); };
+    ld.assign(wins.size(), {});
+    st.assign(wins.size(), {});
+    std::vector<int> L;
+    for (int q = 0; q < C; ++q) L.push_back(q);
+    for (size_t k = 0; k < wins.size(); ++k) {
+        std::vector<int> out;
+        if (k + 1 == wins.size()) {
+            for (int q = 0; q < C; ++q) out.push_back(q);
+        } else {
+            const std::vector<int>& N = wins[k + 1];
+            for (int q : L)
+                if (holds(N, q)) out.push_back(q);
+            std::vector<int> rest;
+            for (int q : wins[k])
+                if (holds(N, q) && !holds(out, q)) rest.push_back(q);
+            std::sort(rest.begin(), rest.end());
+            for (size_t i = 0; (int)out.size() < C && i < rest.size(); ++i) out.push_back(rest[i]);
+        }
+        std::sort(out.begin(), out.end());
+        bool ok = (int)out.size() == C;
+        for (int q : out) ok = ok && holds(wins[k], q);
+        for (int q : L) ok = ok && holds(wins[k], q);
+        if (!ok) fail(QLX_EHIP, "sweep planner: consecutive windows share fewer than the coalesced qubits");
+        ld[k] = L;
+        st[k] = out;
+        L = out;
+    }
+}
+
 enum class Chooser { Greedy, Wide, Given };
 
 // Builds the plan along one walk. Windows come from the greedy chooser, the wide chooser,
@@ -1061,38 +1098,17 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
     // and the last store (the logical layout); between two sweeps, C qubits both windows hold,
     // kept from one sweep to the next while they can be. With windows that all hold 0..C-1 these
     // are 0..C-1 throughout.
-    const int C = std::min(geo.c, (int)nq);
     std::vector<std::vector<int>> ld_low(raw.size()), st_low(raw.size());
     {
-        std::vector<int> L;
-        for (int q = 0; q < C; ++q) L.push_back(q);
-        auto holds = [](const std::vector<int>& S, int q) { return std::find(S.begin(), S.end(), q) != S.end(); };
-        for (size_t k = 0; k < raw.size(); ++k) {
-            if (!raw[k].sweep) continue;
-            size_t k2 = k + 1;
-            while (k2 < raw.size() && !raw[k2].sweep) ++k2;
-            std::vector<int> st;
-            if (k2 == raw.size()) {
-                for (int q = 0; q < C; ++q) st.push_back(q);
-            } else {
-                const std::vector<int>& N = raw[k2].S;
-                for (int q : L)
-                    if (holds(N, q)) st.push_back(q);
-                std::vector<int> rest;
-                for (int q : raw[k].S)
-                    if (holds(N, q) && !holds(st, q)) rest.push_back(q);
-                std::sort(rest.begin(), rest.end());
-                for (size_t i = 0; (int)st.size() < C && i < rest.size(); ++i) st.push_back(rest[i]);
+        std::vector<std::vector<int>> wins, ld, st;
+        for (const RawStep& r : raw)
+            if (r.sweep) wins.push_back(r.S);
+        coalesce_chain(wins, std::min(geo.c, (int)nq), ld, st);
+        for (size_t k = 0, i = 0; k < raw.size(); ++k)
+            if (raw[k].sweep) {
+                ld_low[k] = ld[i];
+                st_low[k] = st[i++];
             }
-            std::sort(st.begin(), st.end());
-            bool ok = (int)st.size() == C;
-            for (int q : st) ok = ok && holds(raw[k].S, q);
-            for (int q : L) ok = ok && holds(raw[k].S, q);
-            if (!ok) fail(QLX_EHIP, "sweep planner: consecutive windows share fewer than the coalesced qubits");
-            ld_low[k] = L;
-            st_low[k] = st;
-            L = st;
-        }
     }
 
     // 4. the steps: plain kernels, and sweeps with their register phases
@@ -1275,12 +1291,33 @@ std::vector<DenseWindow> plan_dense_windows(uint32_t nq, const qlx_gate_instr* i
     geo.c = c;
     std::vector<DenseWindow> out;
     if (gates.empty()) return out;
-    const std::vector<std::vector<int>> wins = beam_windows(gates, preds, (int)nq, geo, 8);
+    // out-of-place sweeps may take free windows (Geometry::free), kept when fewer
+    std::vector<std::vector<int>> wins = beam_windows(gates, preds, (int)nq, geo, 8);
+    if ((opts.specialize & QLX_SPECIALIZE_RELABEL) && !(opts.plan_flags & QLX_PLAN_FIXED_WINDOWS)) {
+        Geometry fg = geo;
+        fg.free = true;
+        try {
+            std::vector<std::vector<int>> fw = beam_windows(gates, preds, (int)nq, fg, 8);
+            std::vector<std::vector<int>> ld, st;
+            coalesce_chain(fw, std::min(c, (int)nq), ld, st);
+            if (fw.size() < wins.size()) wins = std::move(fw);
+        } catch (const Error&) {
+        }
+    }
+    std::vector<std::vector<int>> ld, st;
+    coalesce_chain(wins, std::min(c, (int)nq), ld, st);
     Walk walk(gates, preds);
-    for (const std::vector<int>& S : wins) {
+    for (size_t k = 0; k < wins.size(); ++k) {
         DenseWindow d;
-        d.S = S;
-        for (int gi : walk.take(S, (int)nq)) d.instrs.push_back(gates[gi].src);
+        // local bits: the load's coalesced qubits first, then the rest ascending
+        d.S = ld[k];
+        std::vector<int> rest;
+        for (int q : wins[k])
+            if (std::find(d.S.begin(), d.S.end(), q) == d.S.end()) rest.push_back(q);
+        std::sort(rest.begin(), rest.end());
+        d.S.insert(d.S.end(), rest.begin(), rest.end());
+        d.st_low = st[k];
+        for (int gi : walk.take(wins[k], (int)nq)) d.instrs.push_back(gates[gi].src);
         if (d.instrs.empty()) fail(QLX_EHIP, "dense sweep planner: empty sweep");
         out.push_back(std::move(d));
     }

@@ -466,10 +466,14 @@ struct SweepPlan {
 };
 
 // Windows of dense sweeps (QLX_EXEC_DENSE_SWEEPS): every instruction of arity <= 5 runs
-// inside a window of `w` qubits that holds the low `c` qubits; the windows come from the
-// beam search over window sequences. Each window lists its instructions in program order.
+// inside a window of `w` qubits that holds the low `c` qubits (out-of-place sweeps: the `c`
+// qubits its load and its store are coalesced on); the windows come from the beam search
+// over window sequences. Each window lists its instructions in program order.
 struct DenseWindow {
-    std::vector<int> S;       // window qubits, ascending
+    // window qubits: the c qubits on address bits 0..c-1 at the load first (ascending), then
+    // the rest ascending
+    std::vector<int> S;
+    std::vector<int> st_low;  // the c qubits on address bits 0..c-1 at the store, ascending
     std::vector<int> instrs;  // indices into the instruction stream
 };
 std::vector<DenseWindow> plan_dense_windows(uint32_t nq, const qlx_gate_instr* instrs, size_t n,

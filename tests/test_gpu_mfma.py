@@ -300,3 +300,28 @@ def test_dense_sweeps_global_swap_modes(gpu, gswap):
         got = st.to_host()
         st.close()
         assert infidelity(got, want) <= CIRCUIT_BOUND["f16"], (gswap, relabel)
+
+
+@pytest.mark.parametrize("n,d,seed", [(22, 16, 5), (20, 12, 4)])
+def test_dense_sweeps_free_windows_vs_oracle(gpu, n, d, seed):
+    """Relabelled dense sweeps whose windows leave out qubits 0-3 (fewer sweeps than windows
+    that hold them; each store is coalesced on qubits the next window holds, walked in its own
+    order through LDS): against the oracle's unfused circuit, and against the same program in
+    windows that hold qubits 0-3."""
+    prog = gpu.layered(n, d, seed)
+    fused = gpu.transpile_gpu(prog, max_width=5, rotations=True)
+    free = gpu.plan_windowed(fused, "c64", gpu.exec_opts("dense", relabel=True))[2]["sweeps"]
+    fixed = gpu.plan_windowed(fused, "c64", gpu.exec_opts("dense", relabel=True, fixed_windows=True))[2]["sweeps"]
+    assert free < fixed
+    psi = rand_state(np.random.default_rng(seed), n)
+    want, _ = oracle.execute(n, prog.instrs, amps=psi)
+    got, t = run_c64(gpu, fused, psi, "f16", "dense", relabel=True)
+    st = gpu.State(amps=psi, dtype="c64")
+    st.execute(fused, gpu.exec_opts("dense", dense="f16", relabel=True, fixed_windows=True))
+    ref = st.to_host()
+    st.close()
+    print(f"{n}q depth {d}: {free} free-window sweeps ({fixed} fixed): infidelity "
+          f"{infidelity(got, want):.2e} (fixed windows {infidelity(ref, want):.2e})", flush=True)
+    assert "k_dsweep" in t
+    assert infidelity(got, want) <= CIRCUIT_BOUND["f16"] and infidelity(ref, want) <= CIRCUIT_BOUND["f16"]
+    assert np.abs(got - ref).max() <= 1e-5 * np.abs(want).max()

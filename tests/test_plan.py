@@ -258,3 +258,24 @@ def test_free_window_plans_are_valid_reorderings(case):
             k = g["flags_arity"] & 15
             qs |= set(int(x) for x in g["qubits"][:k])
         assert len(qs) <= info["window_qubits"]
+
+
+@pytest.mark.parametrize("n,d,seed", [(22, 16, 5), (18, 12, 3)])
+def test_relabelled_dense_sweep_plans_with_free_windows(n, d, seed):
+    """Relabelled dense sweeps take windows without qubits 0-3 when that needs fewer sweeps:
+    still a valid reordering, every window holds its gates (13 qubits)."""
+    prog = qlx.transpile_gpu(qlx.layered(n, d, seed), max_width=5, rotations=True)
+    order, sweep, info = qlx.plan_windowed(prog, "c64", qlx.exec_opts("dense", relabel=True))
+    fixed = qlx.plan_windowed(prog, "c64", qlx.exec_opts("dense", relabel=True, fixed_windows=True))[2]
+    assert info["sweeps"] < fixed["sweeps"]
+    assert sorted(order.tolist()) == list(range(len(prog))) and (sweep >= 0).all()
+    init = bf.random_state(n, np.random.default_rng(seed))
+    a, _ = oracle.execute(n, prog.instrs, prog.pool, prog.meta, amps=init)
+    b, _ = oracle.execute(n, prog.instrs[order], prog.pool, prog.meta, amps=init)
+    np.testing.assert_allclose(b, a, atol=1e-12)
+    for s_ in range(info["sweeps"]):
+        qs = set()
+        for g in prog.instrs[order[sweep == s_]]:
+            a_ = int(g["flags_arity"]) & 15
+            qs |= set(int(x) for x in g["qubits"][:min(a_, 4)]) | ({int(g["pad"]) & 255} if a_ == 5 else set())
+        assert len(qs) <= 13
