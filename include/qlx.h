@@ -106,9 +106,9 @@ typedef struct qlx_exec_opts {
     uint32_t dense_kernel;    /* QLX_DENSE_*: arithmetic of UNITARY / FUSED_BLOCK gates */
     uint32_t sweep_workgroups; /* window sweeps with <= 16 registers per work-item: work-groups
                                   per launch (each loops over chunks); 0 = every resident slot */
-    uint32_t specialize;      /* QLX_SPECIALIZE_* flags. windowed: 0 = the generic sweep kernel;
-                                 dense sweeps: QLX_SPECIALIZE_RELABEL alone selects out-of-place
-                                 relabelled sweeps */
+    uint32_t specialize;      /* QLX_SPECIALIZE_* flags. windowed: without QLX_SPECIALIZE the
+                                 generic sweep kernel; QLX_SPECIALIZE_RELABEL (windowed or dense
+                                 sweeps) selects out-of-place relabelled sweeps */
     uint32_t plan_flags;      /* QLX_PLAN_* flags of the window-sweep planner; 0 = defaults */
 } qlx_exec_opts;
 
@@ -119,8 +119,8 @@ typedef struct qlx_exec_opts {
  * for RY a Y, for RZ a Z, S or a control of a CX/CCX/CSWAP. The state is the same up to
  * rounding (the angles are summed in double precision, as the cosines and sines are formed).
  * QLX_PLAN_KEEP_ROTATIONS applies every rotation as written. Plain execution never merges.
- * Out-of-place specialised sweeps (QLX_SPECIALIZE | QLX_SPECIALIZE_RELABEL) may take windows
- * that leave out the low qubits: a sweep's loads and stores stay coalesced on qubits it shares
+ * Out-of-place sweeps (QLX_SPECIALIZE_RELABEL: window sweeps, generic or specialised, and dense
+ * sweeps) may take windows that leave out the low qubits: a sweep's loads and stores stay coalesced on qubits it shares
  * with the next sweep, which the store moves to the low address bits. The planner keeps such a
  * plan when it needs fewer passes. QLX_PLAN_FIXED_WINDOWS keeps the low qubits in every window. */
 #define QLX_PLAN_KEEP_ROTATIONS 1u
@@ -131,7 +131,7 @@ typedef struct qlx_exec_opts {
  *   program and state; later executions of the same program reuse them). The ROCm code-object
  *   compiler keeps its own on-disk cache, so a program compiled once is loaded in well under a
  *   second by later processes.
- * QLX_SPECIALIZE_RELABEL (windowed: with QLX_SPECIALIZE; dense sweeps: on its own): every
+ * QLX_SPECIALIZE_RELABEL (windowed, with or without QLX_SPECIALIZE; dense sweeps): every
  *   sweep reads the state from one buffer
  *   and writes it to a second one with its qubits re-laid out so that the next sweep's window
  *   sits on the low address bits (short strides, fewer DRAM pages per chunk); the last sweep
@@ -140,7 +140,8 @@ typedef struct qlx_exec_opts {
  *   qlx_state_device_ptr may return the other buffer after such an execution.
  * QLX_SPECIALIZE_ASYNC (windowed, with QLX_SPECIALIZE): the program's sweeps are planned and
  *   compiled on a background host thread, shared by every state of the process; until they are
- *   ready each execution runs the generic sweep kernel in place (the same gates, to rounding).
+ *   ready each execution runs the generic sweep kernel (out of place when the second buffer is
+ *   there, else in place; the same gates, to rounding).
  *   qlx_specialize_wait waits for the background work. */
 #define QLX_SPECIALIZE 1u
 #define QLX_SPECIALIZE_RELABEL 2u
@@ -408,6 +409,13 @@ int qlx_batch_destroy(qlx_batch* b);
  * released. qlx_server_pump never blocks on device work. */
 typedef struct qlx_server qlx_server;
 int qlx_server_create(int dtype, uint64_t hbm_budget_bytes, qlx_server** out);
+/* A request larger than one window runs out of place by default: its arena block holds two
+ * state buffers, and each sweep re-lays out the qubits for the next (relabelled sweeps,
+ * windows free of the low qubits: fewer sweeps). QLX_SERVE_IN_PLACE keeps one buffer per
+ * request (half the HBM per request, more sweeps). Requests whose two buffers exceed the budget
+ * run in place. Set before the first submit. */
+#define QLX_SERVE_IN_PLACE 1u
+int qlx_server_set_flags(qlx_server* s, uint32_t flags);
 int qlx_server_submit(qlx_server* s, uint32_t num_qubits, const qlx_gate_instr* instrs, size_t n,
                       uint64_t* request_id);
 /* Harvests finished launches, admits queued requests, launches the next sweep of every

@@ -46,16 +46,16 @@ __device__ __forceinline__ void run_task_chunk(A* arena, const BatchCircuitDev& 
     const bool init = sweep == 0;
     if constexpr (sizeof(A) == 8) {
         switch (c.rb) {
-            case 2: run_chunk<A, 8, 2>(amp, s, chunk, ph, ops, coef, lds, init); break;
-            case 3: run_chunk<A, 8, 3>(amp, s, chunk, ph, ops, coef, lds, init); break;
-            case 4: run_chunk<A, 8, 4>(amp, s, chunk, ph, ops, coef, lds, init); break;
-            default: run_chunk<A, 8, 5>(amp, s, chunk, ph, ops, coef, lds, init); break;
+            case 2: run_chunk<A, 8, 2>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
+            case 3: run_chunk<A, 8, 3>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
+            case 4: run_chunk<A, 8, 4>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
+            default: run_chunk<A, 8, 5>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
         }
     } else {
         switch (c.rb) {
-            case 2: run_chunk<A, 8, 2>(amp, s, chunk, ph, ops, coef, lds, init); break;
-            case 3: run_chunk<A, 8, 3>(amp, s, chunk, ph, ops, coef, lds, init); break;
-            default: run_chunk<A, 8, 4>(amp, s, chunk, ph, ops, coef, lds, init); break;
+            case 2: run_chunk<A, 8, 2>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
+            case 3: run_chunk<A, 8, 3>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
+            default: run_chunk<A, 8, 4>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
         }
     }
 }

@@ -11,7 +11,10 @@
 // device. Up to two launches per group are in flight; a request's sweeps are ordered by
 // its group's stream. Plans are formed on host planner threads (a few ms per circuit of
 // 20-28 qubits): submit returns at once, and admission takes the queue's head as soon as
-// its plan is ready, so planning overlaps the device's work.
+// its plan is ready, so planning overlaps the device's work. A request larger than one window
+// runs out of place between two halves of its arena block (each sweep re-lays out the qubits so
+// that the next window sits on the low address bits, and windows need not hold the low qubits),
+// unless the server runs in place (QLX_SERVE_IN_PLACE) or the two halves exceed the budget.
 
 #include <algorithm>
 #include <atomic>
@@ -31,13 +34,15 @@
 namespace qlx {
 
 struct ServeTask {
-    uint64_t amp_off;    // elements into the arena
+    uint64_t src_off;    // elements into the arena: the buffer this sweep reads
+    uint64_t dst_off;    // ... and the one it writes (the same in place)
     uint64_t block_off;  // bytes into the descriptor pool (this request's image)
     uint32_t sweep;      // sweep index within the request
     uint32_t chunk;
     uint32_t rb;
     uint32_t init;       // first sweep: synthesise |0...0>
-    uint32_t off_ph, off_op, off_cf, pad;  // byte offsets of phases/ops/coefs inside the image
+    uint32_t off_ph, off_op, off_cf;  // byte offsets of phases/ops/coefs inside the image
+    uint32_t oop;        // out of place: the sweep's load and store layouts differ
 };
 
 template <class A>
@@ -51,20 +56,21 @@ __global__ void __launch_bounds__(256) k_serve_step(A* __restrict__ arena, const
     const DevPhase* ph = reinterpret_cast<const DevPhase*>(img + tk.off_ph);
     const DevOp* ops = reinterpret_cast<const DevOp*>(img + tk.off_op);
     const double* coef = reinterpret_cast<const double*>(img + tk.off_cf);
-    A* amp = arena + tk.amp_off;
-    const bool init = tk.init != 0;
+    const A* src = arena + tk.src_off;
+    A* dst = arena + tk.dst_off;
+    const bool init = tk.init != 0, oop = tk.oop != 0;
     if constexpr (sizeof(A) == 8) {
         switch (tk.rb) {
-            case 2: run_chunk<A, 8, 2>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
-            case 3: run_chunk<A, 8, 3>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
-            case 4: run_chunk<A, 8, 4>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
-            default: run_chunk<A, 8, 5>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
+            case 2: run_chunk<A, 8, 2>(src, dst, oop, sw, tk.chunk, ph, ops, coef, lds, init); break;
+            case 3: run_chunk<A, 8, 3>(src, dst, oop, sw, tk.chunk, ph, ops, coef, lds, init); break;
+            case 4: run_chunk<A, 8, 4>(src, dst, oop, sw, tk.chunk, ph, ops, coef, lds, init); break;
+            default: run_chunk<A, 8, 5>(src, dst, oop, sw, tk.chunk, ph, ops, coef, lds, init); break;
         }
     } else {
         switch (tk.rb) {
-            case 2: run_chunk<A, 8, 2>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
-            case 3: run_chunk<A, 8, 3>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
-            default: run_chunk<A, 8, 4>(amp, sw, tk.chunk, ph, ops, coef, lds, init); break;
+            case 2: run_chunk<A, 8, 2>(src, dst, oop, sw, tk.chunk, ph, ops, coef, lds, init); break;
+            case 3: run_chunk<A, 8, 3>(src, dst, oop, sw, tk.chunk, ph, ops, coef, lds, init); break;
+            default: run_chunk<A, 8, 4>(src, dst, oop, sw, tk.chunk, ph, ops, coef, lds, init); break;
         }
     }
 }
@@ -118,6 +124,8 @@ struct PlanJob {
     uint32_t nq = 0;
     int dtype = QLX_C64;
     uint64_t pool_bytes = 0;
+    bool allow_oop = false;  // the request may run out of place (two state buffers fit the budget)
+    bool oop = false;        // planned out of place
     std::vector<qlx_gate_instr> instrs;
     std::unique_ptr<SweepPlan> plan;
     std::vector<char> image;
@@ -137,8 +145,13 @@ void plan_job(PlanJob& j) {
     geo.w = std::min<int>((int)j.nq, j.dtype == QLX_C64 ? 13 : 12);
     geo.rb = geo.w - 8;
     geo.c = j.dtype == QLX_C64 ? 4 : 3;  // one 128-byte cache line of amplitudes
+    // out of place: relabelled sweeps, whose windows need not hold the low qubits
+    j.oop = j.allow_oop && (int)j.nq > geo.w;
+    geo.free = j.oop;
+    if (j.oop) o.specialize = QLX_SPECIALIZE_RELABEL;
     j.plan = build_plan_geo(j.nq, geo, j.instrs.data(), j.instrs.size(), nullptr, nullptr, o, /*thorough=*/false);
     if (j.dtype == QLX_C64) densify_phases(*j.plan);
+    if (j.oop) assign_layouts(*j.plan, j.nq);
     const SweepPlan& p = *j.plan;
     j.n_sweeps = (uint32_t)p.dsw.size();
     j.rb = (uint32_t)geo.rb;
@@ -233,6 +246,10 @@ struct qlx_server {
         Status status = QUEUED;
         std::shared_ptr<PlanJob> job;  // the plan and descriptor image (until the upload is complete)
         uint64_t amp_off = 0, state_bytes = 0, block_off = 0, block_bytes = 0;
+        // out of place: the second buffer follows the first (alloc_bytes = 2 state_bytes) and the
+        // result sits in the buffer the last sweep wrote
+        bool oop = false;
+        uint64_t alloc_bytes = 0, alt_off = 0, result_off = 0;
         uint32_t off_ph = 0, off_op = 0, off_cf = 0;
         uint32_t next_sweep = 0, n_sweeps = 0, rb = 0, log_chunks = 0;
         double submit_s = 0, admit_s = 0, done_s = 0;
@@ -259,6 +276,7 @@ struct qlx_server {
     DevBuf scratch;
     hipStream_t aux = nullptr;
     Planner planner;
+    uint32_t flags = 0;  // QLX_SERVE_*
 };
 
 namespace {
@@ -318,6 +336,8 @@ void admit(qlx_server* s) {
         }
         PlanJob& j = *r.job;
         if (!r.block_bytes) {
+            r.oop = j.oop;
+            r.alloc_bytes = r.state_bytes * (j.oop ? 2 : 1);
             r.block_bytes = j.block_bytes;
             r.n_sweeps = j.n_sweeps;
             r.rb = j.rb;
@@ -327,12 +347,15 @@ void admit(qlx_server* s) {
             r.off_cf = j.off_cf;
         }
         uint64_t aoff = 0, boff = 0;
-        if (!s->arena_alloc.alloc(r.state_bytes, aoff)) return;
+        if (!s->arena_alloc.alloc(r.alloc_bytes, aoff)) return;
         if (!s->pool_alloc.alloc(r.block_bytes, boff)) {
-            s->arena_alloc.release(aoff, r.state_bytes);
+            s->arena_alloc.release(aoff, r.alloc_bytes);
             return;
         }
         r.amp_off = aoff / s->amp;
+        r.alt_off = r.oop ? r.amp_off + (r.state_bytes / s->amp) : r.amp_off;
+        // sweep k reads buffer k % 2 and writes the other: the result is in buffer n_sweeps % 2
+        r.result_off = (r.oop && (r.n_sweeps & 1)) ? r.alt_off : r.amp_off;
         r.block_off = boff;
         Group& g = s->groups[r.nq];
         if (!g.stream) QLX_HIP_CHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
@@ -382,7 +405,10 @@ void launch_steps(qlx_server* s) {
             const uint32_t nch = 1u << r.log_chunks;
             for (uint32_t k = 0; k < nch; ++k) {
                 ServeTask t{};
-                t.amp_off = r.amp_off;
+                const bool odd = (r.next_sweep & 1) != 0;
+                t.src_off = odd ? r.alt_off : r.amp_off;
+                t.dst_off = r.oop ? (odd ? r.amp_off : r.alt_off) : r.amp_off;
+                t.oop = r.oop ? 1u : 0u;
                 t.block_off = r.block_off;
                 t.sweep = r.next_sweep;
                 t.chunk = k;
@@ -484,6 +510,7 @@ int qlx_server_submit(qlx_server* s, uint32_t nq, const qlx_gate_instr* instrs, 
         auto job = std::make_shared<PlanJob>();
         job->nq = nq;
         job->dtype = s->dtype;
+        job->allow_oop = !(s->flags & QLX_SERVE_IN_PLACE) && 2 * r.state_bytes <= s->budget;
         job->pool_bytes = s->pool_bytes;
         job->instrs.assign(instrs, instrs + n);
         r.job = job;
@@ -493,6 +520,15 @@ int qlx_server_submit(qlx_server* s, uint32_t nq, const qlx_gate_instr* instrs, 
         s->queue.push_back(id);
         s->planner.push(std::move(job));
         if (request_id) *request_id = id;
+    });
+}
+
+int qlx_server_set_flags(qlx_server* s, uint32_t flags) {
+    return guard([&] {
+        if (!s) fail(QLX_EINVAL, "null server");
+        if (!s->reqs.empty()) fail(QLX_EINVAL, "server flags are set before the first request");
+        if (flags & ~QLX_SERVE_IN_PLACE) fail(QLX_EINVAL, "unknown server flag");
+        s->flags = flags;
     });
 }
 
@@ -551,7 +587,7 @@ int qlx_server_compare(qlx_server* s, uint64_t id, qlx_state* ref, double* infid
         if (!ref || !ref->amp || ref->nq != r.nq) fail(QLX_EINVAL, "reference state does not match");
         QLX_HIP_CHECK(hipStreamSynchronize(ref->stream));
         double re = 0, im = 0, m = 0, nb = 0;
-        compare_amplitudes(ref->amp, ref->dtype, (char*)s->arena.ptr + r.amp_off * s->amp, s->dtype, 1ull << r.nq,
+        compare_amplitudes(ref->amp, ref->dtype, (char*)s->arena.ptr + r.result_off * s->amp, s->dtype, 1ull << r.nq,
                            s->aux, s->scratch, &re, &im, &m, &nb);
         if (infidelity) *infidelity = infidelity_from(re, im, reduce_norm_sq(ref), nb);
         if (max_abs) *max_abs = m;
@@ -565,7 +601,7 @@ int qlx_server_state_to_host(qlx_server* s, uint64_t id, double* out, size_t dim
         if (r.status != qlx_server::DONE) fail(QLX_EINVAL, "request not completed");
         const size_t n = size_t(1) << r.nq;
         if (!out || dim < n) fail(QLX_EINVAL, "output buffer too small");
-        const char* src = (const char*)s->arena.ptr + r.amp_off * s->amp;
+        const char* src = (const char*)s->arena.ptr + r.result_off * s->amp;
         if (s->dtype == QLX_C128) {
             QLX_HIP_CHECK(hipMemcpy(out, src, n * 16, hipMemcpyDeviceToHost));
         } else {
@@ -581,7 +617,7 @@ int qlx_server_release(qlx_server* s, uint64_t id) {
         if (!s) fail(QLX_EINVAL, "null server");
         Req& r = get_req(s, id);
         if (r.status != qlx_server::DONE) fail(QLX_EINVAL, "request not completed");
-        s->arena_alloc.release(r.amp_off * s->amp, r.state_bytes);
+        s->arena_alloc.release(r.amp_off * s->amp, r.alloc_bytes);
         s->reqs.erase(id);
     });
 }

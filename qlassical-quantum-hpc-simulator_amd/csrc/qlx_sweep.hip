@@ -41,37 +41,41 @@ namespace qlx {
 // Persistent sweep: work-group b runs chunks b, b + G, b + 2G, ... (G = grid size, the
 // resident work-groups of the device). With up to 16 amplitude registers per work-item,
 // the next chunk's HBM loads are issued into a second register set before the current
-// chunk's phases run, so they overlap its compute.
-template <class A, int LOGT, int RB>
-__global__ void __launch_bounds__(1 << LOGT) k_sweep(A* __restrict__ amp, const DevSweep* __restrict__ sw,
+// chunk's phases run, so they overlap its compute. OOP: out of place (relabelled plans),
+// reading `src` and writing `dst` through the sweep's load and store layouts; in place
+// src == dst.
+template <class A, int LOGT, int RB, bool OOP>
+__global__ void __launch_bounds__(1 << LOGT) k_sweep(const A* src, A* dst, const DevSweep* __restrict__ sw,
                                                     const DevPhase* __restrict__ ph, const DevOp* __restrict__ ops,
                                                     const double* __restrict__ coef, uint64_t nchunks) {
     extern __shared__ __align__(16) unsigned char smem[];
     A* lds = reinterpret_cast<A*>(smem);
     uint64_t c = blockIdx.x;
     if (c >= nchunks) return;
+    auto ld_base = [&](uint64_t k) { return OOP ? chunk_base_oop(sw->ld_cpos, sw->nc, k) : chunk_base<LOGT, RB>(sw, k); };
+    auto st_base = [&](uint64_t k, uint64_t lb) { return OOP ? chunk_base_oop(sw->st_cpos, sw->nc, k) : lb; };
     // 32 registers of amplitudes per work-item leave no room for a second set
     constexpr bool kPrefetch = RB <= 4;
     if constexpr (!kPrefetch) {
         // one chunk per work-group (grid = chunks): a loop here would keep every register's
         // offsets live across iterations and push the kernel past the register file
         A v[1 << RB];
-        const uint64_t base = chunk_base<LOGT, RB>(sw, c);
-        load_chunk<A, LOGT, RB>(v, amp, sw, base);
+        const uint64_t base = ld_base(c);
+        load_chunk<A, LOGT, RB>(v, src, sw, base);
         compute_chunk<A, LOGT, RB>(v, sw, ph, ops, coef, lds);
-        store_chunk<A, LOGT, RB>(v, amp, sw, base);
+        store_chunk<A, LOGT, RB>(v, dst, sw, st_base(c, base));
         return;
     }
     A v[1 << RB], w[1 << RB];
-    uint64_t base = chunk_base<LOGT, RB>(sw, c);
-    load_chunk<A, LOGT, RB>(v, amp, sw, base);
+    uint64_t base = ld_base(c);
+    load_chunk<A, LOGT, RB>(v, src, sw, base);
     for (;;) {
         const uint64_t nc = c + gridDim.x;
         const bool more = nc < nchunks;
-        const uint64_t nbase = more ? chunk_base<LOGT, RB>(sw, nc) : 0;
-        if (more) load_chunk<A, LOGT, RB>(w, amp, sw, nbase);
+        const uint64_t nbase = more ? ld_base(nc) : 0;
+        if (more) load_chunk<A, LOGT, RB>(w, src, sw, nbase);
         compute_chunk<A, LOGT, RB>(v, sw, ph, ops, coef, lds);
-        store_chunk<A, LOGT, RB>(v, amp, sw, base);
+        store_chunk<A, LOGT, RB>(v, dst, sw, st_base(c, base));
         if (!more) break;
 #pragma unroll
         for (int i = 0; i < (1 << RB); ++i) v[i] = w[i];
@@ -110,8 +114,7 @@ static Geometry geometry_for(int dtype, uint32_t nq, const qlx_exec_opts& o) {
     // low qubits kept in the window: one 128-byte cache line of amplitudes (8 x 16 B, 16 x 8 B)
     g.c = o.coalesce_qubits ? (int)o.coalesce_qubits : (dtype == QLX_C64 ? 4 : 3);
     if (g.c < 0 || g.c > g.logt) fail(QLX_EINVAL, "coalesce_qubits out of range");
-    g.free = (o.specialize & QLX_SPECIALIZE) && (o.specialize & QLX_SPECIALIZE_RELABEL) &&
-             !(o.plan_flags & QLX_PLAN_FIXED_WINDOWS);
+    g.free = (o.specialize & QLX_SPECIALIZE_RELABEL) && !(o.plan_flags & QLX_PLAN_FIXED_WINDOWS);
     (void)nq;
     return g;
 }
@@ -1003,6 +1006,7 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
     Walk walk(gates, preds);
     walk.budget = take_budget;
     size_t next_window = 0;
+    std::vector<int> prev_S;  // the last sweep's window (free windows)
     while (true) {
         // plain-only gates whose predecessors are done run as their own kernels
         for (int gi : walk.run_plain()) raw.push_back(RawStep{false, gi, {}, {}});
@@ -1016,7 +1020,24 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
             if (next_window >= given->size()) fail(QLX_EHIP, "sweep planner: window sequence too short");
             S = (*given)[next_window++];
         } else if (chooser == Chooser::Wide) {
-            S = choose_window_wide(gates, walk.pending, walk.done, preds, (int)nq, geo, seed_gate);
+            if (geo.free && !prev_S.empty()) {
+                // free windows: the candidate (sharing C qubits with the previous window) that
+                // runs the most pending gates
+                int best_n = -1;
+                std::vector<char> run(gates.size(), 0);
+                for (const std::vector<int>& c : window_candidates(walk, (int)nq, geo, seed_gate, &prev_S)) {
+                    std::vector<char> in_set(nq, 0);
+                    for (int q : c) in_set[q] = 1;
+                    const int cnt = count_runnable(gates, walk.pending, walk.done, in_set, run, preds);
+                    if (cnt > best_n) {
+                        best_n = cnt;
+                        S = c;
+                    }
+                }
+                if (S.empty()) fail(QLX_EHIP, "sweep planner: no window shares the coalesced qubits");
+            } else {
+                S = choose_window_wide(gates, walk.pending, walk.done, preds, (int)nq, geo, seed_gate);
+            }
         } else {
             std::vector<char> in_set(nq, 0);
             int have = 0;
@@ -1034,7 +1055,13 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
         // the sweep's gates, in program order
         const std::vector<int> taken = walk.take(S, (int)nq);
         if (taken.empty()) fail(QLX_EHIP, "sweep planner: empty sweep");
+        if (geo.free && chooser == Chooser::Wide && !prev_S.empty() && walk.pending.empty()) {
+            // the last window stores the logical layout: it holds qubits 0..C-1
+            S = close_window(S, taken, gates, prev_S, std::min(geo.c, (int)nq), W);
+            if (S.empty()) fail(QLX_EHIP, "sweep planner: the last window cannot hold the coalesced qubits");
+        }
         raw.push_back(RawStep{true, -1, S, taken});
+        prev_S = S;
     }
     // thorough: a sweep whose gates cost less than a pass as plain kernels runs them as plain
     // kernels. With free windows only where the sweeps around it still chain: the first
@@ -1337,7 +1364,17 @@ std::unique_ptr<SweepPlan> build_plan_geo(uint32_t nq, const Geometry& geo, cons
                                           const qlx_exec_opts& opts, bool thorough) {
     Geometry fixed = geo;
     fixed.free = false;
-    if (!thorough) return build_plan_with(nq, fixed, instrs, n, pool, meta, opts, Chooser::Wide, nullptr, false);
+    if (!thorough) {
+        auto p = build_plan_with(nq, fixed, instrs, n, pool, meta, opts, Chooser::Wide, nullptr, false);
+        if (geo.free && p->sweeps.size() > 1) {
+            try {
+                auto f = build_plan_with(nq, geo, instrs, n, pool, meta, opts, Chooser::Wide, nullptr, false);
+                if (f->passes < p->passes - 1e-9) p = std::move(f);
+            } catch (const Error&) {
+            }
+        }
+        return p;
+    }
     auto best = build_plan_with(nq, fixed, instrs, n, pool, meta, opts, Chooser::Greedy, nullptr, true);
     if (best->sweeps.size() <= 1) return best;
     auto consider = [&](std::unique_ptr<SweepPlan> p) {
@@ -1512,6 +1549,15 @@ void assign_layouts(SweepPlan& p, uint32_t nq) {
             sw.ld_cpos.push_back(pos[q]);
             sw.st_cpos.push_back(next[q]);
         }
+        if (outside.size() > (size_t)kMaxOopChunkBits) fail(QLX_EUNSUPPORTED, "state too large for out-of-place sweeps");
+        {
+            DevSweep& d = p.dsw[si];
+            d.nc = (uint32_t)outside.size();
+            for (size_t j = 0; j < outside.size(); ++j) {
+                d.ld_cpos[j] = (uint32_t)sw.ld_cpos[j];
+                d.st_cpos[j] = (uint32_t)sw.st_cpos[j];
+            }
+        }
         sw.ld_lpos.clear();
         sw.st_lpos.clear();
         for (int q : sw.S) {
@@ -1569,15 +1615,19 @@ static uint64_t program_key(uint32_t nq, int dtype, const qlx_gate_instr* instrs
     return key;
 }
 
+// One generic sweep; out of place (oop) it reads st->amp, writes st->alt and swaps the two.
 template <class A, int LOGT, int RB>
 static void launch_sweep(qlx_state* st, const char* dev, size_t off_sw, size_t off_ph, size_t off_op, size_t off_cf,
-                         int si, uint32_t wg_override) {
+                         int si, uint32_t wg_override, bool oop) {
     const uint64_t chunks = 1ull << (st->nq - (LOGT + RB));
     const size_t lds = (size_t(1) << (LOGT + RB)) * sizeof(A);
-    static thread_local int resident[2][8][8] = {};  // work-groups per CU, by (dtype, LOGT-5, RB-1)
-    int& per_cu = resident[sizeof(A) == 16][LOGT - 5][RB - 1];
+    static thread_local int resident[2][2][8][8] = {};  // work-groups per CU, by (dtype, oop, LOGT-5, RB-1)
+    int& per_cu = resident[sizeof(A) == 16][oop][LOGT - 5][RB - 1];
     if (!per_cu) {
-        QLX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep<A, LOGT, RB>, 1 << LOGT, lds));
+        if (oop)
+            QLX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep<A, LOGT, RB, true>, 1 << LOGT, lds));
+        else
+            QLX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep<A, LOGT, RB, false>, 1 << LOGT, lds));
         per_cu = std::max(per_cu, 1);
     }
     int dev_id = 0, cus = 0;
@@ -1585,10 +1635,21 @@ static void launch_sweep(qlx_state* st, const char* dev, size_t off_sw, size_t o
     QLX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id));
     uint64_t grid = RB <= 4 ? std::min<uint64_t>(chunks, (uint64_t)per_cu * (uint64_t)cus) : chunks;
     if (RB <= 4 && wg_override) grid = std::min<uint64_t>(chunks, wg_override);
-    k_sweep<A, LOGT, RB><<<(unsigned)grid, 1 << LOGT, lds, st->stream>>>(
-        (A*)st->amp, (const DevSweep*)(dev + off_sw) + si, (const DevPhase*)(dev + off_ph),
-        (const DevOp*)(dev + off_op), (const double*)(dev + off_cf), chunks);
-    QLX_HIP_CHECK(hipGetLastError());
+    const DevSweep* sw = (const DevSweep*)(dev + off_sw) + si;
+    const DevPhase* ph = (const DevPhase*)(dev + off_ph);
+    const DevOp* ops = (const DevOp*)(dev + off_op);
+    const double* cf = (const double*)(dev + off_cf);
+    if (oop) {
+        if (!st->alt || st->alt == st->amp) fail(QLX_EHIP, "out-of-place sweep without a second buffer");
+        k_sweep<A, LOGT, RB, true><<<(unsigned)grid, 1 << LOGT, lds, st->stream>>>(
+            (const A*)st->amp, (A*)st->alt, sw, ph, ops, cf, chunks);
+        QLX_HIP_CHECK(hipGetLastError());
+        std::swap(st->amp, st->alt);
+    } else {
+        k_sweep<A, LOGT, RB, false><<<(unsigned)grid, 1 << LOGT, lds, st->stream>>>(
+            (const A*)st->amp, (A*)st->amp, sw, ph, ops, cf, chunks);
+        QLX_HIP_CHECK(hipGetLastError());
+    }
 }
 
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -1621,7 +1682,7 @@ static std::unique_ptr<SweepPlan> make_plan(uint32_t nq, int dtype, const qlx_ga
     std::unique_ptr<SweepPlan> p = build_plan(nq, dtype, instrs, n, pool, meta, opts);
     p->tag = key;
     if (dtype == QLX_C64 && !(opts.specialize & QLX_SPECIALIZE)) densify_phases(*p);
-    if ((opts.specialize & QLX_SPECIALIZE) && (opts.specialize & QLX_SPECIALIZE_RELABEL)) assign_layouts(*p, nq);
+    if (opts.specialize & QLX_SPECIALIZE_RELABEL) assign_layouts(*p, nq);
     return p;
 }
 
@@ -1823,13 +1884,13 @@ static void run_plan(qlx_state* st, const SweepPlan& p, const JitSweeps* jit, co
             const size_t a = cp->off_sw, b = cp->off_ph, c = cp->off_op, d = cp->off_cf;
             const int si = s.sweep_index;
             switch (key) {
-                case 1124: launch_sweep<double2, 8, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
-                case 1125: launch_sweep<double2, 7, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
-                case 1114: launch_sweep<double2, 7, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
-                case 1113: launch_sweep<double2, 8, 3>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
-                case 135: launch_sweep<float2, 8, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
-                case 125: launch_sweep<float2, 7, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
-                case 124: launch_sweep<float2, 8, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups); break;
+                case 1124: launch_sweep<double2, 8, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups, p.relabel); break;
+                case 1125: launch_sweep<double2, 7, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups, p.relabel); break;
+                case 1114: launch_sweep<double2, 7, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups, p.relabel); break;
+                case 1113: launch_sweep<double2, 8, 3>(st, dev, a, b, c, d, si, opts.sweep_workgroups, p.relabel); break;
+                case 135: launch_sweep<float2, 8, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups, p.relabel); break;
+                case 125: launch_sweep<float2, 7, 5>(st, dev, a, b, c, d, si, opts.sweep_workgroups, p.relabel); break;
+                case 124: launch_sweep<float2, 8, 4>(st, dev, a, b, c, d, si, opts.sweep_workgroups, p.relabel); break;
                 default: fail(QLX_EUNSUPPORTED, "sweep geometry not built");
             }
         }
@@ -1852,8 +1913,7 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
         return;
     }
     // out-of-place sweeps need the second buffer; without it they run in place
-    if ((o.specialize & QLX_SPECIALIZE) && (o.specialize & QLX_SPECIALIZE_RELABEL) && !ensure_alt(st))
-        o.specialize &= ~QLX_SPECIALIZE_RELABEL;
+    if ((o.specialize & QLX_SPECIALIZE_RELABEL) && !ensure_alt(st)) o.specialize &= ~QLX_SPECIALIZE_RELABEL;
     if (o.specialize & QLX_SPECIALIZE_ASYNC) {
         // keyed (and its kernels named) as the same program compiled in the foreground, so the
         // two share the compiler's code-object cache
@@ -1869,8 +1929,9 @@ void execute_windowed(qlx_state* st, const qlx_gate_instr* instrs, size_t n, con
             return;
         }
         // until the compiled sweeps are ready (or if their compilation failed): the generic
-        // kernel, in place, on the default geometry when the requested one is not built for it
-        o.specialize = 0;
+        // kernel (out of place when the second buffer is there), on the default geometry when
+        // the requested one is not built for it
+        o.specialize &= QLX_SPECIALIZE_RELABEL;
         if (!geometry_built(st->dtype, geo.w, geo.rb, false)) {
             o.window_qubits = 0;
             o.register_qubits = 0;

@@ -63,6 +63,8 @@ struct DevPhase {
     uint32_t pad2[2];
 };
 
+constexpr int kMaxOopChunkBits = 24;  // chunk-index bits of an out-of-place sweep (states of <= 24 + W qubits)
+
 struct DevSweep {
     uint32_t n_phases, phase_begin;
     uint32_t S[kMaxW];           // global qubit of local bit j (ascending, except in free-window
@@ -71,6 +73,9 @@ struct DevSweep {
     uint32_t st_tglob[kMaxLogT]; // ... in the last layout
     uint64_t ld_roff[kMaxR];     // global offset of register i in the first layout
     uint64_t st_roff[kMaxR];     // ... in the last layout
+    // out of place (relabelled plans): address bit of chunk-index bit j at the load and at the store
+    uint32_t nc, pad[3];
+    uint32_t ld_cpos[kMaxOopChunkBits], st_cpos[kMaxOopChunkBits];
 };
 static_assert(sizeof(DevPhase) % 16 == 0 && sizeof(DevSweep) % 16 == 0, "descriptor alignment");
 
@@ -239,6 +244,13 @@ __device__ __forceinline__ uint64_t chunk_base(const DevSweep* __restrict__ sw, 
     return base;
 }
 
+// Out of place: the chunk index's bits scattered to the layout's chunk-index address bits.
+__device__ __forceinline__ uint64_t chunk_base_oop(const uint32_t* __restrict__ cpos, uint32_t nc, uint64_t chunk) {
+    uint64_t base = 0;
+    for (uint32_t j = 0; j < nc; ++j) base |= ((chunk >> j) & 1ull) << cpos[j];
+    return base;
+}
+
 template <class A, int LOGT, int RB>
 __device__ __forceinline__ void load_chunk(A (&v)[1 << RB], const A* __restrict__ amp,
                                            const DevSweep* __restrict__ sw, uint64_t base) {
@@ -389,17 +401,20 @@ __device__ __forceinline__ void compute_chunk(A (&v)[1 << RB], const DevSweep* _
     }
 }
 
-// One chunk start to end (the batched and served kernels).
+// One chunk start to end (the batched and served kernels). In place (dst == src) the chunk's
+// base comes from the window's qubits; out of place from the layouts' chunk-index bits.
 template <class A, int LOGT, int RB>
-__device__ __forceinline__ void run_chunk(A* __restrict__ amp, const DevSweep* __restrict__ sw, uint64_t chunk,
+__device__ __forceinline__ void run_chunk(const A* src, A* dst, bool oop,
+                                          const DevSweep* __restrict__ sw, uint64_t chunk,
                                           const DevPhase* __restrict__ ph, const DevOp* __restrict__ ops,
                                           const double* __restrict__ coef, A* __restrict__ lds, bool init_zero) {
     A v[1 << RB];
-    const uint64_t base = chunk_base<LOGT, RB>(sw, chunk);
+    const uint64_t base = oop ? chunk_base_oop(sw->ld_cpos, sw->nc, chunk) : chunk_base<LOGT, RB>(sw, chunk);
     if (init_zero) zero_chunk<A, LOGT, RB>(v, sw, base);
-    else load_chunk<A, LOGT, RB>(v, amp, sw, base);
+    else load_chunk<A, LOGT, RB>(v, src, sw, base);
     compute_chunk<A, LOGT, RB>(v, sw, ph, ops, coef, lds);
-    store_chunk<A, LOGT, RB>(v, amp, sw, base);
+    const uint64_t sbase = oop ? chunk_base_oop(sw->st_cpos, sw->nc, chunk) : base;
+    store_chunk<A, LOGT, RB>(v, dst, sw, sbase);
 }
 
 // ---------------------------------------------------------------------------------

@@ -48,8 +48,8 @@ struct HIPEngineOptions {
     // windowed: compile each program's sweeps into straight-line kernels (seconds of hipRTC
     // time on a program's first execution, reused afterwards: for programs run many times)
     bool specialize = false;
-    // with specialize (or dense_sweeps): out-of-place sweeps through a second buffer of the
-    // state's size that keep every sweep's window on the low address bits
+    // window sweeps (generic or specialised) or dense sweeps: out of place through a second
+    // buffer of the state's size, every sweep's window on the low address bits
     bool relabel = false;
     // complex64 fused programs: every gate inside 13-qubit LDS windows, 4/5-qubit gates on the
     // matrix cores (QLX_EXEC_DENSE_SWEEPS; takes precedence over windowed)
@@ -115,7 +115,7 @@ public:
         if (opts_.dense_sweeps)
             o.specialize = opts_.relabel ? QLX_SPECIALIZE_RELABEL : 0u;
         else
-            o.specialize = opts_.specialize ? (QLX_SPECIALIZE | (opts_.relabel ? QLX_SPECIALIZE_RELABEL : 0u)) : 0u;
+            o.specialize = (opts_.specialize ? QLX_SPECIALIZE : 0u) | (opts_.relabel ? QLX_SPECIALIZE_RELABEL : 0u);
         const auto* instrs = reinterpret_cast<const qlx_gate_instr*>(program.data());
         const auto* pdata = reinterpret_cast<const double*>(pool.data.data());
         const auto* meta = reinterpret_cast<const qlx_matrix_entry*>(pool.metadata.data());

@@ -33,6 +33,7 @@ QLX_SPECIALIZE_RELABEL = 2
 QLX_SPECIALIZE_ASYNC = 4
 QLX_PLAN_KEEP_ROTATIONS = 1
 QLX_PLAN_FIXED_WINDOWS = 2
+QLX_SERVE_IN_PLACE = 1
 QLX_DENSE_AUTO = 0
 QLX_DENSE_VALU = 1
 QLX_DENSE_MFMA_BF16 = 2
@@ -230,6 +231,7 @@ SIGNATURES = [
     ("qlx_batch_compare", [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("qlx_batch_destroy", [C.c_void_p]),
     ("qlx_server_create", [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("qlx_server_set_flags", [C.c_void_p, C.c_uint32]),
     ("qlx_server_submit", [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint64)]),
     ("qlx_server_pump", [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("qlx_server_request_info", [C.c_void_p, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_double)]),

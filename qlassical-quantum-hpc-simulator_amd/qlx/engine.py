@@ -41,15 +41,13 @@ def exec_opts(mode: str = "plain", window_qubits: int = 0, coalesce_qubits: int 
     window sweeps, with specialised relabelled sweeps compiled in the background for states of
     >= 20 qubits). dense: arithmetic of 4/5-qubit dense gates on complex64 states
     ('auto' = 'f16' matrix cores (FP16 hi/lo split), 'bf16' matrix cores (BF16 split), 'f32'
-    matrix cores, 'valu' vector units). relabel (windowed: with specialize; dense: alone):
+    matrix cores, 'valu' vector units). relabel (windowed, with or without specialize; dense):
     out-of-place sweeps that re-lay out the qubits between two buffers. background (windowed,
     with specialize): compile on a host thread and run the generic kernel until ready.
     keep_rotations (window sweeps): apply every rotation as written instead of merging
     same-axis rotations through the gates they commute with (QLX_PLAN_KEEP_ROTATIONS).
     fixed_windows (relabelled sweeps): every window holds the low qubits
     (QLX_PLAN_FIXED_WINDOWS)."""
-    if relabel and not specialize and mode != "dense":
-        raise ValueError("relabel needs specialize")
     if background and not specialize:
         raise ValueError("background needs specialize")
     o = _abi.ExecOpts()

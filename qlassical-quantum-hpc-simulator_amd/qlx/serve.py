@@ -14,10 +14,14 @@ from .ir import MT19937_64, Program, layered
 
 
 class Server:
-    def __init__(self, hbm_budget_bytes: int, dtype="c64"):
+    def __init__(self, hbm_budget_bytes: int, dtype="c64", in_place: bool = False):
+        """in_place: one state buffer per request (QLX_SERVE_IN_PLACE); by default requests
+        larger than one window run out of place between two buffers (fewer sweeps)."""
         h = C.c_void_p()
         _abi.check(_abi.lib().qlx_server_create(_dt(dtype), int(hbm_budget_bytes), C.byref(h)))
         self._h = h
+        if in_place:
+            _abi.check(_abi.lib().qlx_server_set_flags(self._h, _abi.QLX_SERVE_IN_PLACE))
 
     def submit(self, prog: Program) -> int:
         ins = np.ascontiguousarray(prog.instrs)
@@ -94,7 +98,7 @@ def poisson_arrivals(count: int, rate: float, seed: int = 5006) -> np.ndarray:
 
 
 def run_server(programs: List[Program], arrivals: np.ndarray, budget_bytes: int, dtype="c64",
-               check_every: int = 0, timeout_s: float = 300.0) -> Dict:
+               check_every: int = 0, timeout_s: float = 300.0, in_place: bool = False) -> Dict:
     """Replays the arrival schedule against a Server in real time; returns latency stats.
     Every check_every-th request is compared (FP64, on the GPU) with a complex128 state of the
     same circuit; those reference states are computed before the clock starts, so a check
@@ -107,7 +111,7 @@ def run_server(programs: List[Program], arrivals: np.ndarray, budget_bytes: int,
             ref.execute(programs[j], exec_opts("windowed") if programs[j].num_qubits >= 12 else None)
             ref.sync()
             refs[j] = ref
-    srv = Server(budget_bytes, dtype)
+    srv = Server(budget_bytes, dtype, in_place)
     rid_to_j: Dict[int, int] = {}
     lat: Dict[int, float] = {}
     checks = []

@@ -114,10 +114,12 @@ def test_requests_admitted_while_others_run_match_alone_and_oracle(gpu):
     srv.close()
 
 
-def test_hbm_budget_limits_admission(gpu):
-    # 16 requests of 16 qubits (512 KiB each, c64) through a 1.25 MiB budget: two fit, three do not
+@pytest.mark.parametrize("in_place,fit", [(True, 2), (False, 1)])
+def test_hbm_budget_limits_admission(gpu, in_place, fit):
+    # 16 requests of 16 qubits (512 KiB each, c64) through a 1.25 MiB budget: in place two fit,
+    # three do not; out of place (two buffers per request) one fits
     ps = [gpu.layered(16, 6, 520000 + j) for j in range(16)]
-    srv = gpu.Server(5 << 18, "c64")
+    srv = gpu.Server(5 << 18, "c64", in_place=in_place)
     ids = [srv.submit(p) for p in ps]
     max_active = 0
     done = {}
@@ -130,7 +132,7 @@ def test_hbm_budget_limits_admission(gpu):
         if len(done) == len(ps):
             break
     assert len(done) == len(ps)
-    assert max_active == 2
+    assert max_active == fit
     srv.close()
     with pytest.raises(gpu.QlxOutOfMemory):
         gpu.Server(1 << 20, "c64").submit(gpu.layered(20, 5, 1))
@@ -142,3 +144,32 @@ def test_poisson_driver_small(gpu):
     out = gpu.run_server(ps, arr, 1 << 30, "c64", check_every=10)
     assert out["requests"] == 50 and out["p99_ms"] >= out["p50_ms"] > 0
     assert out["max_infidelity"] <= 1e-6
+
+
+def test_out_of_place_requests_match_in_place_and_oracle(gpu):
+    """Requests larger than one window run out of place by default (relabelled sweeps between
+    two buffers, windows free of the low qubits): each result against the complex128 path, the
+    in-place server's result and, up to 18 qubits, the oracle."""
+    ps = progs(gpu, 24, (14, 22), (5, 30), 91)
+    results = {}
+    for in_place in (False, True):
+        srv = gpu.Server(1 << 31, "c64", in_place=in_place)
+        ids = [srv.submit(p) for p in ps]
+        drain(srv, len(ps))
+        for j, (p, i) in enumerate(zip(ps, ids)):
+            results[(in_place, j)] = srv.state(i, p.num_qubits)
+            if not in_place:
+                ref = gpu.State(p.num_qubits, "c128")
+                ref.execute(p)
+                ref.sync()
+                inf, mx = srv.compare(i, ref)
+                ref.close()
+                assert inf <= 1e-6 and mx <= 1e-5, (j, inf)
+            srv.release(i)
+        srv.close()
+    for j, p in enumerate(ps):
+        a, b = results[(False, j)], results[(True, j)]
+        fid = abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+        assert 1.0 - fid <= 1e-6, j
+        if p.num_qubits <= 18:
+            assert oracle_infidelity(p, a) <= 1e-6, j

@@ -145,3 +145,39 @@ def test_c64_phases_on_matrix_cores_vs_oracle(gpu, seed):
         assert info["mfma_phases"] > 0, (len(p), info)
         assert inf <= 1e-10
         assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_generic_kernel_out_of_place_vs_oracle(gpu, dtype):
+    """The generic sweep kernel out of place (relabel without specialize: windows free of the
+    low qubits, the qubits re-laid out between two buffers): layered circuits and random
+    programs from random states against the oracle (complex128: max |d| <= 1e-12; complex64:
+    infidelity <= 1e-6), and against the same kernel in place."""
+    from test_oracle import random_program
+    rng = np.random.default_rng(808)
+    ins, pool, meta = random_program(16, 150, rng)
+    rnd = gpu.Program(16, ins, pool if pool is not None else np.zeros(0, complex),
+                      meta if meta is not None else np.zeros(0, gpu.META_DTYPE))
+    for prog in (gpu.layered(22, 16, 8), gpu.layered(24, 20, 8), rnd):
+        n = prog.num_qubits
+        init = bf.random_state(n, np.random.default_rng(n))
+        pl = prog.pool if prog.pool.size else None
+        me = prog.meta if len(prog.meta) else None
+        want, _ = oracle.execute(n, prog.instrs, pl, me, amps=init)
+        outs = {}
+        for relabel in (True, False):
+            st = gpu.State(amps=init, dtype=dtype)
+            st.timing(True)
+            st.execute(prog, gpu.exec_opts("windowed", relabel=relabel))
+            t = st.timing_read()
+            stats = st.last_stats()
+            outs[relabel] = st.to_host()
+            st.close()
+            assert "k_sweep" in t and "k_sweep_spec" not in t, t
+            assert stats["relabelled"] == int(relabel), stats
+        if dtype == "c128":
+            assert np.abs(outs[True] - want).max() <= 1e-12
+            assert np.abs(outs[True] - outs[False]).max() <= 1e-13
+        else:
+            inf = 1 - abs(np.vdot(want, outs[True])) ** 2 / np.vdot(outs[True], outs[True]).real
+            assert inf <= 1e-6, inf
