@@ -731,19 +731,24 @@ def main():
     bytes_step = stats["algorithmic_bytes"]
     gbps = bytes_step / (ms / 1e3) / 1e9
     dom = max(timing.items(), key=lambda kv: kv[1][1])
-    generic = None
+    generic = generic_in_place = None
     if opts.specialize:
-        # the same circuit through the generic (descriptor-interpreting) sweep kernel
-        og = qlx.exec_opts(mode, window_qubits=args.window, register_qubits=args.regs)
-        try:
+        # the same circuit through the generic (descriptor-interpreting) sweep kernel: out of
+        # place between the two buffers (what the drop-in default runs until its sweeps are
+        # compiled), and in place
+        def time_generic(relabel):
+            og = qlx.exec_opts(mode, window_qubits=args.window, register_qubits=args.regs, relabel=relabel)
             st.execute(prog, og)
             st.sync()
             tg = time.perf_counter()
             for _ in range(max(1, args.steps // 2)):
                 st.execute(prog, og)
             st.sync()
-            generic = (time.perf_counter() - tg) / max(1, args.steps // 2) * 1e3
-            log(f"bench: generic sweep kernel {generic:.2f} ms/step")
+            return (time.perf_counter() - tg) / max(1, args.steps // 2) * 1e3
+        try:
+            generic = time_generic((opts.specialize & qlx._abi.QLX_SPECIALIZE_RELABEL) != 0)
+            generic_in_place = time_generic(False)
+            log(f"bench: generic sweep kernel {generic:.2f} ms/step (in place {generic_in_place:.2f})")
         except qlx.QlxUnsupported as e:      # a window only the specialised sweeps take
             log(f"bench: generic sweep kernel: {e}")
     # the same circuit after pass_gate_fusion with width 5 (the reference's pass, extended to RX),
@@ -797,6 +802,7 @@ def main():
             "dropin_default": dropin,
             "dropin_ms": dropin["ms_per_step"] if dropin else None,
             "generic_kernel_ms_per_step": round(generic, 3) if generic else None,
+            "generic_kernel_in_place_ms_per_step": round(generic_in_place, 3) if generic_in_place else None,
             "fused_width5_ms_per_step": round(fused_ms, 3),
             "fused_width5_kernels_ms": {k: round(v[1] / max(1, v[0]), 4) for k, v in fused_timing.items()},
             "fused_width5_note": f"pass_gate_fusion (width 5, rotations fused) gives {len(fused)} instructions; "
