@@ -1365,15 +1365,15 @@ std::unique_ptr<SweepPlan> build_plan_geo(uint32_t nq, const Geometry& geo, cons
     Geometry fixed = geo;
     fixed.free = false;
     if (!thorough) {
-        auto p = build_plan_with(nq, fixed, instrs, n, pool, meta, opts, Chooser::Wide, nullptr, false);
-        if (geo.free && p->sweeps.size() > 1) {
+        // planned once (served and batched circuits): free windows when the walk can close on
+        // qubits 0..C-1, which it nearly always can, else windows that hold them
+        if (geo.free) {
             try {
-                auto f = build_plan_with(nq, geo, instrs, n, pool, meta, opts, Chooser::Wide, nullptr, false);
-                if (f->passes < p->passes - 1e-9) p = std::move(f);
+                return build_plan_with(nq, geo, instrs, n, pool, meta, opts, Chooser::Wide, nullptr, false);
             } catch (const Error&) {
             }
         }
-        return p;
+        return build_plan_with(nq, fixed, instrs, n, pool, meta, opts, Chooser::Wide, nullptr, false);
     }
     auto best = build_plan_with(nq, fixed, instrs, n, pool, meta, opts, Chooser::Greedy, nullptr, true);
     if (best->sweeps.size() <= 1) return best;
