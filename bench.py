@@ -215,7 +215,8 @@ def batch_summary(qlx, n=4096, steps=3, warmup=1, check=64):
     t0 = time.perf_counter()
     tune = qb.autotune(progs, "c64", sample=min(256, len(progs)))
     tune_s = time.perf_counter() - t0
-    b = qlx.Batch(qb.prepare_programs(progs, tune["fusion_width"]), "c64", window_qubits=tune["window_qubits"])
+    b = qlx.Batch(qb.prepare_programs(progs, tune["fusion_width"]), "c64", window_qubits=tune["window_qubits"],
+                  out_of_place=tune["out_of_place"])
     res = {}
     for mode, k in (("persistent", steps), ("levels", steps), ("per_circuit", 1)):
         for _ in range(warmup):
@@ -246,7 +247,7 @@ def batch_summary(qlx, n=4096, steps=3, warmup=1, check=64):
         "circuits_per_s": res[best]["circuits_per_s"], "mode": best, "modes": res,
         "speedup_vs_per_circuit": round(res[best]["circuits_per_s"] / res["per_circuit"]["circuits_per_s"], 1),
         "autotune": {"fusion_width": tune["fusion_width"], "window_qubits": tune["window_qubits"],
-                     "seconds": round(tune_s, 1)},
+                     "out_of_place": tune["out_of_place"], "seconds": round(tune_s, 1)},
         "checked": len(idx), "max_infidelity_vs_c128": worst, "infidelity_ok": bool(worst <= 1e-6),
     }
 
@@ -514,11 +515,12 @@ def run_batch(args):
     t0 = time.perf_counter()
     tune = qb.autotune(progs, "c64", sample=min(256, len(progs)))
     tune_s = time.perf_counter() - t0
-    log(f"batch: autotune {tune_s:.1f} s: fusion width {tune['fusion_width']}, window {tune['window_qubits']}; "
+    log(f"batch: autotune {tune_s:.1f} s: fusion width {tune['fusion_width']}, window {tune['window_qubits']}, "
+        f"out of place {tune['out_of_place']}; "
         f"trials {tune['trials']}")
     run_progs = qb.prepare_programs(progs, tune["fusion_width"])
     t0 = time.perf_counter()
-    b = qlx.Batch(run_progs, "c64", window_qubits=tune["window_qubits"])
+    b = qlx.Batch(run_progs, "c64", window_qubits=tune["window_qubits"], out_of_place=tune["out_of_place"])
     prep_s = time.perf_counter() - t0
     log(f"batch: {len(progs)} circuits, {gates} gates, info {b.info}, prepare {prep_s:.2f} s")
     res = {}
@@ -560,7 +562,8 @@ def run_batch(args):
         "config": {"workload": f"{len(progs)} layered h/rx/cx circuits, N~U[10,20], D~U[5,40], complex64",
                    "gates": gates, "mode": best_mode, "modes": res,
                    "autotune": {"fusion_width": tune["fusion_width"], "window_qubits": tune["window_qubits"],
-                                "trials_ms": [[fw, w, round(ms, 3)] for fw, w, ms in tune["trials"]],
+                                "out_of_place": tune["out_of_place"],
+                                "trials_ms": [[fw, w, oop, round(ms, 3)] for fw, w, oop, ms in tune["trials"]],
                                 "sample": min(256, len(progs)), "seconds": round(tune_s, 1)},
                    "speedup_vs_per_circuit_plain": round(best / base, 2) if best and base else None,
                    "checked": len(list(check)), "max_infidelity_vs_c128": worst, "max_abs_vs_c128": worst_abs,

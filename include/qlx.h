@@ -388,7 +388,11 @@ int qlx_batch_add(qlx_batch* b, uint32_t num_qubits, const qlx_gate_instr* instr
                   const double* pool_c128, size_t pool_elems, const qlx_matrix_entry* meta, size_t n_meta,
                   uint32_t* circuit_id);
 /* Plans every circuit, uploads descriptors, allocates the arena. opts: window/register
- * qubits for circuits at least one window wide (smaller circuits use their own size). */
+ * qubits for circuits at least one window wide (smaller circuits use their own size);
+ * specialize = QLX_SPECIALIZE_RELABEL: circuits larger than one window get two state buffers
+ * and run their sweeps out of place (windows free of the low qubits unless plan_flags has
+ * QLX_PLAN_FIXED_WINDOWS); other specialize flags are ignored (batched sweeps always run the
+ * generic kernel). */
 int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* info);
 /* Enqueues one run of every circuit from |0...0> (asynchronous). */
 int qlx_batch_run(qlx_batch* b, int mode);

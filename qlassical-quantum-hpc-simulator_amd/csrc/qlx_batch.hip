@@ -15,6 +15,9 @@
 // (kernel boundaries order the levels; chunks of a level run in parallel).
 // QLX_BATCH_PER_CIRCUIT(_GRAPH): the baseline — each circuit's gates as
 // one-kernel-per-gate launches (optionally captured once into a hipGraph).
+// With QLX_SPECIALIZE_RELABEL in the prepare options, a circuit larger than one window gets two
+// state buffers and its sweeps run out of place (relabelled layouts, windows free of the low
+// qubits: fewer sweeps); after a sweep mode its result is in the buffer its last sweep wrote.
 
 #include <algorithm>
 #include <cstring>
@@ -27,10 +30,12 @@ namespace qlx {
 
 struct BatchCircuitDev {
     uint64_t amp_off;      // element offset in the arena
+    uint64_t alt_off;      // out of place: the second buffer (== amp_off in place)
     uint32_t sweep_begin;  // first DevSweep of this circuit
     uint32_t n_sweeps;
     uint32_t rb;           // register qubits of this circuit's geometry
     uint32_t log_chunks;   // chunks per sweep = 2^log_chunks
+    uint32_t oop, pad;     // out of place: sweep k reads buffer k % 2 and writes the other
 };
 
 struct BatchTaskDev {
@@ -42,20 +47,22 @@ __device__ __forceinline__ void run_task_chunk(A* arena, const BatchCircuitDev& 
                                                const DevSweep* sw, const DevPhase* ph, const DevOp* ops,
                                                const double* coef, A* lds) {
     const DevSweep* s = sw + c.sweep_begin + sweep;
-    A* amp = arena + c.amp_off;
+    const bool odd = (sweep & 1u) != 0, oop = c.oop != 0;
+    A* src = arena + (odd ? c.alt_off : c.amp_off);
+    A* dst = oop ? arena + (odd ? c.amp_off : c.alt_off) : src;
     const bool init = sweep == 0;
     if constexpr (sizeof(A) == 8) {
         switch (c.rb) {
-            case 2: run_chunk<A, 8, 2>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
-            case 3: run_chunk<A, 8, 3>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
-            case 4: run_chunk<A, 8, 4>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
-            default: run_chunk<A, 8, 5>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
+            case 2: run_chunk<A, 8, 2>(src, dst, oop, s, chunk, ph, ops, coef, lds, init); break;
+            case 3: run_chunk<A, 8, 3>(src, dst, oop, s, chunk, ph, ops, coef, lds, init); break;
+            case 4: run_chunk<A, 8, 4>(src, dst, oop, s, chunk, ph, ops, coef, lds, init); break;
+            default: run_chunk<A, 8, 5>(src, dst, oop, s, chunk, ph, ops, coef, lds, init); break;
         }
     } else {
         switch (c.rb) {
-            case 2: run_chunk<A, 8, 2>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
-            case 3: run_chunk<A, 8, 3>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
-            default: run_chunk<A, 8, 4>(amp, amp, false, s, chunk, ph, ops, coef, lds, init); break;
+            case 2: run_chunk<A, 8, 2>(src, dst, oop, s, chunk, ph, ops, coef, lds, init); break;
+            case 3: run_chunk<A, 8, 3>(src, dst, oop, s, chunk, ph, ops, coef, lds, init); break;
+            default: run_chunk<A, 8, 4>(src, dst, oop, s, chunk, ph, ops, coef, lds, init); break;
         }
     }
 }
@@ -115,6 +122,8 @@ struct qlx_batch {
         std::vector<double> pool;
         std::vector<qlx_matrix_entry> meta;
         uint64_t amp_off = 0;
+        uint64_t alt_off = 0;     // out of place: the second buffer (== amp_off in place)
+        uint64_t result_off = 0;  // after a sweep mode: the buffer the last sweep wrote
         uint64_t pool_off = 0;  // complex elements into the device pool
         std::unique_ptr<SweepPlan> plan;
     };
@@ -132,6 +141,7 @@ struct qlx_batch {
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     std::vector<qlx_state> views;  // non-owning per-circuit states over the arena (plain modes)
+    int last_mode = -1;            // the last qlx_batch_run mode (where the results are)
 };
 
 namespace {
@@ -204,6 +214,14 @@ void launch_levels(qlx_batch* b) {
     }
 }
 
+// A circuit's result: the buffer its last sweep wrote after a sweep mode, its first buffer
+// after a per-circuit mode (in place).
+const void* result_ptr(const qlx_batch* b, uint32_t id) {
+    const bool sweeps = b->last_mode == QLX_BATCH_PERSISTENT || b->last_mode == QLX_BATCH_LEVELS;
+    const uint64_t off = sweeps ? b->circs[id].result_off : b->circs[id].amp_off;
+    return (const char*)b->arena.ptr + off * amp_size(b->dtype);
+}
+
 void launch_per_circuit(qlx_batch* b) {
     qlx_exec_opts o{};
     qlx_exec_stats stats{};
@@ -269,7 +287,9 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
         if (b->circs.empty()) fail(QLX_EINVAL, "empty batch");
         qlx_exec_opts o{};
         if (opts) o = *opts;
-        o.specialize = 0;  // batched sweeps run the generic kernel's descriptors
+        // batched sweeps run the generic kernel's descriptors, out of place on request
+        const bool want_oop = (o.specialize & QLX_SPECIALIZE_RELABEL) != 0;
+        o.specialize = 0;
         o.gswap_mode = QLX_GSWAP_PERMUTE;
         const size_t amp = amp_size(b->dtype);
         // plan every circuit, lay out the arena and the device pool
@@ -282,20 +302,35 @@ int qlx_batch_prepare(qlx_batch* b, const qlx_exec_opts* opts, qlx_batch_info* i
         uint32_t max_sweeps = 0;
         uint64_t gates = 0;
         for (auto& c : b->circs) {
-            const Geometry geo = batch_geometry(b->dtype, c.nq, o);
+            Geometry geo = batch_geometry(b->dtype, c.nq, o);
+            const bool oop = want_oop && (int)c.nq > geo.w;
+            qlx_exec_opts co = o;
+            if (oop) {
+                geo.free = !(o.plan_flags & QLX_PLAN_FIXED_WINDOWS);
+                co.specialize = QLX_SPECIALIZE_RELABEL;
+            }
             c.plan = build_plan_geo(c.nq, geo, c.instrs.data(), c.instrs.size(),
-                                    c.pool.empty() ? nullptr : c.pool.data(), c.meta.empty() ? nullptr : c.meta.data(), o,
+                                    c.pool.empty() ? nullptr : c.pool.data(), c.meta.empty() ? nullptr : c.meta.data(), co,
                                     /*thorough=*/false);
             if (b->dtype == QLX_C64) densify_phases(*c.plan);
+            if (oop) assign_layouts(*c.plan, c.nq);
             for (const PlanStep& s : c.plan->steps)
                 if (!s.sweep) fail(QLX_EUNSUPPORTED, "a batched circuit needs a kernel outside the window sweep");
             c.amp_off = amp_total;
             amp_total += 1ull << c.nq;
+            c.alt_off = c.amp_off;
+            if (oop) {
+                c.alt_off = amp_total;
+                amp_total += 1ull << c.nq;
+            }
+            c.result_off = (oop && (c.plan->dsw.size() & 1)) ? c.alt_off : c.amp_off;
             c.pool_off = pool_total;
             pool_total += c.pool.size() / 2;
             gates += c.instrs.size();
             BatchCircuitDev d{};
             d.amp_off = c.amp_off;
+            d.alt_off = c.alt_off;
+            d.oop = oop ? 1u : 0u;
             d.sweep_begin = (uint32_t)dsw.size();
             d.n_sweeps = (uint32_t)c.plan->dsw.size();
             d.rb = (uint32_t)geo.rb;
@@ -442,6 +477,7 @@ int qlx_batch_run(qlx_batch* b, int mode) {
             default: fail(QLX_EINVAL, "unknown batch mode");
         }
         QLX_HIP_CHECK(hipEventRecord(b->ev1, b->stream));
+        b->last_mode = mode;
     });
 }
 
@@ -466,11 +502,12 @@ int qlx_batch_state_to_host(qlx_batch* b, uint32_t id, double* out, size_t dim) 
         qlx_state& v = b->views[id];
         const size_t n = size_t(1) << v.nq;
         if (!out || dim < n) fail(QLX_EINVAL, "output buffer too small");
+        const void* res = result_ptr(b, id);
         if (b->dtype == QLX_C128) {
-            QLX_HIP_CHECK(hipMemcpy(out, v.amp, n * 16, hipMemcpyDeviceToHost));
+            QLX_HIP_CHECK(hipMemcpy(out, res, n * 16, hipMemcpyDeviceToHost));
         } else {
             std::vector<float> tmp(2 * n);
-            QLX_HIP_CHECK(hipMemcpy(tmp.data(), v.amp, n * 8, hipMemcpyDeviceToHost));
+            QLX_HIP_CHECK(hipMemcpy(tmp.data(), res, n * 8, hipMemcpyDeviceToHost));
             for (size_t i = 0; i < 2 * n; ++i) out[i] = tmp[i];
         }
     });
@@ -486,8 +523,8 @@ int qlx_batch_compare(qlx_batch* b, uint32_t id, qlx_state* ref, double* infidel
         QLX_HIP_CHECK(hipStreamSynchronize(b->stream));
         QLX_HIP_CHECK(hipStreamSynchronize(ref->stream));
         double re = 0, im = 0, m = 0, nb = 0;
-        compare_amplitudes(ref->amp, ref->dtype, v.amp, v.dtype, 1ull << v.nq, b->stream, b->scratch, &re, &im, &m,
-                           &nb);
+        compare_amplitudes(ref->amp, ref->dtype, result_ptr(b, id), v.dtype, 1ull << v.nq, b->stream, b->scratch, &re,
+                           &im, &m, &nb);
         if (infidelity) *infidelity = infidelity_from(re, im, reduce_norm_sq(ref), nb);
         if (max_abs) *max_abs = m;
     });

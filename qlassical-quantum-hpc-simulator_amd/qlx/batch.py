@@ -31,7 +31,10 @@ def batch_workload(count: int = 4096, qubits=(10, 20), depths=(5, 40), shape_see
 class Batch:
     """A set of circuits whose states live in one HBM arena."""
 
-    def __init__(self, programs: Iterable[Program], dtype="c64", window_qubits: int = 0, register_qubits: int = 0):
+    def __init__(self, programs: Iterable[Program], dtype="c64", window_qubits: int = 0, register_qubits: int = 0,
+                 out_of_place: bool = False):
+        """out_of_place: circuits larger than one window run their sweeps between two state
+        buffers (relabelled layouts, windows free of the low qubits: fewer sweeps)."""
         L = _abi.lib()
         h = C.c_void_p()
         _abi.check(L.qlx_batch_create(_dt(dtype), C.byref(h)))
@@ -48,7 +51,8 @@ class Batch:
                                            pool.ctypes.data if pool.size else None, pool.size,
                                            meta.ctypes.data if len(meta) else None, len(meta), C.byref(cid)))
                 self.programs.append(p)
-            o = exec_opts("windowed", window_qubits=window_qubits, register_qubits=register_qubits)
+            o = exec_opts("windowed", window_qubits=window_qubits, register_qubits=register_qubits,
+                          relabel=out_of_place)
             info = _abi.BatchInfo()
             _abi.check(L.qlx_batch_prepare(h, C.byref(o), C.byref(info)))
             self.info = {k: getattr(info, k) for k, _ in info._fields_}
@@ -87,11 +91,12 @@ class Batch:
 
 
 # ---- autotuning -------------------------------------------------------------------------
-# Candidate (fusion width, window qubits) pairs for complex64 batches. Fusion width 0 keeps
-# the circuits as they are; width 2 merges runs of 1- and 2-qubit gates into 4x4 matrices
-# (fewer, heavier in-register ops: the sweep kernel's per-op dispatch cost falls, its
-# arithmetic per op doubles). The window sets the chunk of circuits wider than it.
-TUNE_SPACE = [(0, 13), (2, 13), (0, 12), (2, 12), (0, 11), (2, 11)]
+# Candidate (fusion width, window qubits, out of place) configurations for complex64 batches.
+# Fusion width 0 keeps the circuits as they are; width 2 merges runs of 1- and 2-qubit gates
+# into 4x4 matrices (fewer, heavier in-register ops: the sweep kernel's per-op dispatch cost
+# falls, its arithmetic per op doubles). The window sets the chunk of circuits wider than it.
+# Out of place, circuits wider than the window take two buffers and fewer sweeps.
+TUNE_SPACE = [(fw, w, oop) for oop in (False, True) for w in (13, 12, 11) for fw in (0, 2)]
 
 
 def prepare_programs(programs: Iterable[Program], fusion_width: int) -> List[Program]:
@@ -104,17 +109,18 @@ def prepare_programs(programs: Iterable[Program], fusion_width: int) -> List[Pro
 
 def autotune(programs: List[Program], dtype="c64", sample: int = 256, reps: int = 3, seed: int = 7,
              space=TUNE_SPACE, mode: str = "persistent") -> dict:
-    """Picks the fusion width and window for a batch by timing a random sample of its circuits
-    in every candidate configuration (median of `reps` runs of one persistent launch).
-    Returns {"fusion_width", "window_qubits", "trials": [(fw, w, ms), ...]}."""
+    """Picks the fusion width, window and buffer form for a batch by timing a random sample of
+    its circuits in every candidate configuration (median of `reps` runs of one persistent
+    launch). Returns {"fusion_width", "window_qubits", "out_of_place",
+    "trials": [(fw, w, oop, ms), ...]}."""
     rng = np.random.default_rng(seed)
     idx = rng.choice(len(programs), size=min(sample, len(programs)), replace=False)
     base = [programs[i] for i in sorted(idx)]
     trials = []
-    for fw, w in space:
+    for fw, w, oop in space:
         progs = prepare_programs(base, fw)
         try:
-            b = Batch(progs, dtype, window_qubits=w)
+            b = Batch(progs, dtype, window_qubits=w, out_of_place=oop)
         except _abi.QlxUnsupported:
             continue
         try:
@@ -124,10 +130,10 @@ def autotune(programs: List[Program], dtype="c64", sample: int = 256, reps: int 
             for _ in range(reps):
                 b.run(mode)
                 times.append(b.sync())
-            trials.append((fw, w, float(np.median(times))))
+            trials.append((fw, w, oop, float(np.median(times))))
         finally:
             b.close()
     if not trials:
         raise _abi.QlxUnsupported("no batch configuration runs these circuits")
-    fw, w, _ = min(trials, key=lambda t: t[2])
-    return {"fusion_width": fw, "window_qubits": w, "trials": trials}
+    fw, w, oop, _ = min(trials, key=lambda t: t[3])
+    return {"fusion_width": fw, "window_qubits": w, "out_of_place": oop, "trials": trials}

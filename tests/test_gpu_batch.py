@@ -97,7 +97,8 @@ def test_batch_autotune_and_fused_batch(gpu):
     from qlx import batch as qb
     progs = workload(gpu, 20, (10, 15), (4, 10), 31)
     tune = qb.autotune(progs, "c64", sample=8, reps=1)
-    assert (tune["fusion_width"], tune["window_qubits"]) in [(fw, w) for fw, w, _ in tune["trials"]]
+    assert (tune["fusion_width"], tune["window_qubits"], tune["out_of_place"]) in [
+        (fw, w, oop) for fw, w, oop, _ in tune["trials"]]
     fused = qb.prepare_programs(progs, 2)
     assert any((p.meta["num_qubits"] == 2).any() for p in fused if len(p.meta))
     b = gpu.Batch(fused, "c64", window_qubits=12)
@@ -108,4 +109,41 @@ def test_batch_autotune_and_fused_batch(gpu):
         inf, mx = b.compare(i, ref)
         ref.close()
         assert inf <= 1e-6 and mx <= 1e-5, (i, inf, mx)
+    b.close()
+
+
+@pytest.mark.parametrize("mode", ["persistent", "levels"])
+def test_batch_out_of_place_matches_in_place_and_c128(gpu, mode):
+    """Circuits wider than one window run their sweeps between two buffers (relabelled layouts,
+    windows free of the low qubits): every state against the complex128 plain path and against
+    the in-place batch; after a per-circuit run the results are read from the first buffer."""
+    work = gpu.batch_workload(48, (10, 20), (5, 30), 4010, 401000)
+    progs = [gpu.layered(n, d, s) for n, d, s in work]
+    assert any(p.num_qubits > 14 for p in progs)
+    a = gpu.Batch(progs, "c64", out_of_place=True)
+    b = gpu.Batch(progs, "c64")
+    assert a.info["sweeps"] < b.info["sweeps"]
+    a.run(mode)
+    a.sync()
+    b.run(mode)
+    b.sync()
+    for i, p in enumerate(progs):
+        ref = gpu.State(p.num_qubits, "c128")
+        ref.execute(p)
+        ref.sync()
+        inf, mx = a.compare(i, ref)
+        ref.close()
+        assert inf <= 1e-6 and mx <= 1e-5, (i, inf)
+        x, y = a.state(i), b.state(i)
+        fid = abs(np.vdot(x, y)) ** 2 / (np.vdot(x, x).real * np.vdot(y, y).real)
+        assert 1.0 - fid <= 1e-6, i
+    a.run("per_circuit")
+    a.sync()
+    for i in (0, len(progs) - 1):
+        ref = gpu.State(progs[i].num_qubits, "c128")
+        ref.execute(progs[i])
+        ref.sync()
+        assert a.compare(i, ref)[0] <= 1e-6
+        ref.close()
+    a.close()
     b.close()
