@@ -111,12 +111,13 @@ def test_relabel_falls_back_in_place_when_the_second_buffer_does_not_fit(gpu):
     target = gpu.State(dtype="c128", amps=init)      # 1 GiB
     fillers = []
     try:
-        # take HBM until less than the target's size is free: its second buffer cannot be allocated
+        # take HBM until well under the target's size is free: its second buffer cannot be
+        # allocated (the reported free bytes can understate what an allocation finds)
         for _ in range(64):
             free = gpu.device_info(0)["hbm_free_bytes"]
-            if free < (7 << 27):                        # < 0.875 GiB free
+            if free < (3 << 27):                        # < 0.375 GiB free
                 break
-            k = int(np.floor(np.log2(max(free - (1 << 28), 1 << 26) / 8)))
+            k = int(np.floor(np.log2(max(free - (1 << 27), 1 << 24) / 8)))
             try:
                 fillers.append(gpu.State(k, "c64"))
             except gpu.QlxOutOfMemory:
@@ -124,7 +125,7 @@ def test_relabel_falls_back_in_place_when_the_second_buffer_does_not_fit(gpu):
                 fillers.append(gpu.State(k, "c64"))
         free = gpu.device_info(0)["hbm_free_bytes"]
         print(f"fallback test: {len(fillers)} filler states, {free / 2**20:.0f} MiB free", flush=True)
-        assert free < (1 << 30)
+        assert free < (1 << 29)
         target.execute(prog, gpu.exec_opts("windowed", specialize=True, relabel=True))
         target.sync()
         s = target.last_stats()
