@@ -373,14 +373,19 @@ __device__ __forceinline__ void compute_chunk(A (&v)[1 << RB], const DevSweep* _
     for (uint32_t p = 0; p < nph; ++p) {
         const DevPhase* P = ph + sw->phase_begin + p;
         if (P->remap) {
+            // the work-item and register parts of a local index are disjoint bit sets and the
+            // swizzle is linear over XOR on them: swz(lt | roff) = swz(lt) ^ swz(roff), the
+            // register part uniform (scalar arithmetic on the descriptor)
+            const uint32_t s0 = swz<ELEM>(lt);
 #pragma unroll
-            for (int i = 0; i < R; ++i) lds[swz<ELEM>(lt | cur->roff[i])] = v[i];
+            for (int i = 0; i < R; ++i) lds[s0 ^ swz<ELEM>(cur->roff[i])] = v[i];
             __syncthreads();
             lt = 0;
 #pragma unroll
             for (int b = 0; b < LOGT; ++b) lt |= ((t >> b) & 1u) << P->tlocal[b];
+            const uint32_t s1 = swz<ELEM>(lt);
 #pragma unroll
-            for (int i = 0; i < R; ++i) v[i] = lds[swz<ELEM>(lt | P->roff[i])];
+            for (int i = 0; i < R; ++i) v[i] = lds[s1 ^ swz<ELEM>(P->roff[i])];
             __syncthreads();
         }
         cur = P;
