@@ -308,3 +308,33 @@ def test_measure_collapse_vs_oracle_large(gpu):
     want, om = oracle.execute(n, p.instrs, amps=init, seed=2024)
     assert e.measurements() == om
     assert np.abs(e.state() - want).max() <= TOL
+
+
+@pytest.mark.parametrize("mode", ["auto", "generic_relabel", "specialised_relabel"])
+def test_measurement_between_out_of_place_segments(gpu, mode):
+    """MEASURE inside a 21-qubit layered circuit with the segments between measurements run out
+    of place (the state moves between the two buffers): outcomes equal the oracle's for the same
+    seed and the final state matches it to 1e-12."""
+    n = 21
+    base = gpu.layered(n, 8, 2101)
+    instrs = list(base.instrs)
+    cut = {len(instrs) // 3: 5, 2 * len(instrs) // 3: 17}
+    prog_instrs = []
+    for k, g in enumerate(instrs):
+        prog_instrs.append(g)
+        if k in cut:
+            m = gpu.Circuit(n).measure(cut[k]).release().instrs[0]
+            prog_instrs.append(m)
+    prog = gpu.Program(n, np.array(prog_instrs, dtype=gpu.GATE_DTYPE), np.zeros(0, complex),
+                       np.zeros(0, gpu.META_DTYPE))
+    init = bf.random_state(n, np.random.default_rng(2102))
+    opts = {"auto": gpu.exec_opts("auto"),
+            "generic_relabel": gpu.exec_opts("windowed", relabel=True),
+            "specialised_relabel": gpu.exec_opts("windowed", specialize=True, relabel=True)}[mode]
+    e = gpu.Engine(initial_state=init, seed=91, opts=opts)
+    e.execute(prog)
+    m = e.measurements()
+    got = e.state()
+    want, om = oracle.execute(n, prog.instrs, amps=init, seed=91)
+    assert m == om and len(m) == 2
+    assert np.abs(got - want).max() <= 1e-12
