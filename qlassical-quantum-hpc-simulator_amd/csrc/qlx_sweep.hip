@@ -822,10 +822,13 @@ static std::vector<std::vector<int>> beam_windows(const std::vector<PGate>& gate
 
 // Arithmetic a complex128 sweep may carry before it outlasts its HBM pass, in vector-unit work
 // units (one unit: a one-qubit rotation or H on every amplitude of the chunk; a CX counts a
-// quarter, its cost being the register phases it adds). Measured on the 30-qubit layered
-// circuit (12-qubit window, 16 registers): sweeps of up to ~85 units run at the streaming rate.
+// quarter, its cost being the register phases it adds). With windows holding the low qubits
+// (12-qubit window, 16 registers) sweeps of up to ~85 units ran at the streaming rate; with
+// free windows each sweep also carries more LDS re-layouts, and a budget of 55 measured faster
+// on the 30-qubit layered circuit (two kernel traces each: 65.1 and 66.4 ms against 66.6 and
+// 67.0 ms with 85; the heaviest sweeps 0 and 3 give gates to sweeps 4 and 5).
 // A complex64 pass moves half the bytes in the same instructions: half the budget.
-constexpr double kBalanceWork = 85.0;
+constexpr double kBalanceWork = 55.0;
 // The walk's cap on one sweep's arithmetic (same units): a sweep stops taking runnable gates,
 // in program order, once it holds this much, leaving later layers to later sweeps. Caps of
 // 90-130 units measured slower on the 30-qubit layered circuit (102.4-103.2 ms against
