@@ -48,6 +48,90 @@ uint32_t swz_host(uint32_t l, int elem) {
     return l ^ (((l >> sh) ^ (l >> (2 * sh)) ^ (l >> (3 * sh))) & m);
 }
 
+// The LDS slot of a local index during one re-layout: an invertible linear map over GF(2),
+// given by the slot bits of each local bit (col[j]); slot(l) = XOR of col[j] over the bits j of l.
+struct LdsMap {
+    std::vector<uint32_t> col;
+    uint32_t apply(uint32_t l) const {
+        uint32_t a = 0;
+        for (size_t j = 0; j < col.size(); ++j)
+            if ((l >> j) & 1u) a ^= col[j];
+        return a;
+    }
+};
+
+// rank over GF(2) of a set of bit vectors
+int gf2_rank(std::vector<uint32_t> v) {
+    int r = 0;
+    for (int bit = 31; bit >= 0; --bit) {
+        size_t piv = r;
+        while (piv < v.size() && !((v[piv] >> bit) & 1u)) ++piv;
+        if (piv == v.size()) continue;
+        std::swap(v[r], v[piv]);
+        for (size_t k = 0; k < v.size(); ++k)
+            if (k != (size_t)r && ((v[k] >> bit) & 1u)) v[k] ^= v[r];
+        ++r;
+    }
+    return r;
+}
+
+// Bank rules of 16-byte LDS accesses on gfx950, measured with the SQ_LDS_BANK_CONFLICT counter
+// over lane -> slot maps that are bit permutations: a ds_write_b128 serves the 8 lanes of each
+// lane-bit-0..2 group in one cycle when their slot bits 0-2 differ; a ds_read_b128 serves each
+// 32-lane half in two cycles when its lanes cover the 16 values of slot bits 0-3 and the 4
+// lanes of every quad (lane bits 0-1) differ in them. `w` holds the local-index vectors of
+// lane bits 0-2 of the writing layout, `r` those of lane bits 0-4 of the reading layout.
+bool lds_conflict_free(const LdsMap& m, const std::vector<uint32_t>& w, const std::vector<uint32_t>& r) {
+    std::vector<uint32_t> a;
+    for (uint32_t x : w) a.push_back(m.apply(x) & 7u);
+    if (gf2_rank(a) < 3) return false;
+    std::vector<uint32_t> b;
+    for (uint32_t x : r) b.push_back(m.apply(x) & 15u);
+    if (gf2_rank({b[0], b[1]}) < 2) return false;
+    return gf2_rank(b) == 4;
+}
+
+// A slot map for one re-layout of 16-byte elements (w local bits): the generic kernel's swizzle
+// when it is conflict-free for this write and read, otherwise one whose slot bits 0-3 are
+// parities chosen for these two layouts (a seeded search) and whose other slot bits are local
+// bits completing a basis.
+LdsMap choose_lds_map(int w, const std::vector<uint32_t>& wv, const std::vector<uint32_t>& rv, uint64_t seed) {
+    LdsMap m;
+    m.col.resize(w);
+    for (int j = 0; j < w; ++j) m.col[j] = swz_host(1u << j, 16);
+    if (lds_conflict_free(m, wv, rv)) return m;
+    uint64_t s = seed * 0x9e3779b97f4a7c15ull + 1;
+    auto next = [&]() {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        return s;
+    };
+    const uint32_t full = (1u << w) - 1;
+    for (int attempt = 0; attempt < 100000; ++attempt) {
+        uint32_t rho[4];
+        for (uint32_t& x : rho) x = (uint32_t)next() & full;
+        if (gf2_rank({rho[0], rho[1], rho[2], rho[3]}) < 4) continue;
+        // slot bits 4.. : the local bits that complete rho to a basis, ascending
+        std::vector<uint32_t> basis(rho, rho + 4);
+        std::vector<int> extra;
+        for (int j = 0; j < w && (int)basis.size() < w; ++j) {
+            basis.push_back(1u << j);
+            if (gf2_rank(basis) < (int)basis.size()) basis.pop_back();
+            else extra.push_back(j);
+        }
+        LdsMap c;
+        c.col.assign(w, 0u);
+        for (int j = 0; j < w; ++j) {
+            for (int k = 0; k < 4; ++k) c.col[j] |= ((rho[k] >> j) & 1u) << k;
+            for (size_t e = 0; e < extra.size(); ++e)
+                if (extra[e] == j) c.col[j] |= 1u << (4 + e);
+        }
+        if (lds_conflict_free(c, wv, rv)) return c;
+    }
+    return m;
+}
+
 struct Gen {
     const SweepPlan& p;
     int si;
@@ -122,6 +206,16 @@ struct Gen {
         std::string e = "0u";
         for (int b = 0; b < logt; ++b)
             e += " | (((" + tv + " >> " + std::to_string(b) + ") & 1u) << " + std::to_string(pos[b]) + ")";
+        return e;
+    }
+    // LDS slot of the work-item part of a layout: the XOR of the slots of the local bits its
+    // work-item bits hold (each work-item bit selects a constant by a sign-extended bit mask)
+    std::string thread_slot(const uint32_t* pos, const LdsMap& M) const {
+        std::string e = "0u";
+        for (int b = 0; b < logt; ++b) {
+            const uint32_t c = M.apply(1u << pos[b]);
+            if (c) e += " ^ ((0u - ((" + tv + " >> " + std::to_string(b) + ") & 1u)) & " + std::to_string(c) + "u)";
+        }
         return e;
     }
     std::string thread_bits64(const uint32_t* pos) const {
@@ -309,7 +403,6 @@ struct Gen {
     void emit() {
         const DevSweep& sw = p.dsw[si];
         const std::string T2 = f32 ? "float2" : "double2";
-        const int sh = elem == 16 ? 4 : 5;
         // the persistent form is not generated for out-of-place sweeps
         const bool persistent = jit_persistent(p.geo, f32 ? QLX_C64 : QLX_C128) && !p.relabel;
         // persistent kernels hold two register sets: ask for at least two waves per SIMD
@@ -355,30 +448,43 @@ struct Gen {
             const DevPhase& P = p.dph[sw.phase_begin + ph];
             if (P.remap) {
                 materialize_all_z();  // Z: sign flips in the registers; X: folded into the addresses
+                // the re-layout's LDS slot map: linear, so a local index's slot is the XOR of the
+                // slots of its work-item part, its register part and (frame) the flipped register bits
+                LdsMap M;
+                if (elem == 16 && logt >= 5) {
+                    std::vector<uint32_t> wv, rv;
+                    for (int b = 0; b < 3; ++b) {
+                        uint32_t u = 1u << cur->tlocal[b];
+                        for (int r = 0; r < rb; ++r)
+                            if ((xf[r] >> b) & 1u) u ^= cur->roff[1 << r];
+                        wv.push_back(u);
+                    }
+                    for (int b = 0; b < 5; ++b) rv.push_back(1u << P.tlocal[b]);
+                    M = choose_lds_map(w, wv, rv, (uint64_t)si * 1000003ull + ph);
+                } else {
+                    M.col.resize(w);
+                    for (int j = 0; j < w; ++j) M.col[j] = swz_host(1u << j, elem);
+                }
                 line("  {");
-                line("    const unsigned l0 = " + thread_bits(cur->tlocal) + ";");
-                line("    const unsigned s0 = l0 ^ (((l0 >> " + std::to_string(sh) + ") ^ (l0 >> " + std::to_string(2 * sh) +
-                     ") ^ (l0 >> " + std::to_string(3 * sh) + ")) & " + std::to_string((1 << sh) - 1) + "u);");
+                line("    const unsigned s0 = " + thread_slot(cur->tlocal, M) + ";");
                 std::string s0v = "s0";
                 if (frame_pending()) {
-                    // register i holds the amplitude of position i ^ f: write it there (the swizzle
-                    // is linear, so the frame adds a per-work-item XOR to every address)
+                    // register i holds the amplitude of position i ^ f: write it there (the slot
+                    // map is linear, so the frame adds a per-work-item XOR to every address)
                     frame_offset("unsigned", "fx", [&](int r) {
-                        return std::to_string(swz_host(cur->roff[1 << r], elem)) + "u";
+                        return std::to_string(M.apply(cur->roff[1 << r])) + "u";
                     });
                     line("    const unsigned s0f = s0 ^ fx;");
                     s0v = "s0f";
                     std::fill(xf.begin(), xf.end(), 0u);
                 }
                 for (int i = 0; i < R; ++i)
-                    line("    lds[" + s0v + " ^ " + std::to_string(swz_host(cur->roff[i], elem)) + "u] = " + var(i) + ";");
+                    line("    lds[" + s0v + " ^ " + std::to_string(M.apply(cur->roff[i])) + "u] = " + var(i) + ";");
                 line("    __syncthreads();");
-                line("    const unsigned l1 = " + thread_bits(P.tlocal) + ";");
-                line("    const unsigned s1 = l1 ^ (((l1 >> " + std::to_string(sh) + ") ^ (l1 >> " + std::to_string(2 * sh) +
-                     ") ^ (l1 >> " + std::to_string(3 * sh) + ")) & " + std::to_string((1 << sh) - 1) + "u);");
+                line("    const unsigned s1 = " + thread_slot(P.tlocal, M) + ";");
                 for (int i = 0; i < R; ++i) {
                     map[i] = i;
-                    line("    r" + std::to_string(i) + " = lds[s1 ^ " + std::to_string(swz_host(P.roff[i], elem)) + "u];");
+                    line("    r" + std::to_string(i) + " = lds[s1 ^ " + std::to_string(M.apply(P.roff[i])) + "u];");
                 }
                 line("    __syncthreads();");
                 line("  }");
