@@ -75,19 +75,21 @@ int gf2_rank(std::vector<uint32_t> v) {
     return r;
 }
 
-// Bank rules of 16-byte LDS accesses on gfx950, measured with the SQ_LDS_BANK_CONFLICT counter
-// over lane -> slot maps that are bit permutations: a ds_write_b128 serves the 8 lanes of each
-// lane-bit-0..2 group in one cycle when their slot bits 0-2 differ; a ds_read_b128 serves each
-// 32-lane half in two cycles when its lanes cover the 16 values of slot bits 0-3 and the 4
-// lanes of every quad (lane bits 0-1) differ in them. `w` holds the local-index vectors of
-// lane bits 0-2 of the writing layout, `r` those of lane bits 0-4 of the reading layout.
+// Bank rules of 16-byte LDS accesses on gfx950, fitted exactly to the SQ_LDS_BANK_CONFLICT
+// counter over 30 lane -> slot maps that permute address bits and the 52 re-layouts of the 30-qubit
+// layered circuit: a ds_write_b128 serves the 8 lanes of each lane-bit-0..2 group in one cycle
+// when their slot bits 0-2 differ; a ds_read_b128 serves each 32-lane half in two cycles, the
+// lanes whose lane bits 2, 3 and 4 have even parity in one and the others in the other, each
+// without conflicts when its 16 lanes differ in slot bits 0-3. `w` holds the local-index
+// vectors of lane bits 0-2 of the writing layout, `r` those of lane bits 0-4 of the reading one.
 bool lds_conflict_free(const LdsMap& m, const std::vector<uint32_t>& w, const std::vector<uint32_t>& r) {
     std::vector<uint32_t> a;
     for (uint32_t x : w) a.push_back(m.apply(x) & 7u);
     if (gf2_rank(a) < 3) return false;
+    // the even-parity lanes of a half: spanned by lane bits 0, 1, 2^3 and 3^4
+    const uint32_t e[4] = {r[0], r[1], r[2] ^ r[3], r[3] ^ r[4]};
     std::vector<uint32_t> b;
-    for (uint32_t x : r) b.push_back(m.apply(x) & 15u);
-    if (gf2_rank({b[0], b[1]}) < 2) return false;
+    for (uint32_t x : e) b.push_back(m.apply(x) & 15u);
     return gf2_rank(b) == 4;
 }
 
