@@ -52,6 +52,9 @@ uint32_t swz_host(uint32_t l, int elem) {
 // given by the slot bits of each local bit (col[j]); slot(l) = XOR of col[j] over the bits j of l.
 struct LdsMap {
     std::vector<uint32_t> col;
+    // the reading layout's register bits map to slot bits that its work-item bits never set, so
+    // a read's slot is the work-item part plus a constant (an immediate offset of the LDS read)
+    bool read_disjoint = false;
     uint32_t apply(uint32_t l) const {
         uint32_t a = 0;
         for (size_t j = 0; j < col.size(); ++j)
@@ -93,15 +96,14 @@ bool lds_conflict_free(const LdsMap& m, const std::vector<uint32_t>& w, const st
     return gf2_rank(b) == 4;
 }
 
-// A slot map for one re-layout of 16-byte elements (w local bits): the generic kernel's swizzle
-// when it is conflict-free for this write and read, otherwise one whose slot bits 0-3 are
-// parities chosen for these two layouts (a seeded search) and whose other slot bits are local
-// bits completing a basis.
-LdsMap choose_lds_map(int w, const std::vector<uint32_t>& wv, const std::vector<uint32_t>& rv, uint64_t seed) {
-    LdsMap m;
-    m.col.resize(w);
-    for (int j = 0; j < w; ++j) m.col[j] = swz_host(1u << j, 16);
-    if (lds_conflict_free(m, wv, rv)) return m;
+// A slot map for one re-layout of 16-byte elements (w local bits; `rregs`: the local bits the
+// reading layout keeps in registers). First choice (a seeded search): the reading layout's
+// register bits on slot bits 4.. of their own, its work-item bits mixed over the other slot
+// bits, conflict-free for this write and read, so the reads take immediate offsets. Otherwise
+// the generic kernel's swizzle when it is conflict-free, or one whose slot bits 0-3 are parities
+// found by a seeded search and whose other slot bits are local bits completing a basis.
+LdsMap choose_lds_map(int w, const std::vector<uint32_t>& wv, const std::vector<uint32_t>& rv, uint32_t rregs,
+                      uint64_t seed) {
     uint64_t s = seed * 0x9e3779b97f4a7c15ull + 1;
     auto next = [&]() {
         s ^= s << 13;
@@ -109,6 +111,39 @@ LdsMap choose_lds_map(int w, const std::vector<uint32_t>& wv, const std::vector<
         s ^= s << 17;
         return s;
     };
+    std::vector<int> rb, tb;  // register / work-item local bits of the reading layout
+    for (int j = 0; j < w; ++j) ((rregs >> j) & 1u ? rb : tb).push_back(j);
+    if (!rb.empty() && (int)rb.size() <= w - 4) {
+        for (int attempt = 0; attempt < 20000; ++attempt) {
+            // register bits -> distinct slot bits >= 4; work-item bits -> independent vectors over the rest
+            std::vector<int> hi;
+            for (int k = 4; k < w; ++k) hi.push_back(k);
+            for (size_t k = hi.size(); k > 1; --k) std::swap(hi[k - 1], hi[next() % k]);
+            uint32_t bmask = 0;
+            LdsMap c;
+            c.col.assign(w, 0u);
+            for (size_t k = 0; k < rb.size(); ++k) {
+                c.col[rb[k]] = 1u << hi[k];
+                bmask |= 1u << hi[k];
+            }
+            const uint32_t free = ((1u << w) - 1) & ~bmask;
+            std::vector<uint32_t> tv;
+            for (int j : tb) {
+                uint32_t v = 0;
+                while (!v) v = (uint32_t)next() & free;
+                c.col[j] = v;
+                tv.push_back(v);
+            }
+            if (gf2_rank(tv) < (int)tv.size()) continue;
+            if (!lds_conflict_free(c, wv, rv)) continue;
+            c.read_disjoint = true;
+            return c;
+        }
+    }
+    LdsMap m;
+    m.col.resize(w);
+    for (int j = 0; j < w; ++j) m.col[j] = swz_host(1u << j, 16);
+    if (lds_conflict_free(m, wv, rv)) return m;
     const uint32_t full = (1u << w) - 1;
     for (int attempt = 0; attempt < 100000; ++attempt) {
         uint32_t rho[4];
@@ -210,12 +245,13 @@ struct Gen {
             e += " | (((" + tv + " >> " + std::to_string(b) + ") & 1u) << " + std::to_string(pos[b]) + ")";
         return e;
     }
-    // LDS slot of the work-item part of a layout: the XOR of the slots of the local bits its
-    // work-item bits hold (each work-item bit selects a constant by a sign-extended bit mask)
+    // LDS byte address of the work-item part of a layout: the XOR of the slots (times the element
+    // size) of the local bits its work-item bits hold (each work-item bit selects a constant by a
+    // sign-extended bit mask)
     std::string thread_slot(const uint32_t* pos, const LdsMap& M) const {
         std::string e = "0u";
         for (int b = 0; b < logt; ++b) {
-            const uint32_t c = M.apply(1u << pos[b]);
+            const uint32_t c = M.apply(1u << pos[b]) * (uint32_t)elem;
             if (c) e += " ^ ((0u - ((" + tv + " >> " + std::to_string(b) + ") & 1u)) & " + std::to_string(c) + "u)";
         }
         return e;
@@ -416,6 +452,7 @@ struct Gen {
              (persistent ? " __attribute__((amdgpu_waves_per_eu(2)))" : "") + " " + jit_kernel_name(p, si) + "(" +
              params + ", unsigned long long nchunks) {");
         line("  __shared__ " + T2 + " lds[" + std::to_string(1 << w) + "];");
+        line("  char* const ldsb = reinterpret_cast<char*>(lds);");
         line("  const unsigned t = threadIdx.x;");
         line("  unsigned long long c = blockIdx.x;");
         line("  if (c >= nchunks) return;");
@@ -462,31 +499,37 @@ struct Gen {
                         wv.push_back(u);
                     }
                     for (int b = 0; b < 5; ++b) rv.push_back(1u << P.tlocal[b]);
-                    M = choose_lds_map(w, wv, rv, (uint64_t)si * 1000003ull + ph);
+                    uint32_t rregs = 0;
+                    for (int r = 0; r < rb; ++r) rregs |= P.roff[1 << r];
+                    M = choose_lds_map(w, wv, rv, rregs, (uint64_t)si * 1000003ull + ph);
                 } else {
                     M.col.resize(w);
                     for (int j = 0; j < w; ++j) M.col[j] = swz_host(1u << j, elem);
                 }
                 line("  {");
+                // byte addresses: slot x element size
                 line("    const unsigned s0 = " + thread_slot(cur->tlocal, M) + ";");
                 std::string s0v = "s0";
                 if (frame_pending()) {
                     // register i holds the amplitude of position i ^ f: write it there (the slot
                     // map is linear, so the frame adds a per-work-item XOR to every address)
                     frame_offset("unsigned", "fx", [&](int r) {
-                        return std::to_string(M.apply(cur->roff[1 << r])) + "u";
+                        return std::to_string(M.apply(cur->roff[1 << r]) * elem) + "u";
                     });
                     line("    const unsigned s0f = s0 ^ fx;");
                     s0v = "s0f";
                     std::fill(xf.begin(), xf.end(), 0u);
                 }
                 for (int i = 0; i < R; ++i)
-                    line("    lds[" + s0v + " ^ " + std::to_string(M.apply(cur->roff[i])) + "u] = " + var(i) + ";");
+                    line("    *reinterpret_cast<" + T2 + "*>(ldsb + (" + s0v + " ^ " +
+                         std::to_string(M.apply(cur->roff[i]) * elem) + "u)) = " + var(i) + ";");
                 line("    __syncthreads();");
                 line("    const unsigned s1 = " + thread_slot(P.tlocal, M) + ";");
                 for (int i = 0; i < R; ++i) {
                     map[i] = i;
-                    line("    r" + std::to_string(i) + " = lds[s1 ^ " + std::to_string(M.apply(P.roff[i])) + "u];");
+                    const std::string off = std::to_string(M.apply(P.roff[i]) * elem) + "u";
+                    line("    r" + std::to_string(i) + " = *reinterpret_cast<const " + T2 + "*>(ldsb + (s1 " +
+                         (M.read_disjoint ? "+ " : "^ ") + off + "));");
                 }
                 line("    __syncthreads();");
                 line("  }");
