@@ -47,7 +47,9 @@ def test_source_structure(dtype, regs):
         # gate coefficients are literals; no descriptor arrays are read
         assert "coef[" not in src and "ops[" not in src
         # every remap writes and reads all registers through LDS
-        assert src.count("lds[s0 ^") + src.count("lds[s0f ^") == regs * src.count("__syncthreads();") // 2
+        remaps = src.count("__syncthreads();") // 2
+        assert len(re.findall(r"ldsb \+ \(s0f? \^ \d+u\)\) = r\d+;", src)) == regs * remaps
+        assert len(re.findall(r"= \*reinterpret_cast<const \w+\*>\(ldsb \+ \(s1 [+^] \d+u\)\);", src)) == regs * remaps
 
 
 def test_cx_and_swap_are_register_renaming():
@@ -59,7 +61,7 @@ def test_cx_and_swap_are_register_renaming():
     assert sorted(int(v) for _, v in stores) == list(range(16))
     # the renamed registers show up where the chunk leaves the registers (an LDS re-layout or
     # the final store): some register index is written from another register's variable
-    writes = [int(v) for v in re.findall(r"lds\[s0 \^ \d+u\] = r(\d+);", src)] + [int(v) for _, v in stores]
+    writes = [int(v) for v in re.findall(r"ldsb \+ \(s0 \^ \d+u\)\) = r(\d+);", src)] + [int(v) for _, v in stores]
     assert any(v != i % 16 for i, v in enumerate(writes))
 
 
@@ -257,3 +259,82 @@ def test_relabelled_sources_compile_for_gfx950():
     for dtype in ("c128", "c64"):
         rc, log = compile_ok(qlx.sweep_source(prog, 0, dtype, o))
         assert rc == 0, log[-2000:]
+
+
+def _parity(x):
+    return bin(x).count("1") & 1
+
+
+def _relayouts(src):
+    """Per LDS re-layout of a generated complex128 sweep: the byte address every (work-item,
+    register) writes and reads, from the generated expressions."""
+    out, cur = [], None
+    terms = lambda ln: [(int(b), int(c)) for b, c in re.findall(r"\(t >> (\d+)\) & 1u\)\) & (\d+)u\)", ln)]
+    for ln in src.splitlines():
+        if "const unsigned s0 =" in ln:
+            cur = {"s0": terms(ln), "fx": [], "w": [], "r": []}
+            out.append(cur)
+        elif "const unsigned s1 =" in ln:
+            cur["s1"] = terms(ln)
+        elif cur is not None:
+            m = re.search(r"popcount\(t & (\d+)u\) & 1u\) != 0u\) fx \^= (\d+)u;", ln)
+            if m:
+                cur["fx"].append((int(m.group(1)), int(m.group(2))))
+            m = re.search(r"ldsb \+ \((s0f?) \^ (\d+)u\)\) = r\d+;", ln)
+            if m:
+                cur["w"].append((m.group(1) == "s0f", int(m.group(2))))
+            m = re.search(r"ldsb \+ \(s1 ([+^]) (\d+)u\)\);", ln)
+            if m:
+                cur["r"].append((m.group(1), int(m.group(2))))
+
+    def addr(ts, t):
+        a = 0
+        for b, c in ts:
+            if (t >> b) & 1:
+                a ^= c
+        return a
+
+    for rl in out:
+        w = [[addr(rl["s0"], t) for t in range(256)] for _ in rl["w"]]
+        for k, (f, c) in enumerate(rl["w"]):
+            for t in range(256):
+                a = w[k][t]
+                if f:
+                    for msk, cc in rl["fx"]:
+                        if _parity(t & msk):
+                            a ^= cc
+                w[k][t] = a ^ c
+        r = [[(addr(rl["s1"], t) + c) if op == "+" else (addr(rl["s1"], t) ^ c) for t in range(256)]
+             for op, c in rl["r"]]
+        yield w, r
+
+
+def test_relayout_slot_maps():
+    """Every LDS re-layout of the 30-qubit headline circuit's specialised sweeps writes each of the
+    chunk's 4096 16-byte slots exactly once and reads back the same slots, and obeys the bank rules
+    of 16-byte LDS accesses measured on gfx950 (writes: each group of 8 consecutive lanes differs
+    in slot bits 0-2; reads: in each 32-lane half, the 16 lanes of either parity of lane bits 2-4
+    differ in slot bits 0-3). Register offsets that the reads add (rather than XOR) never carry
+    into the work-item part."""
+    prog = qlx.layered(30, 20, 1002)
+    opts = qlx.exec_opts("windowed", specialize=True, relabel=True)
+    _, _, info = qlx.plan_windowed(prog, "c128", opts)
+    n = 0
+    for s in range(info["sweeps"]):
+        for w, r in _relayouts(qlx.sweep_source(prog, s, "c128", opts)):
+            n += 1
+            ws = sorted(a for col in w for a in col)
+            rs = sorted(a for col in r for a in col)
+            assert ws == [16 * i for i in range(4096)] and rs == ws
+            for col in w:
+                for wave in range(4):
+                    for g in range(0, 64, 8):
+                        grp = {(col[64 * wave + g + k] >> 4) & 7 for k in range(8)}
+                        assert len(grp) == 8
+            for col in r:
+                for wave in range(4):
+                    for h in (0, 32):
+                        for side in (0, 1):
+                            lanes = [L for L in range(h, h + 32) if _parity(L & 28) == side]
+                            assert len({(col[64 * wave + L] >> 4) & 15 for L in lanes}) == 16
+    assert n >= 40
