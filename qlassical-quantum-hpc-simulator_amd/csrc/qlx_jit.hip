@@ -256,6 +256,85 @@ struct Gen {
         }
         return e;
     }
+    // Complex128 out-of-place sweeps re-lay out in two rounds through half a chunk of LDS (32 KiB):
+    // slot bit k (>= 4, so the banks are those of the one-round map) picks the round, and the
+    // buffer holds the slots without bit k. Registers read in round 0 go to new variables, so the
+    // values that round 1 still writes stay intact. With 32 KiB per work-group the compute unit
+    // holds 3-4 work-groups instead of 2 (the sweeps use 117-168 VGPRs): on the 30-qubit layered
+    // circuit 65.1 -> 63.4 ms of sweep time (kernel traces on one box), every sweep equal or faster.
+    bool two_round_ok() const { return elem == 16 && logt == 8 && p.relabel; }
+    void emit_two_round(const LdsMap& M, const DevPhase& cur, const DevPhase& P) {
+        const std::string T2 = f32 ? "float2" : "double2";
+        // work-item columns (write side with the frame) and the round bit with the fewest of them
+        std::vector<uint32_t> wcol(logt), rcol(logt);
+        for (int b = 0; b < logt; ++b) {
+            uint32_t u = 1u << cur.tlocal[b];
+            for (int r = 0; r < rb; ++r)
+                if ((xf[r] >> b) & 1u) u ^= cur.roff[1 << r];
+            wcol[b] = M.apply(u);
+            rcol[b] = M.apply(1u << P.tlocal[b]);
+        }
+        int k = -1, best = 1 << 30;
+        for (int kk = 4; kk < w; ++kk) {
+            int cost = 0;
+            for (int b = 0; b < logt; ++b) {
+                const int wt = b >= 6 ? 1 : 100;  // lane-dependent rounds diverge
+                cost += (((wcol[b] >> kk) & 1u) + ((rcol[b] >> kk) & 1u)) * wt;
+            }
+            if (cost < best) { best = cost; k = kk; }
+        }
+        const uint32_t kb = (uint32_t)k + 4;  // in byte addresses (16-byte elements)
+        auto comp = [&](uint32_t x) { return ((x >> (kb + 1)) << kb) | (x & ((1u << kb) - 1)); };
+        line("  {");
+        line("    const unsigned s0 = " + thread_slot(cur.tlocal, M) + ";");
+        std::string s0v = "s0";
+        if (frame_pending()) {
+            frame_offset("unsigned", "fx", [&](int r) {
+                return std::to_string(M.apply(cur.roff[1 << r]) * elem) + "u";
+            });
+            line("    const unsigned s0f = s0 ^ fx;");
+            s0v = "s0f";
+            std::fill(xf.begin(), xf.end(), 0u);
+        }
+        const std::string K = std::to_string(kb), K1 = std::to_string(kb + 1);
+        line("    const unsigned rw = (" + s0v + " >> " + K + ") & 1u;");
+        line("    const unsigned a0 = ((" + s0v + " >> " + K1 + ") << " + K + ") | (" + s0v + " & ((1u << " + K + ") - 1u));");
+        line("    const unsigned s1 = " + thread_slot(P.tlocal, M) + ";");
+        line("    const unsigned rr = (s1 >> " + K + ") & 1u;");
+        line("    const unsigned a1 = ((s1 >> " + K1 + ") << " + K + ") | (s1 & ((1u << " + K + ") - 1u));");
+        std::string decl = "    " + T2;
+        for (int i = 0; i < R; ++i) decl += std::string(i ? "," : "") + " n" + std::to_string(i);
+        line(decl + ";");
+        for (int round = 0; round < 2; ++round) {
+            // writes: register i is in this round when rw ^ (bit k of its constant) == round
+            for (int side = 0; side < 2; ++side) {
+                line(std::string("    ") + (side == 0 ? "if" : "else") + (side == 0 ? " (rw == " + std::to_string(round) + "u) {" : " {"));
+                for (int i = 0; i < R; ++i) {
+                    const uint32_t c = M.apply(cur.roff[i]) * elem;
+                    if ((int)((c >> kb) & 1u) != side) continue;
+                    line("      *reinterpret_cast<" + T2 + "*>(ldsb + (a0 ^ " + std::to_string(comp(c)) + "u)) = " + var(i) + ";");
+                }
+                line("    }");
+            }
+            line("    __syncthreads();");
+            for (int side = 0; side < 2; ++side) {
+                line(std::string("    ") + (side == 0 ? "if" : "else") + (side == 0 ? " (rr == " + std::to_string(round) + "u) {" : " {"));
+                for (int i = 0; i < R; ++i) {
+                    const uint32_t c = M.apply(P.roff[i]) * elem;
+                    if ((int)((c >> kb) & 1u) != side) continue;
+                    line("      n" + std::to_string(i) + " = *reinterpret_cast<const " + T2 + "*>(ldsb + (a1 " +
+                         (M.read_disjoint ? "+ " : "^ ") + std::to_string(comp(c)) + "u));");
+                }
+                line("    }");
+            }
+            line("    __syncthreads();");
+        }
+        for (int i = 0; i < R; ++i) {
+            map[i] = i;
+            line("    r" + std::to_string(i) + " = n" + std::to_string(i) + ";");
+        }
+        line("  }");
+    }
     std::string thread_bits64(const uint32_t* pos) const {
         std::string e = "0ull";
         for (int b = 0; b < logt; ++b)
@@ -451,7 +530,8 @@ struct Gen {
         line("extern \"C\" __global__ void __launch_bounds__(" + std::to_string(1 << logt) + ")" +
              (persistent ? " __attribute__((amdgpu_waves_per_eu(2)))" : "") + " " + jit_kernel_name(p, si) + "(" +
              params + ", unsigned long long nchunks) {");
-        line("  __shared__ " + T2 + " lds[" + std::to_string(1 << w) + "];");
+        const bool two = two_round_ok();
+        line("  __shared__ " + T2 + " lds[" + std::to_string(1 << (two ? w - 1 : w)) + "];");
         line("  char* const ldsb = reinterpret_cast<char*>(lds);");
         line("  const unsigned t = threadIdx.x;");
         line("  unsigned long long c = blockIdx.x;");
@@ -505,6 +585,12 @@ struct Gen {
                 } else {
                     M.col.resize(w);
                     for (int j = 0; j < w; ++j) M.col[j] = swz_host(1u << j, elem);
+                }
+                if (two) {
+                    emit_two_round(M, *cur, P);
+                    cur = &P;
+                    for (uint32_t k = 0; k < P.n_ops; ++k) op(p.dop[P.op_begin + k]);
+                    continue;
                 }
                 line("  {");
                 // byte addresses: slot x element size

@@ -266,75 +266,106 @@ def _parity(x):
 
 
 def _relayouts(src):
-    """Per LDS re-layout of a generated complex128 sweep: the byte address every (work-item,
-    register) writes and reads, from the generated expressions."""
-    out, cur = [], None
+    """Per LDS re-layout of a generated out-of-place complex128 sweep (two rounds through half a
+    chunk of LDS): per round, the byte addresses written and read by every (work-item, register)
+    that takes part in it, from the generated expressions."""
     terms = lambda ln: [(int(b), int(c)) for b, c in re.findall(r"\(t >> (\d+)\) & 1u\)\) & (\d+)u\)", ln)]
+    rls, cur, block = [], None, None
     for ln in src.splitlines():
         if "const unsigned s0 =" in ln:
-            cur = {"s0": terms(ln), "fx": [], "w": [], "r": []}
-            out.append(cur)
-        elif "const unsigned s1 =" in ln:
+            cur = {"s0": terms(ln), "fx": [], "ops": []}
+            rls.append(cur)
+            continue
+        if cur is None:
+            continue
+        m = re.search(r"popcount\(t & (\d+)u\) & 1u\) != 0u\) fx \^= (\d+)u;", ln)
+        if m:
+            cur["fx"].append((int(m.group(1)), int(m.group(2))))
+        if "const unsigned s1 =" in ln:
             cur["s1"] = terms(ln)
-        elif cur is not None:
-            m = re.search(r"popcount\(t & (\d+)u\) & 1u\) != 0u\) fx \^= (\d+)u;", ln)
-            if m:
-                cur["fx"].append((int(m.group(1)), int(m.group(2))))
-            m = re.search(r"ldsb \+ \((s0f?) \^ (\d+)u\)\) = r\d+;", ln)
-            if m:
-                cur["w"].append((m.group(1) == "s0f", int(m.group(2))))
-            m = re.search(r"ldsb \+ \(s1 ([+^]) (\d+)u\)\);", ln)
-            if m:
-                cur["r"].append((m.group(1), int(m.group(2))))
+        m = re.search(r"const unsigned rw = \(s0f? >> (\d+)\) & 1u;", ln)
+        if m:
+            cur["k"] = int(m.group(1))
+        m = re.search(r"if \((rw|rr) == (\d)u\) \{", ln)
+        if m:
+            block = (m.group(1), int(m.group(2)), 0)
+        if ln.strip() == "else {":
+            block = (block[0], block[1], 1)
+        m = re.search(r"ldsb \+ \(a0 \^ (\d+)u\)\) = r\d+;", ln)
+        if m:
+            cur["ops"].append(("w", block[1], block[2], "^", int(m.group(1))))
+        m = re.search(r"= \*reinterpret_cast<const double2\*>\(ldsb \+ \(a1 ([+^]) (\d+)u\)\);", ln)
+        if m:
+            cur["ops"].append(("r", block[1], block[2], m.group(1), int(m.group(2))))
 
-    def addr(ts, t):
+    def xor_terms(ts, t):
         a = 0
         for b, c in ts:
             if (t >> b) & 1:
                 a ^= c
         return a
 
-    for rl in out:
-        w = [[addr(rl["s0"], t) for t in range(256)] for _ in rl["w"]]
-        for k, (f, c) in enumerate(rl["w"]):
-            for t in range(256):
-                a = w[k][t]
-                if f:
-                    for msk, cc in rl["fx"]:
-                        if _parity(t & msk):
-                            a ^= cc
-                w[k][t] = a ^ c
-        r = [[(addr(rl["s1"], t) + c) if op == "+" else (addr(rl["s1"], t) ^ c) for t in range(256)]
-             for op, c in rl["r"]]
-        yield w, r
+    for rl in rls:
+        k = rl["k"]
+        comp = lambda x: ((x >> (k + 1)) << k) | (x & ((1 << k) - 1))
+        s0 = []
+        for t in range(256):
+            a = xor_terms(rl["s0"], t)
+            for msk, cc in rl["fx"]:
+                if _parity(t & msk):
+                    a ^= cc
+            s0.append(a)
+        s1 = [xor_terms(rl["s1"], t) for t in range(256)]
+        rounds = []
+        for rnd in (0, 1):
+            ins = []   # (kind, {work-item: byte address})
+            for kind, r, side, op, c in rl["ops"]:
+                if r != rnd:
+                    continue
+                base = s0 if kind == "w" else s1
+                act = {}
+                for t in range(256):
+                    if (((base[t] >> k) & 1) == rnd) == (side == 0):
+                        b = comp(base[t])
+                        act[t] = (b + c) if op == "+" else (b ^ c)
+                ins.append((kind, act))
+            rounds.append(ins)
+        yield rounds
 
 
 def test_relayout_slot_maps():
-    """Every LDS re-layout of the 30-qubit headline circuit's specialised sweeps writes each of the
-    chunk's 4096 16-byte slots exactly once and reads back the same slots, and obeys the bank rules
-    of 16-byte LDS accesses measured on gfx950 (writes: each group of 8 consecutive lanes differs
-    in slot bits 0-2; reads: in each 32-lane half, the 16 lanes of either parity of lane bits 2-4
-    differ in slot bits 0-3). Register offsets that the reads add (rather than XOR) never carry
-    into the work-item part."""
+    """Every LDS re-layout of the 30-qubit headline circuit's specialised sweeps runs in two rounds
+    through half a chunk of LDS: in each round the work-items write 2048 distinct 16-byte slots of
+    the 32 KiB buffer and read back exactly those, every register taking part in one round. The
+    accesses obey the bank rules of 16-byte LDS accesses measured on gfx950 (writes: the active lanes
+    of each group of 8 consecutive lanes differ in slot bits 0-2; reads: in each 32-lane half, the
+    active lanes of either parity of lane bits 2-4 differ in slot bits 0-3). Register offsets that
+    the reads add (rather than XOR) never carry into the work-item part."""
     prog = qlx.layered(30, 20, 1002)
     opts = qlx.exec_opts("windowed", specialize=True, relabel=True)
     _, _, info = qlx.plan_windowed(prog, "c128", opts)
     n = 0
     for s in range(info["sweeps"]):
-        for w, r in _relayouts(qlx.sweep_source(prog, s, "c128", opts)):
+        for rounds in _relayouts(qlx.sweep_source(prog, s, "c128", opts)):
             n += 1
-            ws = sorted(a for col in w for a in col)
-            rs = sorted(a for col in r for a in col)
-            assert ws == [16 * i for i in range(4096)] and rs == ws
-            for col in w:
-                for wave in range(4):
-                    for g in range(0, 64, 8):
-                        grp = {(col[64 * wave + g + k] >> 4) & 7 for k in range(8)}
-                        assert len(grp) == 8
-            for col in r:
-                for wave in range(4):
-                    for h in (0, 32):
-                        for side in (0, 1):
-                            lanes = [L for L in range(h, h + 32) if _parity(L & 28) == side]
-                            assert len({(col[64 * wave + L] >> 4) & 15 for L in lanes}) == 16
+            nw = nr = 0
+            for ins in rounds:
+                ws = sorted(a for kind, act in ins if kind == "w" for a in act.values())
+                rs = sorted(a for kind, act in ins if kind == "r" for a in act.values())
+                assert ws == [16 * i for i in range(2048)] and rs == ws
+                nw += len(ws)
+                nr += len(rs)
+                for kind, act in ins:
+                    for wave in range(4):
+                        if kind == "w":
+                            groups = [range(g, g + 8) for g in range(0, 64, 8)]
+                            bits = 7
+                        else:
+                            groups = [[L for L in range(h, h + 32) if _parity(L & 28) == side]
+                                      for h in (0, 32) for side in (0, 1)]
+                            bits = 15
+                        for grp in groups:
+                            sl = [(act[64 * wave + L] >> 4) & bits for L in grp if 64 * wave + L in act]
+                            assert len(set(sl)) == len(sl)
+            assert nw == nr == 4096
     assert n >= 40
