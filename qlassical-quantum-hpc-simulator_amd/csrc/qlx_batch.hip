@@ -172,6 +172,7 @@ void destroy_batch(qlx_batch* b) {
 Geometry batch_geometry(int dtype, uint32_t nq, const qlx_exec_opts& o) {
     const int wmax = o.window_qubits ? (int)o.window_qubits : (dtype == QLX_C128 ? 12 : 13);
     Geometry g{};
+    g.elem = dtype == QLX_C128 ? 16 : 8;
     g.logt = 8;
     g.w = std::min<int>((int)nq, wmax);
     g.rb = g.w - g.logt;

@@ -141,6 +141,7 @@ inline size_t align_to(size_t x, size_t a) { return (x + a - 1) / a * a; }
 void plan_job(PlanJob& j) {
     qlx_exec_opts o{};
     Geometry geo{};
+    geo.elem = j.dtype == QLX_C128 ? 16 : 8;
     geo.logt = 8;
     geo.w = std::min<int>((int)j.nq, j.dtype == QLX_C64 ? 13 : 12);
     geo.rb = geo.w - 8;

@@ -1235,6 +1235,7 @@ static std::unique_ptr<SweepPlan> build_plan_with(uint32_t nq, const Geometry& g
                 for (int b = 0; b < geo.rb; ++b)
                     if ((i >> b) & 1) o |= 1u << ph.st2loc[geo.logt + b];
                 dp.roff[i] = o;
+                dp.rslot[i] = swz_bytes(o, geo.elem);
             }
             plan->dph.push_back(dp);
             for (const PlanOp& op : ph.ops) {

@@ -61,6 +61,10 @@ struct DevPhase {
     uint32_t roff[kMaxR];        // local-index contribution of register index i
     uint32_t tlocal[kMaxLogT];   // local bit held by work-item bit b
     uint32_t pad2[2];
+    // LDS byte offset of register index i's part of a local index: swz(roff[i]) x the element
+    // size (the swizzle is linear over XOR, so a re-layout address is the work-item part's
+    // offset XOR this; precomputed on the host instead of per register and phase in the kernel)
+    uint32_t rslot[kMaxR];
 };
 
 constexpr int kMaxOopChunkBits = 24;  // chunk-index bits of an out-of-place sweep (states of <= 24 + W qubits)
@@ -102,6 +106,13 @@ __device__ __forceinline__ uint32_t swz(uint32_t l) {
     constexpr int SH = ELEM == 16 ? 4 : 5;
     constexpr uint32_t M = (1u << SH) - 1;
     return l ^ (((l >> SH) ^ (l >> (2 * SH)) ^ (l >> (3 * SH))) & M);
+}
+
+// swz for a runtime element size (host side of the descriptors)
+inline uint32_t swz_bytes(uint32_t l, int elem) {
+    const int sh = elem == 16 ? 4 : 5;
+    const uint32_t m = (1u << sh) - 1;
+    return (l ^ (((l >> sh) ^ (l >> (2 * sh)) ^ (l >> (3 * sh))) & m)) * (uint32_t)elem;
 }
 
 template <int B, class A, int R>
@@ -376,16 +387,18 @@ __device__ __forceinline__ void compute_chunk(A (&v)[1 << RB], const DevSweep* _
             // the work-item and register parts of a local index are disjoint bit sets and the
             // swizzle is linear over XOR on them: swz(lt | roff) = swz(lt) ^ swz(roff), the
             // register part uniform (scalar arithmetic on the descriptor)
-            const uint32_t s0 = swz<ELEM>(lt);
+            // (byte addresses; the register parts come precomputed with the descriptor)
+            char* const ldsb = reinterpret_cast<char*>(lds);
+            const uint32_t s0 = swz<ELEM>(lt) * ELEM;
 #pragma unroll
-            for (int i = 0; i < R; ++i) lds[s0 ^ swz<ELEM>(cur->roff[i])] = v[i];
+            for (int i = 0; i < R; ++i) *reinterpret_cast<A*>(ldsb + (s0 ^ cur->rslot[i])) = v[i];
             __syncthreads();
             lt = 0;
 #pragma unroll
             for (int b = 0; b < LOGT; ++b) lt |= ((t >> b) & 1u) << P->tlocal[b];
-            const uint32_t s1 = swz<ELEM>(lt);
+            const uint32_t s1 = swz<ELEM>(lt) * ELEM;
 #pragma unroll
-            for (int i = 0; i < R; ++i) v[i] = lds[s1 ^ swz<ELEM>(P->roff[i])];
+            for (int i = 0; i < R; ++i) v[i] = *reinterpret_cast<const A*>(ldsb + (s1 ^ P->rslot[i]));
             __syncthreads();
         }
         cur = P;
